@@ -1,0 +1,518 @@
+// fx_api.cpp — the C ABI of libfxcorr_gpu (include/fxcorr_gpu.h): contexts,
+// jobs and per-subintegration processing on one gfx950 device.
+//
+// One subintegration runs as (all on the job's stream):
+//   k_frames    frame validity for every datastream
+//   k_prologue  FP64 block geometry for every (datastream, block)
+//   per chunk of blocks:
+//     k_fstage  spectra of every recorded band of every datastream
+//     k_xmac    accumulate V[a][b][c] for every XMAC frequency
+//   k_weights   baseline / autocorrelation weights
+//   k_scatter   V and weights into the core result layout
+#include "fx_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace fxg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) { g_err = msg; return code; }
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err(FXG_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+// device buffer that grows on demand
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes) return 0;
+    if (p) { hipFree(p); p = nullptr; bytes = 0; }
+    if (n == 0) return 0;
+    if (hipMalloc(&p, n) != hipSuccess) return set_err(FXG_ENOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
+    bytes = n;
+    return 0;
+  }
+  void release() { if (p) hipFree(p); p = nullptr; bytes = 0; }
+  template <typename T> T* as() const { return (T*)p; }
+};
+
+}  // namespace
+
+struct fxg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;   // for standalone calls
+  DBuf lut8;
+};
+
+struct fxg_job {
+  fxg_ctx* ctx = nullptr;
+  Plan plan;
+  fxg_job_desc desc{};
+  std::vector<fxg_freq> freqs;
+  std::vector<fxg_datastream> dss;
+  std::vector<fxg_baseline> bls;
+  hipStream_t stream = nullptr;
+  int keep_taps = 0;
+  int have_subint = 0;
+  float scale = 1.0f;             // spectrum scale for reduced precision
+  // device tables
+  DBuf d_st, d_bands, d_segs, d_blpairs, d_inoff, d_tw, d_lut8;
+  // per-subint
+  DBuf d_ss, d_flags, d_ftab, d_blocks, d_weights, d_results;
+  std::vector<DBuf> d_raw;        // staged host data per datastream
+  // big buffers
+  DBuf d_spec, d_V;
+  std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
+  // tap bookkeeping: which chunk is resident in d_spec
+  int tap_chunk0 = -1, tap_chunkn = 0;
+};
+
+extern "C" {
+
+int fxg_abi_version(void) { return FXG_ABI_VERSION; }
+const char* fxg_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// planning (host only)
+// ---------------------------------------------------------------------------
+int fxg_plan(const fxg_job_desc* desc, fxg_plan_info* info) {
+  if (!info) return set_err(FXG_EINVAL, "null info");
+  Plan p;
+  int rc = build_plan(desc, &p, &g_err);
+  if (rc) return rc;
+  *info = p.info;
+  return 0;
+}
+
+static int plan_lookup(const fxg_job_desc* desc, Plan* p) { return build_plan(desc, p, &g_err); }
+
+int fxg_plan_baseline_offset(const fxg_job_desc* desc, int freq, int baseline, int64_t* off) {
+  Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
+  if (!off || freq < 0 || freq >= p.nfreqs || baseline < 0 || baseline >= p.nbl) return set_err(FXG_EINVAL, "index");
+  *off = p.bloff[freq][baseline];
+  return 0;
+}
+int fxg_plan_bweight_offset(const fxg_job_desc* desc, int freq, int baseline, int64_t* off) {
+  Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
+  if (!off || freq < 0 || freq >= p.nfreqs || baseline < 0 || baseline >= p.nbl) return set_err(FXG_EINVAL, "index");
+  *off = p.bwoff[freq][baseline];
+  return 0;
+}
+int fxg_plan_autocorr_offset(const fxg_job_desc* desc, int ds, int64_t* off) {
+  Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
+  if (!off || ds < 0 || ds >= p.nds) return set_err(FXG_EINVAL, "index");
+  *off = p.acoff[ds];
+  return 0;
+}
+int fxg_plan_acweight_offset(const fxg_job_desc* desc, int ds, int64_t* off) {
+  Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
+  if (!off || ds < 0 || ds >= p.nds) return set_err(FXG_EINVAL, "index");
+  *off = p.acwoff[ds];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static void lut8_values(float* t) {
+  for (int b = 0; b < 256; b++) t[b] = (b - 128) / 3.3;   // format_vdif.c:126
+}
+
+int fxg_ctx_create(int device, fxg_ctx** out) {
+  if (!out) return set_err(FXG_EINVAL, "null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return set_err(FXG_ENODEV, "no HIP device");
+  if (device < 0 || device >= n) return set_err(FXG_ENODEV, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+    return set_err(FXG_ENODEV, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+  HIPCHK(hipSetDevice(device));
+  fxg_ctx* c = new fxg_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_err(FXG_EHIP, "stream creation failed");
+  }
+  float lut[256];
+  lut8_values(lut);
+  if (c->lut8.ensure(sizeof(lut))) { hipStreamDestroy(c->stream); delete c; return FXG_ENOMEM; }
+  HIPCHK(hipMemcpy(c->lut8.p, lut, sizeof(lut), hipMemcpyHostToDevice));
+  *out = c;
+  return 0;
+}
+
+int fxg_ctx_destroy(fxg_ctx* c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamDestroy(c->stream);
+  c->lut8.release();
+  delete c;
+  return 0;
+}
+
+int fxg_device_alloc(fxg_ctx* c, int64_t bytes, void** p) {
+  if (!c || !p || bytes < 0) return set_err(FXG_EINVAL, "bad args");
+  HIPCHK(hipSetDevice(c->device));
+  if (hipMalloc(p, (size_t)std::max<int64_t>(bytes, 1)) != hipSuccess) return set_err(FXG_ENOMEM, "hipMalloc failed");
+  return 0;
+}
+int fxg_device_free(fxg_ctx* c, void* p) {
+  if (!c) return set_err(FXG_EINVAL, "null ctx");
+  HIPCHK(hipSetDevice(c->device));
+  if (p) HIPCHK(hipFree(p));
+  return 0;
+}
+int fxg_memcpy_h2d(fxg_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0) return set_err(FXG_EINVAL, "bad args");
+  HIPCHK(hipSetDevice(c->device));
+  if (bytes) HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+int fxg_memcpy_d2h(fxg_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0) return set_err(FXG_EINVAL, "bad args");
+  HIPCHK(hipSetDevice(c->device));
+  if (bytes) HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// standalone unpack: frames -> floats for one datastream
+// ---------------------------------------------------------------------------
+int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_t nbytes,
+               int64_t sampleoffset, int32_t nsamp, float* out, float* weight) {
+  if (!c || !ds || !data || !out || nsamp <= 0 || sampleoffset < 0 || nbytes < 0)
+    return set_err(FXG_EINVAL, "bad args");
+  // a one-station, one-freq job description gives the decoder constants
+  fxg_freq fq{};
+  fq.freq_mhz = 1000.0; fq.bw_mhz = 16.0; fq.nchan = 64; fq.chans_to_avg = 1;
+  fxg_datastream d = *ds;
+  for (int i = 0; i < d.nrecfreqs && i < FXG_MAX_RECFREQS; i++) d.recfreq_index[i] = 0;
+  fxg_job_desc jd{};
+  jd.nfreqs = 1; jd.ndatastreams = 1; jd.nbaselines = 0; jd.blockspersend = 1; jd.fringe_order = 1;
+  jd.freqs = &fq; jd.datastreams = &d; jd.baselines = nullptr;
+  Plan p;
+  int rc = build_plan(&jd, &p, &g_err);
+  if (rc) return rc;
+  DevStation S = p.st[0];
+  HIPCHK(hipSetDevice(c->device));
+  DBuf d_st, d_ss, d_ftab;
+  if ((rc = d_st.ensure(sizeof(DevStation))) || (rc = d_ss.ensure(sizeof(DevStationSub)))) return rc;
+  DevStationSub Q{};
+  Q.data = data; Q.nbytes = nbytes; Q.nframes = nbytes / S.framebytes; Q.fbase = 0;
+  if ((rc = d_ftab.ensure(sizeof(int2) * std::max<long long>(1, Q.nframes)))) return rc;
+  HIPCHK(hipMemcpyAsync(d_st.p, &S, sizeof(S), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ss.p, &Q, sizeof(Q), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_frames(d_st.as<DevStation>(), d_ss.as<DevStationSub>(), d_ftab.as<int2>(), 1, Q.nframes, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // the window's validity is computed on the host from the frame table
+  std::vector<int2> ftab((size_t)std::max<long long>(1, Q.nframes));
+  if (Q.nframes) HIPCHK(hipMemcpy(ftab.data(), d_ftab.p, sizeof(int2) * Q.nframes, hipMemcpyDeviceToHost));
+  long long ns = sampleoffset, tend = sampleoffset + nsamp;
+  long long unpackstart = ns - ns % S.granularity;
+  long long validend = 0x7fffffffffffffffLL;
+  if (S.quirk8) {
+    for (long long f = unpackstart / S.framesamples; f <= (tend - 1) / S.framesamples; f++) {
+      bool fv = f < Q.nframes && ftab[f].x < ftab[f].y;
+      if (!fv) { validend = std::max(unpackstart, f * (long long)S.framesamples); break; }
+    }
+  }
+  long long good = 0;
+  for (long long t = ns; t < std::min(tend, validend);) {
+    long long f = t / S.framesamples;
+    long long fe = std::min(std::min(tend, validend), (f + 1) * (long long)S.framesamples);
+    if (f < Q.nframes) {
+      int2 z = ftab[f];
+      if (z.x < z.y) {
+        long long base = f * (long long)S.framesamples;
+        long long g0 = ((long long)z.x + S.groupbytes - 1) / S.groupbytes;
+        long long g1 = ((long long)z.y + S.groupbytes - 1) / S.groupbytes;
+        long long a = std::max(t - base, g0 * S.groupsamples), b = std::min(fe - base, g1 * S.groupsamples);
+        if (b > a) good += b - a;
+      }
+    }
+    t = fe;
+  }
+  HIPCHK(launch_unpack(d_st.as<DevStation>(), d_ss.as<DevStationSub>(), d_ftab.as<int2>(), c->lut8.as<float>(),
+                       ns, nsamp, S.nchan, validend, out, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (weight) *weight = (float)good / (float)nsamp;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// jobs
+// ---------------------------------------------------------------------------
+static void job_free(fxg_job* j) {
+  DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
+                  &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
+                  &j->d_spec, &j->d_V};
+  for (DBuf* b : bufs) b->release();
+  for (DBuf& b : j->d_raw) b.release();
+  if (j->stream) hipStreamDestroy(j->stream);
+}
+
+int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
+  if (!c || !desc || !out) return set_err(FXG_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  fxg_job* j = new fxg_job();
+  j->ctx = c;
+  int rc = build_plan(desc, &j->plan, &g_err);
+  if (rc) { delete j; return rc; }
+  Plan& P = j->plan;
+  // own copies of the descriptor tables
+  j->desc = *desc;
+  j->freqs.assign(desc->freqs, desc->freqs + desc->nfreqs);
+  j->dss.assign(desc->datastreams, desc->datastreams + desc->ndatastreams);
+  if (desc->nbaselines) j->bls.assign(desc->baselines, desc->baselines + desc->nbaselines);
+  j->desc.freqs = j->freqs.data();
+  j->desc.datastreams = j->dss.data();
+  j->desc.baselines = j->bls.empty() ? nullptr : j->bls.data();
+  auto fail = [&](int code) { job_free(j); delete j; return code; };
+  if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(FXG_EHIP, "stream"));
+
+  const int prec = desc->xmac_precision;
+  // spectrum scale for reduced precision: 2^-round(log2(sqrt(L)))
+  j->scale = prec == FXG_XMAC_FP32 ? 1.0f : std::ldexp(1.0f, -(int)std::lround(0.5 * std::log2((double)P.L)));
+
+  // spectra offsets: [global band][chunk block][C]
+  const long long chunk = P.chunk_blocks;
+  for (size_t g = 0; g < P.bands.size(); g++) P.bands[g].specoff = (long long)g * chunk * P.C;
+  std::vector<long long> inoff;
+  for (XmacFreq& x : P.xf) {
+    j->inoff_base.push_back((long long)inoff.size());
+    for (size_t i = 0; i < x.in_station.size(); i++) {
+      int gband = P.st[x.in_station[i]].band0 + x.in_band[i];
+      x.in_off[i] = P.bands[gband].specoff + x.in_off[i];
+      inoff.push_back(x.in_off[i]);
+    }
+  }
+  std::vector<int2> blp(std::max(1, P.nbl));
+  for (int b = 0; b < P.nbl; b++) blp[b] = make_int2(desc->baselines[b].ds1, desc->baselines[b].ds2);
+  // twiddles e^{-2 pi i k/L} in double, rounded once
+  std::vector<float2> tw(P.L);
+  for (int k = 0; k < P.L; k++) {
+    double ang = -kTwoPi * (double)k / (double)P.L;
+    tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  size_t specbytes = (size_t)P.nbands_total * chunk * P.C * (prec == FXG_XMAC_FP32 ? 8 : 4);
+  if ((rc = j->d_st.ensure(sizeof(DevStation) * P.st.size())) ||
+      (rc = j->d_bands.ensure(sizeof(DevBand) * P.bands.size())) ||
+      (rc = j->d_segs.ensure(sizeof(DevSeg) * std::max<size_t>(1, P.segs.size()))) ||
+      (rc = j->d_blpairs.ensure(sizeof(int2) * blp.size())) ||
+      (rc = j->d_inoff.ensure(sizeof(long long) * std::max<size_t>(1, inoff.size()))) ||
+      (rc = j->d_tw.ensure(sizeof(float2) * tw.size())) ||
+      (rc = j->d_blocks.ensure(sizeof(DevBlock) * (size_t)P.nds * desc->blockspersend)) ||
+      (rc = j->d_weights.ensure(sizeof(float) * (size_t)(P.nbl + P.nds))) ||
+      (rc = j->d_results.ensure(sizeof(float) * 2 * (size_t)std::max<long long>(1, P.info.result_length))) ||
+      (rc = j->d_spec.ensure(specbytes)) ||
+      (rc = j->d_V.ensure(sizeof(float2) * (size_t)std::max<long long>(1, P.vsize))))
+    return fail(rc);
+  if (hipMemcpy(j->d_st.p, P.st.data(), sizeof(DevStation) * P.st.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(j->d_bands.p, P.bands.data(), sizeof(DevBand) * P.bands.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      (P.segs.size() && hipMemcpy(j->d_segs.p, P.segs.data(), sizeof(DevSeg) * P.segs.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(j->d_blpairs.p, blp.data(), sizeof(int2) * blp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      (inoff.size() && hipMemcpy(j->d_inoff.p, inoff.data(), sizeof(long long) * inoff.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(j->d_tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(FXG_EHIP, "table upload failed"));
+  j->d_raw.resize(P.nds);
+  *out = j;
+  return 0;
+}
+
+int fxg_job_destroy(fxg_job* j) {
+  if (!j) return 0;
+  hipSetDevice(j->ctx->device);
+  if (j->stream) hipStreamSynchronize(j->stream);
+  job_free(j);
+  delete j;
+  return 0;
+}
+
+int fxg_job_plan_info(const fxg_job* j, fxg_plan_info* info) {
+  if (!j || !info) return set_err(FXG_EINVAL, "null argument");
+  *info = j->plan.info;
+  return 0;
+}
+
+int fxg_job_set_keep_taps(fxg_job* j, int keep) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  j->keep_taps = keep;
+  return 0;
+}
+
+// XMAC tiling for nin inputs: CT channels per workgroup, 8x8 tiles
+static void xmac_shape(int nin, int nchan, XmacArgs* a) {
+  a->G = (nin + 7) / 8;
+  a->ntiles = a->G * (a->G + 1) / 2;
+  int ct = 16;
+  while (ct > 1 && ct * a->ntiles > 576) ct >>= 1;
+  while (ct > 1 && ct > nchan) ct >>= 1;
+  a->CT = ct;
+  int per_block = a->G * 8 * ct * (int)sizeof(float2);
+  int tb = std::max(1, 65536 / per_block);
+  a->TB = std::min(tb, 16);
+}
+
+int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                       int data_on_device, float* results, int results_on_device) {
+  if (!j || !sub || !st || !results) return set_err(FXG_EINVAL, "null argument");
+  Plan& P = j->plan;
+  const fxg_job_desc& D = j->desc;
+  hipStream_t s = j->stream;
+  HIPCHK(hipSetDevice(j->ctx->device));
+  int rc;
+  const int nblocks = D.blockspersend;
+  // ---- per-datastream subint table
+  std::vector<DevStationSub> ss(P.nds);
+  std::vector<int> flags;
+  long long fbase = 0, maxframes = 0;
+  for (int d = 0; d < P.nds; d++) {
+    const fxg_station_input& in = st[d];
+    DevStationSub& q = ss[d];
+    if (in.nbytes < 0 || (in.nbytes > 0 && !in.data)) return set_err(FXG_EINVAL, "station data");
+    const unsigned char* dp = in.data;
+    if (!data_on_device && in.nbytes > 0) {
+      if ((rc = j->d_raw[d].ensure((size_t)in.nbytes))) return rc;
+      HIPCHK(hipMemcpyAsync(j->d_raw[d].p, in.data, (size_t)in.nbytes, hipMemcpyHostToDevice, s));
+      dp = j->d_raw[d].as<unsigned char>();
+    }
+    q.data = dp;
+    q.nbytes = in.nbytes;
+    q.nframes = in.nbytes / P.st[d].framebytes;
+    q.fbase = fbase;
+    fbase += q.nframes;
+    maxframes = std::max(maxframes, q.nframes);
+    q.datasec = in.sec; q.datans = in.ns;
+    // Mode::setOffsets / setData guards (mode.cpp:1525-1573, 671)
+    q.usable = (sub->model_ok && in.sec > FXG_INVALID_SUBINT && in.scan == sub->scan && in.nbytes > 1) ? 1 : 0;
+    q.nflags = in.validflags ? in.nvalidflags : 0;
+    q.flagoff = (int)flags.size();
+    for (int k = 0; k < q.nflags; k++) flags.push_back(in.validflags[k]);
+    for (int k = 0; k < 3; k++) q.interp[k] = in.interpolator[k];
+  }
+  if ((rc = j->d_ss.ensure(sizeof(DevStationSub) * ss.size())) ||
+      (rc = j->d_flags.ensure(sizeof(int) * std::max<size_t>(1, flags.size()))) ||
+      (rc = j->d_ftab.ensure(sizeof(int2) * (size_t)std::max<long long>(1, fbase))))
+    return rc;
+  HIPCHK(hipMemcpyAsync(j->d_ss.p, ss.data(), sizeof(DevStationSub) * ss.size(), hipMemcpyHostToDevice, s));
+  if (!flags.empty()) HIPCHK(hipMemcpyAsync(j->d_flags.p, flags.data(), sizeof(int) * flags.size(), hipMemcpyHostToDevice, s));
+
+  HIPCHK(launch_frames(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ftab.as<int2>(), P.nds, maxframes, s));
+  DevSubint dsub{sub->sec, sub->ns, sub->scan_start_sec, nblocks};
+  HIPCHK(launch_prologue(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_flags.as<int>(),
+                         j->d_ftab.as<int2>(), dsub, j->d_blocks.as<DevBlock>(), P.nds, s));
+
+  // ---- F and X stages, chunk by chunk
+  const int prec = D.xmac_precision;
+  FStageArgs fa{};
+  fa.st = j->d_st.as<DevStation>();
+  fa.bands = j->d_bands.as<DevBand>();
+  fa.ss = j->d_ss.as<DevStationSub>();
+  fa.blocks = j->d_blocks.as<DevBlock>();
+  fa.ftab = j->d_ftab.as<int2>();
+  fa.lut8 = j->ctx->lut8.as<float>();
+  fa.tw = j->d_tw.as<float2>();
+  fa.spectra = j->d_spec.p;
+  fa.prec = prec;
+  fa.scale = j->scale;
+  fa.nblocks_subint = nblocks;
+  for (int c0 = 0; c0 < nblocks; c0 += P.chunk_blocks) {
+    int cn = std::min(P.chunk_blocks, nblocks - c0);
+    fa.chunk0 = c0;
+    HIPCHK(launch_fstage(P.L, fa, P.nbands_total, cn, s));
+    for (size_t x = 0; x < P.xf.size(); x++) {
+      const XmacFreq& xf = P.xf[x];
+      XmacArgs xa{};
+      xa.spec = j->d_spec.p;
+      xa.in_off = j->d_inoff.as<long long>() + j->inoff_base[x];
+      xa.block_stride = P.C;
+      xa.nin = (int)xf.in_station.size();
+      xa.nchan = xf.nchan;
+      xa.nblocks = cn;
+      xmac_shape(xa.nin, xa.nchan, &xa);
+      xa.V = j->d_V.as<float2>() + xf.voff;
+      xa.accumulate = c0 > 0;
+      HIPCHK(launch_xmac_f32(xa, prec, s));
+    }
+    j->tap_chunk0 = c0;
+    j->tap_chunkn = cn;
+  }
+  HIPCHK(launch_weights(j->d_blocks.as<DevBlock>(), nblocks, j->d_blpairs.as<int2>(), P.nbl, P.nds,
+                        j->d_weights.as<float>(), s));
+  const size_t resbytes = sizeof(float) * 2 * (size_t)P.info.result_length;
+  float* dres = results_on_device ? results : j->d_results.as<float>();
+  HIPCHK(hipMemsetAsync(dres, 0, resbytes, s));
+  float vscale = 1.0f / (j->scale * j->scale);
+  HIPCHK(launch_scatter(j->d_segs.as<DevSeg>(), (int)P.segs.size(), P.maxseglen, j->d_V.as<float2>(),
+                        j->d_weights.as<float>(), vscale, dres, s));
+  if (!results_on_device) HIPCHK(hipMemcpyAsync(results, dres, resbytes, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  j->have_subint = 1;
+  return 0;
+}
+
+int fxg_tap_block_info(fxg_job* j, int ds, int block, fxg_block_info* out) {
+  if (!j || !out) return set_err(FXG_EINVAL, "null argument");
+  if (!j->have_subint) return set_err(FXG_ESTATE, "no subintegration processed");
+  if (ds < 0 || ds >= j->plan.nds || block < 0 || block >= j->desc.blockspersend) return set_err(FXG_EINVAL, "index");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  DevBlock g;
+  HIPCHK(hipMemcpy(&g, j->d_blocks.as<DevBlock>() + (size_t)ds * j->desc.blockspersend + block, sizeof(g),
+                   hipMemcpyDeviceToHost));
+  std::memset(out, 0, sizeof(*out));
+  out->status = g.status; out->nearestsample = g.nearestsample; out->integerdelay = g.integerdelay;
+  out->weight = g.status == 0 ? g.weight : 0.0f; out->fracsampleerror = g.fse; out->intwalltime = g.intwall;
+  out->a = g.a; out->b = g.b; out->fracwalltime = g.fracwall;
+  return 0;
+}
+
+int fxg_tap_spectrum(fxg_job* j, int ds, int band, int block, float* out) {
+  if (!j || !out) return set_err(FXG_EINVAL, "null argument");
+  if (!j->have_subint) return set_err(FXG_ESTATE, "no subintegration processed");
+  const Plan& P = j->plan;
+  if (ds < 0 || ds >= P.nds || band < 0 || band >= j->dss[ds].nrecbands) return set_err(FXG_EINVAL, "index");
+  if (block < j->tap_chunk0 || block >= j->tap_chunk0 + j->tap_chunkn)
+    return set_err(FXG_ESTATE, "block is not in the resident chunk (the last chunk of the subintegration)");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  const DevBand& B = P.bands[P.st[ds].band0 + band];
+  long long off = B.specoff + (long long)(block - j->tap_chunk0) * P.C;
+  if (j->desc.xmac_precision == FXG_XMAC_FP32) {
+    HIPCHK(hipMemcpy(out, j->d_spec.as<float2>() + off, sizeof(float2) * P.C, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint16_t> h(2 * (size_t)P.C);
+    HIPCHK(hipMemcpy(h.data(), j->d_spec.as<uint32_t>() + off, 4 * (size_t)P.C, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); i++) {
+      float v;
+      if (j->desc.xmac_precision == FXG_XMAC_BF16) {
+        uint32_t u = (uint32_t)h[i] << 16; std::memcpy(&v, &u, 4);
+      } else {
+        // IEEE half -> float
+        uint32_t hv = h[i], sign = (hv >> 15) & 1, e = (hv >> 10) & 0x1f, m = hv & 0x3ff, u;
+        if (e == 0) { v = std::ldexp((float)m, -24); if (sign) v = -v; out[i] = v / j->scale; continue; }
+        if (e == 31) u = (sign << 31) | 0x7f800000u | (m << 13);
+        else u = (sign << 31) | ((e - 15 + 127) << 23) | (m << 13);
+        std::memcpy(&v, &u, 4);
+      }
+      out[i] = v / j->scale;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// fx_internal.h — structures shared by the host side (fx_api.cpp, fx_plan.cpp)
+// and the device kernels (fx_station.hip, fx_xmac.hip) of libfxcorr_gpu.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/fxcorr_gpu.h"
+
+namespace fxg {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;  // architecture.h.in:435
+constexpr double kTiny = 0.000000001;                         // mode.cpp:35
+constexpr float kHiMag = 3.3359f;                              // mark5_stream.h:75
+constexpr int kBig = 1 << 30;                                  // "valid to the end" byte index
+
+// ---- per datastream constants (Mode / Mk5Mode construction) --------------
+struct DevStation {
+  int format, nbits, nchan, framebytes, headerbytes, databytes, framesamples;
+  int granularity, groupbytes, groupsamples, quirk8, blanker;
+  int C, L, s, nfrsteps, nfracsteps;
+  int bpbnum, bpbden;
+  int fractionalLo, deltapol;
+  int band0;            // global index of recorded band 0 of this datastream
+  int pad;
+  double sampletime, bw;
+  float lut[4];         // 2-bit level table (VDIF or Mark5B order)
+};
+
+// ---- per recorded band (global band index) -------------------------------
+struct DevBand {
+  int station, band, localfreq, lsb;
+  int useB;             // apply the "B" frac-sample rotator (delta pol offsets)
+  int pad;
+  double lofreq, clock, clockdelta, looffset;
+  long long specoff;    // element offset of this band in the spectra buffer
+};
+
+// ---- per subintegration, per datastream ----------------------------------
+struct DevStationSub {
+  const unsigned char* data;
+  long long nbytes;
+  long long nframes;    // whole frames inside nbytes
+  long long fbase;      // offset of this station in the frame table
+  int datasec, datans, usable, nflags;
+  int flagoff;          // offset of this station's valid flags
+  int pad;
+  double interp[3];
+};
+
+// ---- per block, per datastream (Mode::process prologue result) -----------
+struct DevBlock {
+  int status, nearestsample, integerdelay, intwall;
+  float weight, fse;
+  long long validend;   // samples at or after this index decode as zero
+  double a, b, fracwall;
+};
+
+// ---- subint-wide scalars ---------------------------------------------------
+struct DevSubint {
+  int offsec, offns, scanstart, nblocks;
+};
+
+// ---- result scatter segments -----------------------------------------------
+enum SegKind { SEG_VIS = 0, SEG_FLOATS = 1 };
+struct DevSeg {
+  long long resoff;     // cf32 offset (SEG_VIS) or float offset (SEG_FLOATS)
+  long long voff;       // element offset into V (SEG_VIS) or into the weight table
+  int kind, nchan;
+  int conj;             // take conj(V[b][a]) instead of V[a][b]
+  int vstride;          // unused for floats
+};
+
+// ---- the XMAC description of one frequency --------------------------------
+struct XmacFreq {
+  int freq;                        // freq table index
+  int nchan;
+  std::vector<long long> in_off;   // spectra element offset (band offset + channel offset)
+  std::vector<int> in_station, in_band;  // datastream, recorded (parent) band
+  std::vector<int> in_total;       // datastream total band index (rec or zoom)
+  long long voff;                  // offset of V[nin][nin][nchan] in the V buffer
+};
+
+// ---- job plan (host side) --------------------------------------------------
+struct Plan {
+  fxg_plan_info info{};
+  int nfreqs = 0, nds = 0, nbl = 0;
+  std::vector<char> freqused, equivused;
+  std::vector<std::vector<long long>> bloff, bwoff;    // [freq][baseline]
+  std::vector<std::vector<int>> bllocal;               // [baseline][freq]
+  std::vector<long long> acoff, acwoff;
+  std::vector<int> arraystride;
+  int xmacstride = 0, rotatestride = 0, bandsperac = 1;
+  int C = 0, L = 0;                                    // common to all datastreams
+  int nbands_total = 0;                                // recorded bands over all datastreams
+  std::vector<DevStation> st;
+  std::vector<DevBand> bands;
+  std::vector<XmacFreq> xf;                            // XMAC frequencies
+  std::vector<int> xf_of_freq;                         // freq -> index in xf or -1
+  long long vsize = 0;                                 // complex elements of V
+  std::vector<DevSeg> segs;                            // result scatter table
+  int maxseglen = 0;
+  // weight table: [0, nbl) baseline weights, [nbl, nbl+nds) AC weights
+  int chunk_blocks = 0;
+};
+
+int build_plan(const fxg_job_desc* d, Plan* p, std::string* err);
+int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d);
+
+}  // namespace fxg

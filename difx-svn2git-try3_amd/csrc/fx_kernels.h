@@ -1,0 +1,48 @@
+// fx_kernels.h — kernel argument blocks and launchers of libfxcorr_gpu
+// (implemented in fx_station.hip and fx_xmac.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "fx_internal.h"
+
+namespace fxg {
+
+struct FStageArgs {
+  const DevStation* st;
+  const DevBand* bands;
+  const DevStationSub* ss;
+  const DevBlock* blocks;
+  const int2* ftab;
+  const float* lut8;
+  const float2* tw;           // e^{-2 pi i k / L}, k < L
+  void* spectra;
+  int prec;                   // FXG_XMAC_*
+  float scale;                // applied before reduced-precision stores
+  int nblocks_subint;
+  int chunk0;                 // first block of the chunk
+};
+
+struct XmacArgs {
+  const void* spec;
+  const long long* in_off;    // [nin] element offsets of the inputs' block 0
+  long long block_stride;     // elements between consecutive blocks of one input
+  int nin, nchan, nblocks;
+  int CT, TB, ntiles, G;      // channel tile, blocks per stage, tiles, 8-groups
+  float2* V;                  // [nin][nin][nchan]
+  int accumulate;
+};
+
+hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
+                         long long maxframes, hipStream_t stream);
+hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
+                           const int2* ftab, DevSubint sub, DevBlock* out, int nst, hipStream_t stream);
+hipError_t launch_fstage(int L, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
+hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const int2* ftab, const float* lut8,
+                         long long first, int nsamp, int nchan, long long validend, float* out,
+                         hipStream_t stream);
+hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream);
+hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
+                          hipStream_t stream);
+hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w,
+                          float vscale, float* res, hipStream_t stream);
+
+}  // namespace fxg

@@ -1,0 +1,433 @@
+// fx_plan.cpp — host-side job planning for libfxcorr_gpu: descriptor checks,
+// stride selection, the core result layout and the device-side tables.
+//
+// Result layout follows Configuration::populateResultLengths
+// (mpifxcorr/src/configuration.cpp:2212-2441) for one configuration with a
+// single phase centre and no pulsar binning:
+//   cross-correlations  [freq][baseline][polproduct][chan]   (cf32)
+//   baseline weights    [freq][baseline] ceil(npol/2) cf32 slots of floats
+//   autocorrelations    [datastream][pass][total band][chan]  (cf32)
+//   autocorr weights    [datastream] ceil(nbands*pass/2) cf32 slots of floats
+// Used frequencies are those named by any baseline; a frequency that is not
+// used but matches a used one (configuration.cpp:345-367) still gets its
+// autocorrelations.
+#include "fx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace fxg {
+
+namespace {
+
+// smallest divisor of n that is >= sqrt(n) (configuration.cpp:46-59)
+int stride_for(unsigned n) {
+  if (n <= 16) return (int)n;
+  unsigned g = (unsigned)std::sqrt(n + 0.1);
+  while (n % g) --g;
+  return (int)(n / g);
+}
+
+// divisor of the channel-count gcd closest to 150 (configuration.cpp:2464-2511)
+int xmac_stride_for(const fxg_job_desc* d) {
+  long g = 0;
+  for (int b = 0; b < d->nbaselines; b++)
+    for (int f = 0; f < d->baselines[b].nfreqs; f++) {
+      long n = d->freqs[d->baselines[b].freq_index[f]].nchan;
+      g = g ? std::gcd(g, n) : n;
+    }
+  int best = 1;
+  double err = 1e12;
+  for (long c = g; c > 0; --c) {
+    if (g % c) continue;
+    double e = std::fabs((double)c - 150.0);
+    if (e < err) { err = e; best = (int)c; }
+  }
+  return best;
+}
+
+int freq_of_band(const fxg_datastream& ds, int k) {
+  return k < ds.nrecbands ? ds.recfreq_index[ds.recband_freq[k]]
+                          : ds.zoomband_freqindex[k - ds.nrecbands];
+}
+
+int parent_of_band(const fxg_datastream& ds, int k) {
+  return k < ds.nrecbands ? k : ds.zoomband_parent[k - ds.nrecbands];
+}
+
+int choff_of_band(const fxg_datastream& ds, int k) {
+  return k < ds.nrecbands ? 0 : ds.zoomband_choffset[k - ds.nrecbands];
+}
+
+bool power_of_two(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+#define REQUIRE(cond, code, msg) do { if (!(cond)) { *err = (msg); return (code); } } while (0)
+
+int check_desc(const fxg_job_desc* d, std::string* err) {
+  REQUIRE(d && d->freqs && d->datastreams, FXG_EINVAL, "null descriptor table");
+  REQUIRE(d->nbaselines == 0 || d->baselines, FXG_EINVAL, "null baseline table");
+  REQUIRE(d->nfreqs > 0 && d->nfreqs <= FXG_MAX_FREQS, FXG_EINVAL, "nfreqs out of range");
+  REQUIRE(d->ndatastreams > 0, FXG_EINVAL, "ndatastreams must be positive");
+  REQUIRE(d->nbaselines >= 0, FXG_EINVAL, "nbaselines negative");
+  REQUIRE(d->blockspersend > 0, FXG_EINVAL, "blockspersend must be positive");
+  REQUIRE(d->fringe_order == 1, FXG_EUNSUPPORTED, "only fringe rotation order 1 is built");
+  REQUIRE(d->xmac_precision >= FXG_XMAC_FP32 && d->xmac_precision <= FXG_XMAC_BF16, FXG_EINVAL,
+          "xmac_precision");
+  for (int f = 0; f < d->nfreqs; f++) {
+    REQUIRE(d->freqs[f].nchan > 0, FXG_EINVAL, "freq nchan must be positive");
+    REQUIRE(d->freqs[f].bw_mhz > 0, FXG_EINVAL, "freq bandwidth must be positive");
+    REQUIRE(d->freqs[f].chans_to_avg == 1, FXG_EUNSUPPORTED, "channel averaging is not built");
+  }
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    REQUIRE(ds.format == FXG_FORMAT_VDIF || ds.format == FXG_FORMAT_MARK5B, FXG_EINVAL, "format");
+    REQUIRE(ds.nrecfreqs > 0 && ds.nrecfreqs <= FXG_MAX_RECFREQS, FXG_EINVAL, "nrecfreqs");
+    REQUIRE(ds.nrecbands > 0 && ds.nrecbands <= FXG_MAX_BANDS, FXG_EINVAL, "nrecbands");
+    REQUIRE(power_of_two(ds.nrecbands), FXG_EUNSUPPORTED, "nrecbands must be a power of two");
+    REQUIRE(ds.nzoombands >= 0 && ds.nzoombands <= FXG_MAX_ZOOMBANDS, FXG_EINVAL, "nzoombands");
+    if (ds.format == FXG_FORMAT_VDIF) {
+      REQUIRE(ds.nbits == 2 || ds.nbits == 8, FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
+    } else {
+      REQUIRE(ds.nbits == 2, FXG_EUNSUPPORTED, "Mark5B nbits must be 2");
+      REQUIRE(ds.headerbytes == 16 && ds.framebytes == 10016, FXG_EINVAL, "Mark5B frame is 16+10000 bytes");
+      REQUIRE(ds.nrecbands * ds.nbits <= 32, FXG_EUNSUPPORTED, "Mark5B bitstreams <= 32");
+    }
+    REQUIRE(ds.headerbytes >= 0 && ds.framebytes > ds.headerbytes, FXG_EINVAL, "frame sizes");
+    REQUIRE((ds.framebytes - ds.headerbytes) % 8 == 0, FXG_EINVAL, "payload must be a multiple of 8 bytes");
+    for (int i = 0; i < ds.nrecfreqs; i++) {
+      REQUIRE(ds.recfreq_index[i] >= 0 && ds.recfreq_index[i] < d->nfreqs, FXG_EINVAL, "recfreq_index");
+      REQUIRE(ds.recfreq_phaseoffset_deg[i] == 0.0, FXG_EUNSUPPORTED, "pol phase offsets are not built");
+    }
+    for (int b = 0; b < ds.nrecbands; b++)
+      REQUIRE(ds.recband_freq[b] >= 0 && ds.recband_freq[b] < ds.nrecfreqs, FXG_EINVAL, "recband_freq");
+    for (int z = 0; z < ds.nzoombands; z++) {
+      REQUIRE(ds.zoomband_freqindex[z] >= 0 && ds.zoomband_freqindex[z] < d->nfreqs, FXG_EINVAL, "zoom freq");
+      REQUIRE(ds.zoomband_parent[z] >= 0 && ds.zoomband_parent[z] < ds.nrecbands, FXG_EINVAL, "zoom parent");
+      int pc = d->freqs[ds.recfreq_index[ds.recband_freq[ds.zoomband_parent[z]]]].nchan;
+      REQUIRE(ds.zoomband_choffset[z] >= 0 &&
+              ds.zoomband_choffset[z] + d->freqs[ds.zoomband_freqindex[z]].nchan <= pc,
+              FXG_EINVAL, "zoom band outside its parent");
+    }
+  }
+  for (int b = 0; b < d->nbaselines; b++) {
+    const fxg_baseline& bl = d->baselines[b];
+    REQUIRE(bl.ds1 >= 0 && bl.ds1 < d->ndatastreams && bl.ds2 >= 0 && bl.ds2 < d->ndatastreams,
+            FXG_EINVAL, "baseline datastream index");
+    REQUIRE(bl.nfreqs >= 0 && bl.nfreqs <= FXG_MAX_BLFREQS, FXG_EINVAL, "baseline nfreqs");
+    const fxg_datastream& a = d->datastreams[bl.ds1];
+    const fxg_datastream& c = d->datastreams[bl.ds2];
+    for (int f = 0; f < bl.nfreqs; f++) {
+      int fi = bl.freq_index[f];
+      REQUIRE(fi >= 0 && fi < d->nfreqs, FXG_EINVAL, "baseline freq index");
+      REQUIRE(bl.npolproducts[f] > 0 && bl.npolproducts[f] <= FXG_MAX_POLPRODUCTS, FXG_EINVAL, "npolproducts");
+      for (int p = 0; p < bl.npolproducts[f]; p++) {
+        int b1 = bl.band1[f][p], b2 = bl.band2[f][p];
+        REQUIRE(b1 >= 0 && b1 < a.nrecbands + a.nzoombands && b2 >= 0 && b2 < c.nrecbands + c.nzoombands,
+                FXG_EINVAL, "baseline band index");
+        REQUIRE(freq_of_band(a, b1) == fi && freq_of_band(c, b2) == fi, FXG_EINVAL,
+                "baseline band does not carry the baseline frequency");
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
+  int rc = check_desc(d, err);
+  if (rc) return rc;
+  Plan& P = *p;
+  P = Plan();
+  P.nfreqs = d->nfreqs; P.nds = d->ndatastreams; P.nbl = d->nbaselines;
+
+  // ---- used / equivalent-used frequencies, per-baseline local freq index
+  P.freqused.assign(d->nfreqs, 0);
+  P.equivused.assign(d->nfreqs, 0);
+  P.bllocal.assign(d->nbaselines, std::vector<int>(d->nfreqs, -1));
+  int maxprod = 0;
+  for (int b = 0; b < d->nbaselines; b++)
+    for (int f = 0; f < d->baselines[b].nfreqs; f++) {
+      int fi = d->baselines[b].freq_index[f];
+      P.freqused[fi] = 1;
+      P.bllocal[b][fi] = f;
+      maxprod = std::max(maxprod, d->baselines[b].npolproducts[f]);
+    }
+  for (int f = 0; f < d->nfreqs; f++) {
+    if (P.freqused[f]) continue;
+    const fxg_freq& x = d->freqs[f];
+    double xlo = x.freq_mhz - (x.lsb ? x.bw_mhz : 0.0);
+    for (int g = 0; g < d->nfreqs && !P.equivused[f]; g++) {
+      const fxg_freq& y = d->freqs[g];
+      double ylo = y.freq_mhz - (y.lsb ? y.bw_mhz : 0.0);
+      // signed comparisons, as the reference makes them
+      if (P.freqused[g] && x.bw_mhz - y.bw_mhz < kTiny && xlo - ylo < kTiny &&
+          x.nchan == y.nchan && x.chans_to_avg == y.chans_to_avg)
+        P.equivused[f] = 1;
+    }
+  }
+  P.bandsperac = maxprod > 2 ? 2 : 1;
+
+  // ---- strides
+  P.arraystride.resize(d->ndatastreams);
+  for (int s = 0; s < d->ndatastreams; s++) {
+    int nchan = d->freqs[d->datastreams[s].recfreq_index[0]].nchan;
+    P.arraystride[s] = d->arraystridelen > 0 ? d->arraystridelen : stride_for((unsigned)nchan);
+  }
+  P.xmacstride = d->xmacstridelen > 0 ? d->xmacstridelen : xmac_stride_for(d);
+  P.rotatestride = stride_for((unsigned)P.xmacstride);
+  for (int f = 0; f < d->nfreqs; f++)
+    REQUIRE(!P.freqused[f] || d->freqs[f].nchan % P.xmacstride == 0, FXG_EINVAL,
+            "xmac stride must divide every correlated channel count");
+
+  // ---- result layout
+  long long at = 0;
+  P.bloff.assign(d->nfreqs, std::vector<long long>(d->nbaselines, -1));
+  P.bwoff.assign(d->nfreqs, std::vector<long long>(d->nbaselines, -1));
+  for (int f = 0; f < d->nfreqs; f++) {
+    if (!P.freqused[f]) continue;
+    for (int b = 0; b < d->nbaselines; b++) {
+      int lf = P.bllocal[b][f];
+      if (lf < 0) continue;
+      P.bloff[f][b] = at;
+      at += (long long)d->baselines[b].npolproducts[lf] * d->freqs[f].nchan;
+    }
+  }
+  for (int f = 0; f < d->nfreqs; f++) {
+    if (!P.freqused[f]) continue;
+    for (int b = 0; b < d->nbaselines; b++) {
+      int lf = P.bllocal[b][f];
+      if (lf < 0) continue;
+      P.bwoff[f][b] = at;
+      at += (d->baselines[b].npolproducts[lf] + 1) / 2;
+    }
+  }
+  auto ac_wanted = [&](int fi) { return P.freqused[fi] || P.equivused[fi]; };
+  P.acoff.resize(d->ndatastreams);
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    P.acoff[s] = at;
+    for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++)
+      if (ac_wanted(freq_of_band(ds, k))) at += (long long)P.bandsperac * d->freqs[freq_of_band(ds, k)].nchan;
+  }
+  P.acwoff.resize(d->ndatastreams);
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    P.acwoff[s] = at;
+    int n = 0;
+    for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++)
+      if (ac_wanted(freq_of_band(ds, k))) n += P.bandsperac;
+    at += std::max(1, (n + 1) / 2);
+  }
+  // phase-cal results: none (no phase-cal extraction in this build)
+
+  // ---- per-datastream device constants; one FFT length per job
+  P.C = d->freqs[d->datastreams[0].recfreq_index[0]].nchan;
+  P.L = 2 * P.C;
+  int band = 0;
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    const fxg_freq& f0 = d->freqs[ds.recfreq_index[0]];
+    for (int i = 0; i < ds.nrecfreqs; i++) {
+      const fxg_freq& fi = d->freqs[ds.recfreq_index[i]];
+      REQUIRE(fi.nchan == f0.nchan && fi.bw_mhz == f0.bw_mhz, FXG_EUNSUPPORTED,
+              "recorded freqs of a datastream must share nchan and bandwidth");
+    }
+    REQUIRE(f0.nchan == P.C, FXG_EUNSUPPORTED, "all datastreams of a job must share nchan");
+    REQUIRE(power_of_two(P.L) && P.L >= 128 && P.L <= 8192, FXG_EUNSUPPORTED,
+            "FFT length must be a power of two in [128, 8192]");
+    DevStation st{};
+    st.format = ds.format; st.nbits = ds.nbits; st.nchan = ds.nrecbands;
+    st.framebytes = ds.framebytes; st.headerbytes = ds.headerbytes;
+    st.databytes = ds.framebytes - ds.headerbytes; st.blanker = ds.blanker;
+    int bitsper = ds.nrecbands * ds.nbits;
+    st.framesamples = (int)((long long)st.databytes * 8 / bitsper);
+    st.granularity = bitsper >= 8 ? 1 : 8 / bitsper;
+    st.groupbytes = bitsper >= 8 ? bitsper / 8 : 1;
+    st.groupsamples = bitsper >= 8 ? 1 : 8 / bitsper;
+    st.quirk8 = (ds.format == FXG_FORMAT_VDIF && ds.nbits == 8 && ds.nrecbands >= 2 && ds.nrecbands <= 4);
+    st.C = P.C; st.L = P.L; st.s = P.arraystride[s];
+    REQUIRE(st.C % st.s == 0 && st.s <= 256 && P.L / st.s <= 256, FXG_EINVAL,
+            "array stride must divide nchan and give at most 256 sub/step points");
+    st.nfrsteps = P.L / st.s; st.nfracsteps = st.C / st.s;
+    st.bpbnum = bitsper / 8;
+    st.bpbden = 1;
+    if (st.bpbnum == 0) { st.bpbnum = 1; st.bpbden = 8 / bitsper; }
+    st.bw = f0.bw_mhz;
+    st.sampletime = 1.0 / (2.0 * st.bw);
+    st.fractionalLo = 0; st.deltapol = 0;
+    for (int i = 0; i < ds.nrecfreqs; i++) {
+      double lo = d->freqs[ds.recfreq_index[i]].freq_mhz;
+      if (std::fabs(lo - (int)(lo + 0.5)) > kTiny) st.fractionalLo = 1;
+      if (ds.recfreq_clockoffsetdelta_us[i] != 0.0) st.deltapol = 1;
+    }
+    st.band0 = band;
+    if (ds.format == FXG_FORMAT_VDIF) { st.lut[0] = -kHiMag; st.lut[1] = -1.0f; st.lut[2] = 1.0f; st.lut[3] = kHiMag; }
+    else { st.lut[0] = -kHiMag; st.lut[1] = 1.0f; st.lut[2] = -1.0f; st.lut[3] = kHiMag; }
+    P.st.push_back(st);
+    for (int k = 0; k < ds.nrecbands; k++) {
+      DevBand b{};
+      int lf = ds.recband_freq[k];
+      const fxg_freq& fq = d->freqs[ds.recfreq_index[lf]];
+      b.station = s; b.band = k; b.localfreq = lf; b.lsb = fq.lsb ? 1 : 0;
+      b.useB = (st.deltapol && ds.recband_pol[k] != 'R') ? 1 : 0;
+      b.lofreq = fq.freq_mhz;
+      b.clock = ds.recfreq_clockoffset_us[lf];
+      b.clockdelta = ds.recfreq_clockoffsetdelta_us[lf];
+      b.looffset = ds.recfreq_looffset_hz[lf];
+      P.bands.push_back(b);
+      band++;
+    }
+  }
+  P.nbands_total = band;
+
+  // ---- XMAC frequencies: every frequency whose products or autocorrelations
+  //      are output (zoom autocorrelations read their parent's accumulation)
+  std::vector<char> need(d->nfreqs, 0);
+  for (int f = 0; f < d->nfreqs; f++) need[f] = P.freqused[f];
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+      int fk = freq_of_band(ds, k);
+      if (!ac_wanted(fk)) continue;
+      need[ds.recfreq_index[ds.recband_freq[parent_of_band(ds, k)]]] = 1;
+    }
+  }
+  P.xf_of_freq.assign(d->nfreqs, -1);
+  long long voff = 0;
+  for (int f = 0; f < d->nfreqs; f++) {
+    if (!need[f]) continue;
+    XmacFreq x;
+    x.freq = f; x.nchan = d->freqs[f].nchan;
+    for (int s = 0; s < d->ndatastreams; s++) {
+      const fxg_datastream& ds = d->datastreams[s];
+      for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+        if (freq_of_band(ds, k) != f) continue;
+        x.in_station.push_back(s);
+        x.in_band.push_back(parent_of_band(ds, k));
+        x.in_total.push_back(k);
+        x.in_off.push_back(choff_of_band(ds, k));   // completed with band offsets at allocation
+      }
+    }
+    if (x.in_station.empty()) continue;
+    x.voff = voff;
+    voff += (long long)x.in_station.size() * x.in_station.size() * x.nchan;
+    P.xf_of_freq[f] = (int)P.xf.size();
+    P.xf.push_back(x);
+  }
+  P.vsize = voff;
+
+  // ---- scatter segments
+  auto input_index = [&](int xfi, int s, int totalband) {
+    const XmacFreq& x = P.xf[xfi];
+    for (size_t i = 0; i < x.in_station.size(); i++)
+      if (x.in_station[i] == s && x.in_total[i] == totalband) return (int)i;
+    return -1;
+  };
+  bool seg_ok = true;
+  auto vis_seg = [&](long long resoff, int xfi, int a, int b, int nchan, int choff) {
+    if (xfi < 0 || a < 0 || b < 0) { seg_ok = false; return; }
+    const XmacFreq& x = P.xf[xfi];
+    int nin = (int)x.in_station.size();
+    DevSeg g{};
+    g.kind = SEG_VIS; g.resoff = resoff; g.nchan = nchan;
+    // V is accumulated for a <= b tiles and in full inside diagonal tiles;
+    // read the upper triangle and conjugate when the order is swapped
+    if (a <= b) { g.voff = x.voff + ((long long)a * nin + b) * x.nchan + choff; g.conj = 0; }
+    else        { g.voff = x.voff + ((long long)b * nin + a) * x.nchan + choff; g.conj = 1; }
+    P.segs.push_back(g);
+    P.maxseglen = std::max(P.maxseglen, nchan);
+  };
+  for (int f = 0; f < d->nfreqs; f++) {
+    if (!P.freqused[f]) continue;
+    int xfi = P.xf_of_freq[f];
+    for (int b = 0; b < d->nbaselines; b++) {
+      int lf = P.bllocal[b][f];
+      if (lf < 0) continue;
+      const fxg_baseline& bl = d->baselines[b];
+      for (int pp = 0; pp < bl.npolproducts[lf]; pp++) {
+        int ia = input_index(xfi, bl.ds1, bl.band1[lf][pp]);
+        int ib = input_index(xfi, bl.ds2, bl.band2[lf][pp]);
+        vis_seg(P.bloff[f][b] + (long long)pp * d->freqs[f].nchan, xfi, ia, ib, d->freqs[f].nchan, 0);
+      }
+    }
+  }
+  // baseline weights: floats, value = weight table entry b
+  for (int f = 0; f < d->nfreqs; f++) {
+    if (!P.freqused[f]) continue;
+    for (int b = 0; b < d->nbaselines; b++) {
+      int lf = P.bllocal[b][f];
+      if (lf < 0) continue;
+      for (int pp = 0; pp < d->baselines[b].npolproducts[lf]; pp++) {
+        DevSeg g{}; g.kind = SEG_FLOATS; g.resoff = 2 * P.bwoff[f][b] + pp; g.voff = b; g.nchan = 1;
+        P.segs.push_back(g);
+      }
+    }
+  }
+  // autocorrelations (Mode::process accumulation, mode.cpp:1292-1375, copied
+  // out by averageAndSendAutocorrs, core.cpp:1299-1330)
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    long long at2 = P.acoff[s];
+    long long wat = 2 * P.acwoff[s];
+    for (int pass = 0; pass < P.bandsperac; pass++) {
+      for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+        int fk = freq_of_band(ds, k);
+        if (!ac_wanted(fk)) continue;
+        int nch = d->freqs[fk].nchan;
+        int parent = parent_of_band(ds, k);
+        int pf = ds.recfreq_index[ds.recband_freq[parent]];
+        int xfi = P.xf_of_freq[pf];
+        int choff = choff_of_band(ds, k);
+        if (pass == 0) {
+          int a = input_index(xfi, s, parent);
+          vis_seg(at2, xfi, a, a, nch, choff);
+        } else if (d->crosspol_autocorrs) {
+          // the first two recorded bands of the parent's local frequency
+          int lf = ds.recband_freq[parent];
+          int i0 = -1, i1 = -1;
+          for (int q = 0; q < ds.nrecbands; q++)
+            if (ds.recband_freq[q] == lf) { if (i0 < 0) i0 = q; else if (i1 < 0) i1 = q; }
+          if (i1 >= 0 && (parent == i0 || parent == i1)) {
+            int other = parent == i0 ? i1 : i0;
+            vis_seg(at2, xfi, input_index(xfi, s, parent), input_index(xfi, s, other), nch, choff);
+          }
+        }
+        at2 += nch;
+        // AC weights: getWeight(false, parent) for both passes (core.cpp:1331-1366)
+        DevSeg w{}; w.kind = SEG_FLOATS; w.resoff = wat++; w.voff = d->nbaselines + s; w.nchan = 1;
+        if (pass == 0 || d->crosspol_autocorrs) P.segs.push_back(w);
+      }
+    }
+  }
+
+  REQUIRE(seg_ok, FXG_EINVAL, "a result product names a band that no XMAC frequency holds");
+  P.info.result_length = at;
+  P.info.arraystridelen = P.arraystride[0];
+  P.info.xmacstridelen = P.xmacstride;
+  P.info.rotatestridelen = P.rotatestride;
+  P.info.bandsperautocorr = P.bandsperac;
+  P.info.fftchannels = P.L;
+
+  // ---- spectra chunking: keep the spectra of at most ~16 GiB resident
+  long long per_block = (long long)P.nbands_total * P.C * (d->xmac_precision == FXG_XMAC_FP32 ? 8 : 4);
+  long long budget = 16LL << 30;
+  int chunk = d->max_chunk_blocks > 0 ? d->max_chunk_blocks : (int)std::max(1LL, budget / per_block);
+  chunk = std::min(chunk, d->blockspersend);
+  P.chunk_blocks = chunk;
+  P.info.chunk_blocks = chunk;
+  P.info.device_bytes = plan_device_bytes(P, d);
+  return 0;
+}
+
+int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d) {
+  long long spec = (long long)p.nbands_total * p.C * p.chunk_blocks * (d->xmac_precision == FXG_XMAC_FP32 ? 8 : 4);
+  long long v = p.vsize * 8;
+  long long blocks = (long long)p.nds * d->blockspersend * (long long)sizeof(DevBlock);
+  long long res = p.info.result_length * 8;
+  long long tables = (long long)p.segs.size() * sizeof(DevSeg) + p.bands.size() * sizeof(DevBand) + (1 << 20);
+  return spec + v + blocks + res + tables;
+}
+
+}  // namespace fxg

@@ -1,0 +1,490 @@
+// fx_station.hip — station-based (F-engine) kernels for gfx950:
+//   k_frames    per-frame validity / blanking zones   (mark5_stream.c:153-202,
+//               format_vdif.c:4009-4060, blanker_mark5.c:53-165, 282-312)
+//   k_prologue  FP64 block geometry + data weight      (mode.cpp:649-813,
+//               mk5mode.cpp:104-177)
+//   k_fstage<L> LUT unpack -> fringe rotation -> L-point FP32 Stockham FFT
+//               -> sideband select -> fractional-sample rotation -> spectra
+//               (mode.cpp:909-1310)
+//   k_unpack    standalone Mk5Mode::unpack equivalent
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include "fx_kernels.h"
+
+namespace fxg {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ld64(const unsigned char* p) {
+  unsigned long long v = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) v |= (unsigned long long)p[i] << (8 * i);
+  return v;
+}
+__device__ __forceinline__ unsigned ld32(const unsigned char* p) {
+  return (unsigned)p[0] | ((unsigned)p[1] << 8) | ((unsigned)p[2] << 16) | ((unsigned)p[3] << 24);
+}
+
+constexpr unsigned long long kFill64 = 0x1122334411223344ULL;  // fill pattern, little endian
+constexpr unsigned kFill32 = 0x11223344U;
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ---------------------------------------------------------------------------
+// k_frames: one thread per frame -> (startvalid, endvalid) payload byte range
+// ---------------------------------------------------------------------------
+__global__ void k_frames(const DevStation* __restrict__ st, const DevStationSub* __restrict__ ss,
+                         int2* __restrict__ ftab) {
+  const int s = blockIdx.y;
+  const DevStation S = st[s];
+  const DevStationSub Q = ss[s];
+  long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= Q.nframes) return;
+  const unsigned char* fr = Q.data + f * S.framebytes;
+  const unsigned char* pl = fr + S.headerbytes;
+  int2 z = make_int2(0, kBig);
+  bool valid = true;
+  if (S.format == FXG_FORMAT_VDIF) {
+    unsigned w0 = ld32(fr), w2 = ld32(fr + 8);
+    if (w2 == 0u || ((w0 >> 31) & 1u)) valid = false;
+  }
+  if (!valid) {
+    z = make_int2(kBig, 0);
+  } else if (S.blanker == FXG_BLANKER_NONE) {
+    z = make_int2(0, kBig);
+  } else if (S.format == FXG_FORMAT_VDIF) {
+    int nword = S.databytes / 8;
+    if (ld64(pl) == kFill64 || ld64(pl + 8LL * (nword - 1)) == kFill64) z = make_int2(kBig, 0);
+  } else {
+    // Mark5B: one 32 KiB zone covers the whole 10000-byte payload
+    int n = S.databytes / 8;
+    bool startOK = ld64(pl) != kFill64, endOK = ld64(pl + 8LL * (n - 1)) != kFill64;
+    if (!startOK && !endOK) z = make_int2(kBig, 0);
+    else if (startOK && !endOK) {
+      int i = n / 2, r = n - 1;
+      for (int st2 = n / 4; st2 > 0; st2 >>= 1) {
+        if (ld64(pl + 8LL * i) == kFill64) { r = i; i -= st2; } else i += st2;
+      }
+      r = 8 * r;
+      if (ld32(pl + 4LL * (r / 4 - 1)) == kFill32) r -= 4;
+      z = make_int2(0, r);
+    } else if (!startOK && endOK) {
+      int i = n / 2, r = n - 1;
+      for (int st2 = n / 4; st2 > 0; st2 >>= 1) {
+        if (ld64(pl + 8LL * i) != kFill64) { r = i; i -= st2; } else i += st2;
+      }
+      r = 8 * r;
+      if (ld32(pl + 4LL * (r / 4 - 2)) == kFill32) r += 4;
+      z = make_int2(r, kBig);
+    }
+    if (fr[S.headerbytes - 11] & 0x80) z = make_int2(kBig, 0);  // per-group header check, whole frame
+  }
+  ftab[Q.fbase + f] = z;
+}
+
+// number of samples in [t0, t1) of frame f (frame-local sample indices) whose
+// decoder group lies inside the frame's valid byte range
+__device__ __forceinline__ long long valid_in_frame(const DevStation& S, int2 z, long long t0, long long t1) {
+  if (z.x >= z.y) return 0;
+  // group g spans bytes [g*gb, g*gb+gb); the decoder tests the first byte
+  long long g0 = ((long long)z.x + S.groupbytes - 1) / S.groupbytes;
+  long long g1 = ((long long)z.y + S.groupbytes - 1) / S.groupbytes;
+  long long a = max(t0, g0 * S.groupsamples), b = min(t1, g1 * S.groupsamples);
+  return b > a ? b - a : 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_prologue: FP64 geometry of every (datastream, block).  The expression
+// order is the reference's, with FMA contraction off, so nearestsample and
+// integerdelay are the same integers the CPU computes.
+// ---------------------------------------------------------------------------
+__global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSub* __restrict__ ss,
+                           const int* __restrict__ flags, const int2* __restrict__ ftab,
+                           DevSubint sub, DevBlock* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  const int index = blockIdx.x * blockDim.x + threadIdx.x;
+  if (index >= sub.nblocks) return;
+  const DevStation S = st[s];
+  const DevStationSub Q = ss[s];
+  DevBlock g;
+  g.status = 0; g.nearestsample = 0; g.integerdelay = 0; g.intwall = 0;
+  g.weight = 0.0f; g.fse = 0.0f; g.validend = 0x7fffffffffffffffLL; g.a = 0; g.b = 0; g.fracwall = 0;
+  bool ok = Q.usable != 0;
+  if (ok) {
+    int w = index / FXG_FLAGS_PER_INT;
+    ok = w < Q.nflags && ((flags[Q.flagoff + w] >> (index % FXG_FLAGS_PER_INT)) & 1);
+  }
+  if (!ok) { g.status = 1; out[(long long)s * sub.nblocks + index] = g; return; }
+  const double* ip = Q.interp;
+  const int fftchannels = S.L;
+  double fftcentre = index + 0.5;
+  double averagedelay = ip[0] * fftcentre * fftcentre + ip[1] * fftcentre + ip[2];
+  double fftstartmicrosec = index * fftchannels * S.sampletime;
+  double starttime = (sub.offsec - Q.datasec) * 1000000.0 +
+                     ((long long)sub.offns - (long long)Q.datans) / 1000.0 + fftstartmicrosec - averagedelay;
+  int nearestsample = (int)(starttime / S.sampletime + 0.5);
+  double walltimesecs = (sub.scanstart + sub.offsec) + sub.offns / 1.0e9 + fftstartmicrosec / 1.0e6;
+  g.intwall = (int)walltimesecs;
+  g.fracwall = walltimesecs - g.intwall;
+  g.nearestsample = nearestsample;
+  if (nearestsample < -1 ||
+      ((((long long)nearestsample + fftchannels) / 1) * S.bpbnum) / S.bpbden > Q.nbytes) {
+    g.status = 2; out[(long long)s * sub.nblocks + index] = g; return;
+  }
+  if (nearestsample == -1) nearestsample = 0;
+  g.nearestsample = nearestsample;
+  // good samples in [nearestsample, nearestsample + L): the decoder's count
+  // after Mk5Mode's granularity erasure
+  long long t = nearestsample, tend = (long long)nearestsample + S.L;
+  long long good = 0;
+  long long unpackstart = (long long)nearestsample - (nearestsample % S.granularity);
+  if (S.quirk8) {
+    // 2..4-channel 8-bit decoders stop advancing at the first blank group:
+    // everything after it in the unpack window decodes as zero
+    long long f0 = unpackstart / S.framesamples;
+    long long f1 = (tend - 1) / S.framesamples;
+    for (long long f = f0; f <= f1; f++) {
+      bool fv = f < Q.nframes && ftab[Q.fbase + f].x < ftab[Q.fbase + f].y;
+      if (!fv) { g.validend = max(unpackstart, f * (long long)S.framesamples); break; }
+    }
+  }
+  long long tcut = min(tend, g.validend);
+  while (t < tcut) {
+    long long f = t / S.framesamples;
+    long long fe = min(tcut, (f + 1) * (long long)S.framesamples);
+    if (f < Q.nframes) {
+      int2 z = ftab[Q.fbase + f];
+      long long base = f * (long long)S.framesamples;
+      good += valid_in_frame(S, z, t - base, fe - base);
+    }
+    t = fe;
+  }
+  g.weight = (float)good / (float)S.L;
+  if (!(g.weight > 0.0f)) { g.status = 3; out[(long long)s * sub.nblocks + index] = g; return; }
+  double nearestsampletime = nearestsample * S.sampletime;
+  g.fse = (float)(starttime - nearestsampletime);
+  double d0 = ip[0] * index * index + ip[1] * index + ip[2];
+  double d1 = ip[0] * (index + 0.5) * (index + 0.5) + ip[1] * (index + 0.5) + ip[2];
+  double d2 = ip[0] * (index + 1) * (index + 1) + ip[1] * (index + 1) + ip[2];
+  double a = d2 - d0;
+  double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+  int integerdelay = (int)b;
+  b -= integerdelay;
+  g.a = a; g.b = b; g.integerdelay = integerdelay;
+  out[(long long)s * sub.nblocks + index] = g;
+}
+
+// ---------------------------------------------------------------------------
+// sample decoding
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sample_value(const DevStation& S, const DevStationSub& Q,
+                                              const int2* __restrict__ ftab, const float* lut8,
+                                              const float lut2[4], long long t, long long validend, int band) {
+  if (t >= validend || t < 0) return 0.0f;
+  long long f = t / S.framesamples;
+  if (f >= Q.nframes) return 0.0f;
+  int tin = (int)(t - f * S.framesamples);
+  int grp = tin / S.groupsamples;
+  int ib = grp * S.groupbytes;
+  int2 z = ftab[Q.fbase + f];
+  if (ib < z.x || ib >= z.y) return 0.0f;
+  int bit = ((tin - grp * S.groupsamples) * S.nchan + band) * S.nbits;
+  unsigned char byte = Q.data[f * S.framebytes + S.headerbytes + ib + (bit >> 3)];
+  if (S.nbits == 8) return lut8[byte];
+  return lut2[(byte >> (bit & 7)) & 3];
+}
+
+// standalone decode of nsamp sample-times from `first` into out[nchan][nsamp]
+__global__ void k_unpack(const DevStation* __restrict__ st, const DevStationSub* __restrict__ ss,
+                         const int2* __restrict__ ftab, const float* __restrict__ lut8,
+                         long long first, int nsamp, long long validend, float* __restrict__ out) {
+  const DevStation S = *st;
+  const DevStationSub Q = *ss;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.y;
+  if (n >= nsamp) return;
+  out[(long long)c * nsamp + n] = sample_value(S, Q, ftab, lut8, S.lut, first + n, validend, c);
+}
+
+// ---------------------------------------------------------------------------
+// in-register DFTs (forward, e^{-2 pi i nk/R})
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 sub2(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+__device__ __forceinline__ void dft2(float2& a, float2& b) { float2 t = a; a = add2(t, b); b = sub2(t, b); }
+
+__device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2& x3) {
+  float2 a = add2(x0, x2), b = sub2(x0, x2), c = add2(x1, x3), d = sub2(x1, x3);
+  x0 = add2(a, c); x2 = sub2(a, c);
+  float2 md = mul_mi(d);
+  x1 = add2(b, md); x3 = sub2(b, md);
+}
+
+template <int R> struct DFT;
+template <> struct DFT<2> { __device__ static void run(float2* v) { dft2(v[0], v[1]); } };
+template <> struct DFT<4> { __device__ static void run(float2* v) { dft4(v[0], v[1], v[2], v[3]); } };
+template <> struct DFT<8> {
+  __device__ static void run(float2* v) {
+    const float h = 0.70710678118654752440f;
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3); dft4(o0, o1, o2, o3);
+    o1 = make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));      // * e^{-i pi/4}
+    o2 = mul_mi(o2);                                             // * e^{-i pi/2}
+    o3 = make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));     // * e^{-3i pi/4}
+    v[0] = add2(e0, o0); v[4] = sub2(e0, o0);
+    v[1] = add2(e1, o1); v[5] = sub2(e1, o1);
+    v[2] = add2(e2, o2); v[6] = sub2(e2, o2);
+    v[3] = add2(e3, o3); v[7] = sub2(e3, o3);
+  }
+};
+template <> struct DFT<16> {
+  __device__ static void run(float2* v) {
+    // n = 4 n1 + n2, k = k1 + 4 k2
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+      float2 a = v[n2], b = v[4 + n2], c = v[8 + n2], d = v[12 + n2];
+      dft4(a, b, c, d);
+      y[n2][0] = a; y[n2][1] = b; y[n2][2] = c; y[n2][3] = d;
+    }
+    // twiddles W16^(n2*k1)
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+    y[1][1] = cmulf(y[1][1], make_float2(c1, -s1));
+    y[1][2] = cmulf(y[1][2], make_float2(h, -h));
+    y[1][3] = cmulf(y[1][3], make_float2(s1, -c1));
+    y[2][1] = cmulf(y[2][1], make_float2(h, -h));
+    y[2][2] = mul_mi(y[2][2]);
+    y[2][3] = cmulf(y[2][3], make_float2(-h, -h));
+    y[3][1] = cmulf(y[3][1], make_float2(s1, -c1));
+    y[3][2] = cmulf(y[3][2], make_float2(-h, -h));
+    y[3][3] = cmulf(y[3][3], make_float2(-c1, s1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+      float2 a = y[0][k1], b = y[1][k1], c = y[2][k1], d = y[3][k1];
+      dft4(a, b, c, d);
+      v[k1] = a; v[k1 + 4] = b; v[k1 + 8] = c; v[k1 + 12] = d;
+    }
+  }
+};
+
+template <int L> struct FftShape {
+  static constexpr int LOG = (L == 128) ? 7 : (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10
+                           : (L == 2048) ? 11 : (L == 4096) ? 12 : 13;
+  static constexpr int R0 = (LOG % 4) ? (1 << (LOG % 4)) : 16;   // first-pass radix
+  static constexpr int NPASS16 = LOG / 4 - ((LOG % 4) ? 0 : 1);  // radix-16 passes after the first
+  static constexpr int NTF = L / 16;                              // FFT threads
+  static constexpr int NT = NTF < 64 ? 64 : NTF;                  // block size
+};
+
+
+
+template <int L>
+__global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
+  using Sh = FftShape<L>;
+  constexpr int NT = Sh::NT, NTF = Sh::NTF, R0 = Sh::R0;
+  __shared__ float2 buf[L];
+  __shared__ float2 rsub[256], rstep[256], fsub[256], fstep[256];
+  __shared__ float lut8s[256];
+
+  const int tid = threadIdx.x;
+  const int cb = blockIdx.x;                      // block within chunk
+  const int blk = A.chunk0 + cb;
+  const DevBand B = A.bands[blockIdx.y];
+  const DevStation S = A.st[B.station];
+  const DevStationSub Q = A.ss[B.station];
+  const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
+  const int C = L / 2;
+  const long long outbase = B.specoff + (long long)cb * C;
+
+  if (G.status != 0) {
+    if (A.prec == FXG_XMAC_FP32) {
+      float2* o = (float2*)A.spectra + outbase;
+      for (int c = tid; c < C; c += NT) o[c] = make_float2(0.f, 0.f);
+    } else {
+      unsigned* o = (unsigned*)A.spectra + outbase;
+      for (int c = tid; c < C; c += NT) o[c] = 0u;
+    }
+    return;
+  }
+
+  // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
+  {
+#pragma clang fp contract(off)
+    const int s = S.s;
+    const double lofreq = B.lofreq, looffset = B.looffset;
+    const bool looff = looffset > 0.0 || looffset < 0.0;
+    double fraclooffset = 0.0;
+    bool isfraclooffset = false;
+    if (looff) {
+      fraclooffset = fabs(looffset) - (int)fabs(looffset);
+      if (fraclooffset > kTiny) isfraclooffset = true;
+      if (looffset < 0) fraclooffset = -fraclooffset;
+    }
+    for (int k = tid; k < s; k += NT) {
+      double ph = (((double)k / (double)L) * G.a + G.b) * lofreq;
+      if (S.fractionalLo) ph += (lofreq - (int)lofreq) * (double)G.integerdelay;
+      if (looff) {
+        ph += ((k * S.sampletime) / 1e6) * (-looffset);
+        ph += -looffset * G.fracwall;
+        if (isfraclooffset) ph += -fraclooffset * G.intwall;
+      }
+      float arg = (float)(-kTwoPi * (ph - (int)ph));
+      float sn, cs; sincosf(arg, &sn, &cs);
+      rsub[k] = make_float2(cs, sn);
+    }
+    for (int j = tid; j < S.nfrsteps; j += NT) {
+      double ph = (((double)(j * s)) / (double)L) * G.a * lofreq;
+      if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
+      float arg = (float)(-kTwoPi * (ph - (int)ph));
+      float sn, cs; sincosf(arg, &sn, &cs);
+      rstep[j] = make_float2(cs, sn);
+    }
+    const double sign = B.useB ? -1.0 : 1.0;
+    const float val = (float)((double)G.fse - B.clock + sign * B.clockdelta / 2);
+    for (int k = tid; k < s; k += NT) {
+      float cf = (float)((kTwoPi * k * S.bw) / C);
+      float sn, cs; sincosf(cf * val, &sn, &cs);
+      fsub[k] = make_float2(cs, sn);
+    }
+    for (int j = tid; j < S.nfracsteps; j += NT) {
+      double dcf = B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
+      float cf = (float)dcf;
+      float sn, cs; sincosf(cf * val, &sn, &cs);
+      fstep[j] = make_float2(cs, sn);
+    }
+    if (S.nbits == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
+  }
+  __syncthreads();
+
+  float lut2[4] = {S.lut[0], S.lut[1], S.lut[2], S.lut[3]};
+  const int s = S.s;
+  const long long t0 = G.nearestsample;
+
+  // ---- first pass (Ns = 1): decode, rotate, radix-R0 DFT, scatter to LDS
+  if (tid < NTF) {
+#pragma unroll
+    for (int u = 0; u < 16 / R0; u++) {
+      const int j = tid + u * NTF;
+      float2 v[R0];
+#pragma unroll
+      for (int r = 0; r < R0; r++) {
+        const int n = j + r * (L / R0);
+        float x = sample_value(S, Q, A.ftab, lut8s, lut2, t0 + n, G.validend, B.band);
+        float2 rot = n < s ? rsub[n] : cmulf(rsub[n % s], rstep[n / s]);
+        v[r] = make_float2(x * rot.x, x * rot.y);
+      }
+      DFT<R0>::run(v);
+#pragma unroll
+      for (int r = 0; r < R0; r++) buf[j * R0 + r] = v[r];
+    }
+  }
+  __syncthreads();
+
+  // ---- middle radix-16 passes: LDS -> registers -> LDS
+  int Ns = R0;
+#pragma unroll
+  for (int p = 0; p < Sh::NPASS16 - 1; p++) {
+    float2 v[16];
+    const int j = tid;
+    if (tid < NTF) {
+      const int k = j % Ns;
+      const int twstep = L / (Ns * 16);
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        float2 x = buf[j + r * (L / 16)];
+        v[r] = r == 0 ? x : cmulf(x, A.tw[(r * k * twstep) & (L - 1)]);
+      }
+      DFT<16>::run(v);
+    }
+    __syncthreads();
+    if (tid < NTF) {
+      const int k = j % Ns;
+      const int idxD = (j / Ns) * Ns * 16 + k;
+#pragma unroll
+      for (int r = 0; r < 16; r++) buf[idxD + r * Ns] = v[r];
+    }
+    __syncthreads();
+    Ns *= 16;
+  }
+
+  // ---- last pass (Ns = L/16): LDS -> registers -> sideband half -> frac rotation -> HBM
+  if (tid < NTF) {
+    float2 v[16];
+    const int j = tid;                     // j < Ns, so k = j and the bin is j + r*L/16
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      float2 x = buf[j + r * (L / 16)];
+      v[r] = r == 0 ? x : cmulf(x, A.tw[r * j]);
+    }
+    DFT<16>::run(v);
+    const int rbase = B.lsb ? 8 : 0;       // USB keeps bins [0,C), real LSB keeps [C,2C)
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int c = j + q * (L / 16);      // output channel
+      float2 x = v[rbase + q];
+      float2 fr = cmulf(fsub[c % s], fstep[c / s]);
+      float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
+      if (A.prec == FXG_XMAC_FP32) {
+        ((float2*)A.spectra)[outbase + c] = y;
+      } else if (A.prec == FXG_XMAC_FP16) {
+        __half2 h = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
+        ((__half2*)A.spectra)[outbase + c] = h;
+      } else {
+        __hip_bfloat162 h = __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
+        ((__hip_bfloat162*)A.spectra)[outbase + c] = h;
+      }
+    }
+  }
+}
+
+#define INSTANTIATE(Lv) template __global__ void k_fstage<Lv>(FStageArgs);
+INSTANTIATE(128) INSTANTIATE(256) INSTANTIATE(512) INSTANTIATE(1024)
+INSTANTIATE(2048) INSTANTIATE(4096) INSTANTIATE(8192)
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
+                         long long maxframes, hipStream_t stream) {
+  if (maxframes <= 0) return hipSuccess;
+  dim3 grid((unsigned)((maxframes + 255) / 256), nst);
+  hipLaunchKernelGGL(k_frames, grid, dim3(256), 0, stream, st, ss, ftab);
+  return hipGetLastError();
+}
+
+hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
+                           const int2* ftab, DevSubint sub, DevBlock* out, int nst, hipStream_t stream) {
+  dim3 grid((unsigned)((sub.nblocks + 127) / 128), nst);
+  hipLaunchKernelGGL(k_prologue, grid, dim3(128), 0, stream, st, ss, flags, ftab, sub, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fstage(int L, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream) {
+  dim3 grid(nblocks, nbands);
+  switch (L) {
+#define CASE(Lv) case Lv: hipLaunchKernelGGL(k_fstage<Lv>, grid, dim3(FftShape<Lv>::NT), 0, stream, a); break;
+    CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const int2* ftab, const float* lut8,
+                         long long first, int nsamp, int nchan, long long validend, float* out,
+                         hipStream_t stream) {
+  dim3 grid((unsigned)((nsamp + 255) / 256), nchan);
+  hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, stream, st, ss, ftab, lut8, first, nsamp, validend, out);
+  return hipGetLastError();
+}
+
+}  // namespace fxg

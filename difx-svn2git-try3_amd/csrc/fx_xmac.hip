@@ -1,0 +1,188 @@
+// fx_xmac.hip — baseline-based (X-engine) kernels for gfx950:
+//   k_xmac_f32   dense Hermitian cross-multiply-accumulate over every input
+//                (datastream band) of one frequency, FP32 FMA: the plain
+//                drop-in for Core::processdata's vectorAddProduct_cf32
+//                (core.cpp:826-975) and Mode's autocorrelations
+//                (mode.cpp:1292-1375)
+//   k_weights    baseline and autocorrelation weights (core.cpp:998-1045,
+//                mode.cpp:1299-1307)
+//   k_scatter    copy into the core result layout (core.cpp:1781-1787,
+//                1299-1366)
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include "fx_kernels.h"
+
+namespace fxg {
+
+// ---------------------------------------------------------------------------
+// FP32 XMAC.  One workgroup = one channel tile of CT channels of one
+// frequency; every thread owns one channel and one 8x8 tile (ta <= tb) of
+// the input-by-input product matrix, accumulating
+//     V[a][b][c] += X_a[c] * conj(X_b[c])
+// over every block of the chunk.  Input spectra are staged through LDS, TB
+// blocks at a time.
+// ---------------------------------------------------------------------------
+
+
+__device__ __forceinline__ float2 load_spec(const float2* p, long long i) { return p[i]; }
+__device__ __forceinline__ float2 load_spec(const __half2* p, long long i) { return __half22float2(p[i]); }
+__device__ __forceinline__ float2 load_spec(const __hip_bfloat162* p, long long i) {
+  __hip_bfloat162 v = p[i];
+  return make_float2(__bfloat162float(v.x), __bfloat162float(v.y));
+}
+
+template <typename TS>
+__global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
+  const TS* spec = (const TS*)A.spec;
+  extern __shared__ float2 sh[];              // [TB][G*8][CT]
+  const int tid = threadIdx.x;
+  const int nthreads = blockDim.x;
+  const int CT = A.CT, TB = A.TB, NP = A.G * 8;
+  const int c0 = blockIdx.x * CT;
+  const int cl = tid % CT;
+  const int tile = tid / CT;
+  int ta = 0, tb = 0;
+  {
+    int t = tile, row = 0;
+    while (row < A.G && t >= A.G - row) { t -= A.G - row; row++; }
+    ta = row; tb = row + t;
+  }
+  const bool active = tile < A.ntiles;
+  float2 acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[i][k] = make_float2(0.f, 0.f);
+
+  for (int t0 = 0; t0 < A.nblocks; t0 += TB) {
+    const int nstage = TB * NP * CT;
+    for (int e = tid; e < nstage; e += nthreads) {
+      int c = e % CT;
+      int a = (e / CT) % NP;
+      int tt = e / (CT * NP);
+      float2 v = make_float2(0.f, 0.f);
+      if (a < A.nin && t0 + tt < A.nblocks && c0 + c < A.nchan)
+        v = load_spec(spec, A.in_off[a] + (long long)(t0 + tt) * A.block_stride + c0 + c);
+      sh[e] = v;
+    }
+    __syncthreads();
+    if (active) {
+      const int nt = min(TB, A.nblocks - t0);
+      for (int tt = 0; tt < nt; tt++) {
+        const float2* row = sh + (long long)tt * NP * CT;
+        float2 xb[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) xb[k] = row[(tb * 8 + k) * CT + cl];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          float2 xa = row[(ta * 8 + i) * CT + cl];
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            acc[i][k].x += xa.x * xb[k].x + xa.y * xb[k].y;
+            acc[i][k].y += xa.y * xb[k].x - xa.x * xb[k].y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!active || c0 + cl >= A.nchan) return;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int a = ta * 8 + i;
+    if (a >= A.nin) continue;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      int b = tb * 8 + k;
+      if (b >= A.nin) continue;
+      float2* p = A.V + ((long long)a * A.nin + b) * A.nchan + c0 + cl;
+      if (A.accumulate) { float2 o = *p; o.x += acc[i][k].x; o.y += acc[i][k].y; *p = o; }
+      else *p = acc[i][k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weights: one thread per baseline (sum over blocks of w1*w2) and per
+// datastream (sum of w), in block order
+// ---------------------------------------------------------------------------
+__global__ void k_weights(const DevBlock* __restrict__ blocks, int nblocks, const int2* __restrict__ bl,
+                          int nbl, int nds, float* __restrict__ wout) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbl) {
+    int d1 = bl[i].x, d2 = bl[i].y;
+    float acc = 0.0f;
+    for (int t = 0; t < nblocks; t++) {
+      const DevBlock& g1 = blocks[(long long)d1 * nblocks + t];
+      const DevBlock& g2 = blocks[(long long)d2 * nblocks + t];
+      float w1 = g1.status == 0 ? g1.weight : 0.0f;
+      float w2 = g2.status == 0 ? g2.weight : 0.0f;
+      acc += w1 * w2;
+    }
+    wout[i] = acc;
+  } else if (i < nbl + nds) {
+    int d = i - nbl;
+    float acc = 0.0f;
+    for (int t = 0; t < nblocks; t++) {
+      const DevBlock& g = blocks[(long long)d * nblocks + t];
+      acc += g.status == 0 ? g.weight : 0.0f;
+    }
+    wout[i] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scatter: result[seg.resoff + c] = V[seg.voff + c] (conjugated if flagged),
+// scaled by vscale; float segments copy one weight
+// ---------------------------------------------------------------------------
+__global__ void k_scatter(const DevSeg* __restrict__ segs, int nseg, const float2* __restrict__ V,
+                          const float* __restrict__ w, float vscale, float* __restrict__ res) {
+  int sidx = blockIdx.y;
+  if (sidx >= nseg) return;
+  const DevSeg g = segs[sidx];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g.kind == SEG_FLOATS) {
+    if (c == 0) res[g.resoff] = w[g.voff];
+    return;
+  }
+  if (c >= g.nchan) return;
+  float2 v = V[g.voff + c];
+  if (g.conj) v.y = -v.y;
+  res[2 * (g.resoff + c)] = v.x * vscale;
+  res[2 * (g.resoff + c) + 1] = v.y * vscale;
+}
+
+hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream) {
+  size_t shmem = (size_t)a.TB * a.G * 8 * a.CT * sizeof(float2);
+  dim3 grid((unsigned)((a.nchan + a.CT - 1) / a.CT));
+  int threads = a.CT * a.ntiles;
+  threads = ((threads + 63) / 64) * 64;
+  if (threads > 576) return hipErrorInvalidValue;
+  if (prec == FXG_XMAC_FP32) hipLaunchKernelGGL(k_xmac_f32<float2>, grid, dim3(threads), shmem, stream, a);
+  else if (prec == FXG_XMAC_FP16) hipLaunchKernelGGL(k_xmac_f32<__half2>, grid, dim3(threads), shmem, stream, a);
+  else hipLaunchKernelGGL(k_xmac_f32<__hip_bfloat162>, grid, dim3(threads), shmem, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
+                          hipStream_t stream) {
+  int n = nbl + nds;
+  hipLaunchKernelGGL(k_weights, dim3((n + 127) / 128), dim3(128), 0, stream, blocks, nblocks, bl, nbl, nds, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w, float vscale,
+                          float* res, hipStream_t stream) {
+  if (nseg == 0) return hipSuccess;
+  int blocks = (maxlen + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  // grid.y is limited to 65535: launch in slices
+  for (int s0 = 0; s0 < nseg; s0 += 65535) {
+    int n = nseg - s0 < 65535 ? nseg - s0 : 65535;
+    hipLaunchKernelGGL(k_scatter, dim3(blocks, n), dim3(256), 0, stream, segs + s0, n, V, w, vscale, res);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fxg

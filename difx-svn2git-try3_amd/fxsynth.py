@@ -1,0 +1,206 @@
+"""Seeded synthetic VDIF / Mark5B data for correlator jobs.
+
+Signal model (after the reference's simulator, applications/datasim/src/
+subband.cpp:580-662 and util.cpp:272-273): for every subband a common
+white Gaussian sky signal g, and per station and polarisation
+    x(t) = sqrt(rho) * g(t + tau_station) + sqrt(1 - rho) * n(t)
+with unit variance and independent station noise n.  Station delays are
+whole samples here, so the correlator's delay model can equal the data's
+delay exactly.  2-bit samples are quantised at +-0.98 sigma into codes
+0..3 (x < -0.98, [-0.98, 0), [0, 0.98), >= 0.98: the VDIF code order of
+format_vdif.c:78), 8-bit samples as clip(round(8 x) + 128).
+
+Frames are VDIF (32-byte header, word layout of libraries/vdifio/src/
+vdifio.h:42-74) or Mark5B (16-byte header + 10000-byte payload); samples are
+packed sample-time major, channel minor, LSB first.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from fxcorr import (FXG_FORMAT_MARK5B, FXG_FORMAT_VDIF, JobSpec, StationInput,
+                    SubintDesc, FXG_FLAGS_PER_INT)
+import ctypes as C
+
+
+@dataclass
+class StationData:
+    raw: np.ndarray          # uint8 frames
+    datasec: int
+    datans: int
+    validflags: np.ndarray   # int32
+    interpolator: np.ndarray  # float64[3]
+    delay_samples: int
+
+
+@dataclass
+class SubintData:
+    spec: JobSpec
+    sub_sec: int
+    sub_ns: int
+    scan_start_sec: int
+    stations: List[StationData]
+
+    def subint_desc(self) -> SubintDesc:
+        d = SubintDesc()
+        d.scan = 0
+        d.sec = self.sub_sec
+        d.ns = self.sub_ns
+        d.scan_start_sec = self.scan_start_sec
+        d.model_ok = 1
+        return d
+
+    def station_inputs(self, device_ptrs: Optional[Sequence[int]] = None):
+        """ctypes StationInput array (host data unless device_ptrs given)."""
+        arr = (StationInput * len(self.stations))()
+        keep = []
+        for i, st in enumerate(self.stations):
+            a = arr[i]
+            if device_ptrs is None:
+                a.data = st.raw.ctypes.data
+            else:
+                a.data = device_ptrs[i]
+            a.nbytes = st.raw.size
+            a.scan = 0
+            a.sec = st.datasec
+            a.ns = st.datans
+            vf = np.ascontiguousarray(st.validflags, dtype=np.int32)
+            keep.append(vf)
+            a.nvalidflags = vf.size
+            a.validflags = vf.ctypes.data_as(C.POINTER(C.c_int32))
+            for k in range(3):
+                a.interpolator[k] = float(st.interpolator[k])
+        return arr, keep
+
+
+def quantise2(x: np.ndarray) -> np.ndarray:
+    t = 0.98
+    return ((x >= -t).astype(np.uint8) + (x >= 0).astype(np.uint8) + (x >= t).astype(np.uint8))
+
+
+def quantise8(x: np.ndarray) -> np.ndarray:
+    return np.clip(np.rint(x * 8.0) + 128, 0, 255).astype(np.uint8)
+
+
+def pack_codes(codes: np.ndarray, nbits: int) -> np.ndarray:
+    """codes[T, nchan] -> bytes, sample-time major, channel minor, LSB first."""
+    flat = np.ascontiguousarray(codes, dtype=np.uint8).reshape(-1)
+    if nbits == 8:
+        return flat.copy()
+    if nbits == 2:
+        flat = flat.reshape(-1, 4)
+        return (flat[:, 0] | (flat[:, 1] << 2) | (flat[:, 2] << 4) | (flat[:, 3] << 6)).astype(np.uint8)
+    raise ValueError("nbits")
+
+
+def vdif_header(sec: int, frameno: int, framebytes: int, nchan: int, nbits: int,
+                station: int, invalid: bool = False) -> np.ndarray:
+    w = np.zeros(8, dtype=np.uint32)
+    w[0] = (sec & 0x3FFFFFFF) | ((1 << 31) if invalid else 0)
+    w[1] = (frameno & 0xFFFFFF) | (20 << 24)
+    w[2] = ((framebytes // 8) & 0xFFFFFF) | ((int(math.log2(nchan)) & 0x1F) << 24)
+    w[3] = (station & 0xFFFF) | (((nbits - 1) & 0x1F) << 26)
+    return w.view(np.uint8)
+
+
+def mark5b_header(frameno: int, invalid: bool = False) -> np.ndarray:
+    w = np.zeros(4, dtype=np.uint32)
+    w[0] = 0xABADDEED
+    w[1] = (frameno & 0x7FFF) | ((1 << 15) if invalid else 0)
+    return w.view(np.uint8)
+
+
+def frame_stream(spec: JobSpec, codes: np.ndarray, first_frame_sec: int, first_frameno: int,
+                 frames_per_sec: int, station: int, invalid_frames: Sequence[int] = ()) -> np.ndarray:
+    """Pack codes[T, nchan] (T a multiple of framesamples) into frames."""
+    payload = pack_codes(codes, spec.nbits)
+    databytes = spec.framebytes - spec.headerbytes
+    nframes = payload.size // databytes
+    payload = payload[: nframes * databytes].reshape(nframes, databytes)
+    out = np.empty((nframes, spec.framebytes), dtype=np.uint8)
+    out[:, spec.headerbytes:] = payload
+    inval = set(int(i) for i in invalid_frames)
+    for f in range(nframes):
+        fn = first_frameno + f
+        sec = first_frame_sec + fn // frames_per_sec
+        fno = fn % frames_per_sec
+        if spec.fmt == FXG_FORMAT_VDIF:
+            out[f, : spec.headerbytes] = vdif_header(sec, fno, spec.framebytes, spec.nrecbands,
+                                                     spec.nbits, station, f in inval)[: spec.headerbytes]
+        else:
+            out[f, :16] = mark5b_header(fno, f in inval)
+    return out.reshape(-1)
+
+
+def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.05,
+                delays_us: Optional[Sequence[float]] = None,
+                rates_us_per_s: Optional[Sequence[float]] = None,
+                sub_sec: int = 10, guard_frames: Optional[int] = None,
+                invalid_frames: Optional[dict] = None,
+                invalid_blocks: Optional[dict] = None,
+                extra_frames: int = 1) -> SubintData:
+    """Generate one subintegration of raw frames for every station.
+
+    delays_us: model delay per station (the data carry the whole-sample part
+    of it; any fractional part is then a model-vs-data offset).  rates add a
+    linear delay rate to the model only (exercises the fringe rotator).
+    """
+    rng = np.random.default_rng(seed)
+    ts = spec.sampletime_us
+    nst = spec.nstations
+    nsb, npol = spec.nsubbands, spec.npol
+    fs = spec.framesamples
+    frames_per_sec = int(round(1e6 / ts)) // fs
+    if delays_us is None:
+        delays_us = rng.uniform(-50.0, 50.0, nst) * min(1.0, spec.blockspersend * spec.fftchannels * ts / 1e4)
+    if rates_us_per_s is None:
+        rates_us_per_s = np.zeros(nst)
+    dsamp = [int(round(d / ts)) for d in delays_us]
+    maxd = max(abs(d) for d in dsamp) + 2 * spec.fftchannels
+    if guard_frames is None:
+        guard_frames = (maxd + fs - 1) // fs + 1
+    G = guard_frames * fs
+    nsamp_sub = spec.blockspersend * spec.fftchannels
+    nframes = (nsamp_sub + 2 * G + fs - 1) // fs + extra_frames
+    T = nframes * fs
+    rhos = np.broadcast_to(np.asarray(rho, dtype=np.float64), (nsb,))
+    # data start: G samples before the subint start
+    sub_ns = 0
+    start_ns_total = sub_sec * 1_000_000_000 - G * ts * 1000.0
+    if abs(start_ns_total - round(start_ns_total)) > 1e-6:
+        raise ValueError("guard does not land on an integer ns")
+    start_ns_total = int(round(start_ns_total))
+    datasec, datans = divmod(start_ns_total, 1_000_000_000)
+    first_frameno = int(round((datans / 1000.0) / ts)) // fs
+    lo = min(dsamp)
+    hi = max(dsamp)
+    stations = []
+    codes_all = [np.empty((T, spec.nrecbands), dtype=np.uint8) for _ in range(nst)]
+    for sb in range(nsb):
+        g = rng.standard_normal(T + hi - lo)
+        r = float(rhos[sb])
+        for s in range(nst):
+            gs = g[dsamp[s] - lo: dsamp[s] - lo + T]
+            for p in range(npol):
+                x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * rng.standard_normal(T)
+                q = quantise2(x) if spec.nbits == 2 else quantise8(x)
+                codes_all[s][:, sb * npol + p] = q
+    nflags = (spec.blockspersend + FXG_FLAGS_PER_INT - 1) // FXG_FLAGS_PER_INT
+    for s in range(nst):
+        inv = (invalid_frames or {}).get(s, ())
+        raw = frame_stream(spec, codes_all[s], datasec, first_frameno, frames_per_sec, s, inv)
+        vf = np.zeros(nflags, dtype=np.int64)
+        for b in range(spec.blockspersend):
+            if b in set((invalid_blocks or {}).get(s, ())):
+                continue
+            vf[b // FXG_FLAGS_PER_INT] |= 1 << (b % FXG_FLAGS_PER_INT)
+        blk_s = spec.fftchannels * ts * 1e-6
+        interp = np.array([0.0, rates_us_per_s[s] * blk_s, delays_us[s]], dtype=np.float64)
+        stations.append(StationData(raw=raw, datasec=int(datasec), datans=int(datans),
+                                    validflags=vf.astype(np.int32), interpolator=interp,
+                                    delay_samples=dsamp[s]))
+    return SubintData(spec=spec, sub_sec=sub_sec, sub_ns=sub_ns, scan_start_sec=0, stations=stations)

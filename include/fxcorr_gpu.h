@@ -1,0 +1,251 @@
+/*
+ * fxcorr_gpu.h — C ABI of the MI355X FX station-processing + cross-multiply
+ * library (libfxcorr_gpu.so).
+ *
+ * The library replaces, for one GPU, the hot path of DiFX mpifxcorr:
+ *   - Mk5Mode::unpack + mark5access decoders  (mpifxcorr/src/mk5mode.cpp:104-177,
+ *     libraries/mark5access/mark5access/format_vdif.c:768-2020)
+ *   - Mode::process per FFT block             (mpifxcorr/src/mode.cpp:649-1397)
+ *   - Core::processdata XMAC + weights        (mpifxcorr/src/core.cpp:667-1122)
+ *   - uvshiftAndAverage / averageAndSendAutocorrs copy into the slot result
+ *     buffer laid out by Configuration::populateResultLengths
+ *                                              (core.cpp:1424-1867, 1159-1369;
+ *                                               configuration.cpp:2212-2441)
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative FXG_E* code on error;
+ *     fxg_last_error() returns a thread-local message for the last failure.
+ *   - Handles are opaque.  Device memory is owned by the library.  Host pointers
+ *     passed in are borrowed for the duration of the call only.
+ *   - One host thread per job handle at a time.  Jobs on one context may be
+ *     driven from different host threads.
+ *   - All descriptor structs are plain fixed-size C structs (no nested
+ *     pointers other than the documented top-level arrays) so that any FFI
+ *     (ctypes, cgo, JNI) can build them.
+ */
+#ifndef FXCORR_GPU_H
+#define FXCORR_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FXG_ABI_VERSION 1
+
+/* ---- limits of the fixed-size descriptors ------------------------------ */
+#define FXG_MAX_FREQS        64   /* frequency table entries per job          */
+#define FXG_MAX_RECFREQS     32   /* recorded freqs per datastream            */
+#define FXG_MAX_BANDS        64   /* recorded bands per datastream            */
+#define FXG_MAX_ZOOMBANDS    32   /* zoom bands per datastream                */
+#define FXG_MAX_BLFREQS      48   /* frequencies per baseline                 */
+#define FXG_MAX_POLPRODUCTS   4   /* pol products per baseline frequency      */
+
+/* ---- error codes ------------------------------------------------------- */
+#define FXG_OK             0
+#define FXG_EINVAL        -1   /* bad argument / descriptor                  */
+#define FXG_ENOMEM        -2   /* device or host allocation failed           */
+#define FXG_EHIP          -3   /* HIP runtime error                          */
+#define FXG_EUNSUPPORTED  -4   /* valid for the reference, not built here    */
+#define FXG_ESTATE        -5   /* call out of order (e.g. wait w/o submit)   */
+#define FXG_ENODEV        -6   /* no usable gfx950 device                    */
+
+/* ---- enums -------------------------------------------------------------- */
+/* Data formats handled by the Mk5Mode path (configuration.cpp:991-1011). */
+#define FXG_FORMAT_VDIF     0   /* VDIF, 32-byte (or 16-byte legacy) header */
+#define FXG_FORMAT_MARK5B   1   /* Mark5B, 16-byte header, 10000-byte payload */
+
+/* Blanker selection (mark5access): the VDIF default blanks frames whose
+ * payload starts or ends with the fill pattern (blanker_mark5.c:282-312);
+ * network datastreams use blanker_none (mk5mode.cpp:64-65). */
+#define FXG_BLANKER_FORMAT  0
+#define FXG_BLANKER_NONE    1
+
+/* Cross-multiply arithmetic. */
+#define FXG_XMAC_FP32       0   /* plain drop-in: cf32 spectra, FP32 FMA     */
+#define FXG_XMAC_FP16       1   /* fp16 spectra, MFMA, FP32 accumulation     */
+#define FXG_XMAC_BF16       2   /* bf16 spectra, MFMA, FP32 accumulation     */
+
+/* Sentinel used by Mode for a torched subintegration (mode.h:232). */
+#define FXG_INVALID_SUBINT  (-99999999)
+
+/* Number of valid-flag bits per int32 (mpifxcorr.h:39). */
+#define FXG_FLAGS_PER_INT   30
+
+/* ---- frequency table entry (configuration.h freqdata) ------------------ */
+typedef struct {
+  double  freq_mhz;        /* band-edge sky frequency (= LO for fringe rotation) */
+  double  bw_mhz;          /* bandwidth                                         */
+  int32_t lsb;             /* 1 = lower sideband                                */
+  int32_t nchan;           /* output channels (recordedbandchannels)            */
+  int32_t chans_to_avg;    /* must be 1 in this build                           */
+  int32_t pad;
+} fxg_freq;
+
+/* ---- datastream (station) table entry (configuration.h datastreamdata) -- */
+typedef struct {
+  int32_t format;          /* FXG_FORMAT_*                                      */
+  int32_t nbits;           /* 2 or 8 (VDIF); 2 (Mark5B)                         */
+  int32_t framebytes;      /* header + payload bytes of one frame               */
+  int32_t headerbytes;     /* 32 or 16 (VDIF legacy); 16 (Mark5B)               */
+  int32_t blanker;         /* FXG_BLANKER_*                                     */
+  int32_t nrecfreqs;       /* recorded frequencies                              */
+  int32_t nrecbands;       /* recorded bands = channels in the stream           */
+  int32_t nzoombands;      /* zoom bands (aliases of recorded band spectra)     */
+  /* per recorded frequency (index = local freq index) */
+  int32_t recfreq_index[FXG_MAX_RECFREQS];       /* freq table index          */
+  double  recfreq_clockoffset_us[FXG_MAX_RECFREQS];
+  double  recfreq_clockoffsetdelta_us[FXG_MAX_RECFREQS];
+  double  recfreq_phaseoffset_deg[FXG_MAX_RECFREQS];
+  double  recfreq_looffset_hz[FXG_MAX_RECFREQS];
+  /* per recorded band: local freq index and polarisation ('R','L','X','Y') */
+  int32_t recband_freq[FXG_MAX_BANDS];
+  char    recband_pol[FXG_MAX_BANDS];
+  /* per zoom band: freq table index, polarisation, parent recorded band and
+   * channel offset into the parent spectrum (mode.cpp:194-206) */
+  int32_t zoomband_freqindex[FXG_MAX_ZOOMBANDS];
+  char    zoomband_pol[FXG_MAX_ZOOMBANDS];
+  int32_t zoomband_parent[FXG_MAX_ZOOMBANDS];
+  int32_t zoomband_choffset[FXG_MAX_ZOOMBANDS];
+} fxg_datastream;
+
+/* ---- baseline table entry (configuration.h baselinedata) ---------------- */
+/* Band indices are datastream "total" band indices: recorded bands first,
+ * then zoom bands at nrecbands + z (configuration.cpp:1100-1124). */
+typedef struct {
+  int32_t ds1;             /* datastream index of the first antenna             */
+  int32_t ds2;             /* datastream index of the second antenna            */
+  int32_t nfreqs;
+  int32_t freq_index[FXG_MAX_BLFREQS];           /* freq table index           */
+  int32_t npolproducts[FXG_MAX_BLFREQS];
+  int32_t band1[FXG_MAX_BLFREQS][FXG_MAX_POLPRODUCTS];
+  int32_t band2[FXG_MAX_BLFREQS][FXG_MAX_POLPRODUCTS];
+} fxg_baseline;
+
+/* ---- job (one correlator configuration) -------------------------------- */
+typedef struct {
+  int32_t nfreqs;
+  int32_t ndatastreams;
+  int32_t nbaselines;
+  int32_t blockspersend;   /* FFT blocks per subintegration                     */
+  int32_t fringe_order;    /* 1 (linear, the vex2difx default) — 0/2 not built  */
+  int32_t arraystridelen;  /* 0 = automatic (configuration.cpp:46-59)           */
+  int32_t xmacstridelen;   /* 0 = automatic (configuration.cpp:2464-2511)       */
+  int32_t crosspol_autocorrs; /* compute and output cross-pol autocorrelations  */
+  int32_t xmac_precision;  /* FXG_XMAC_*                                        */
+  int32_t max_chunk_blocks;/* blocks of spectra kept in HBM at once (0 = auto)  */
+  const fxg_freq*       freqs;        /* [nfreqs]        */
+  const fxg_datastream* datastreams;  /* [ndatastreams]  */
+  const fxg_baseline*   baselines;    /* [nbaselines]    */
+} fxg_job_desc;
+
+/* ---- per-subintegration inputs (Core::processdata, core.cpp:705-730) ---- */
+typedef struct {
+  const uint8_t* data;     /* raw frames (host or device pointer, see call)     */
+  int64_t nbytes;          /* datalengthbytes                                   */
+  int32_t scan;            /* data time: controlbuffer[0..2]                    */
+  int32_t sec;
+  int32_t ns;
+  int32_t nvalidflags;     /* ceil(blockspersend/30)                            */
+  const int32_t* validflags;   /* host pointer, bit b%30 of word b/30 = block b valid */
+  double interpolator[3];  /* quadratic delay model (µs vs block index), after
+                              the intclockseconds subtraction (mode.cpp:1548-1549) */
+} fxg_station_input;
+
+typedef struct {
+  int32_t scan;            /* procslots[].offsets[0..2]                         */
+  int32_t sec;
+  int32_t ns;
+  int32_t scan_start_sec;  /* Model::getScanStartSec(scan, startmjd, startsec)  */
+  int32_t model_ok;        /* 0 = no delay model found: subint torched          */
+  int32_t pad;
+} fxg_subint_desc;
+
+/* ---- per-block station geometry (debug tap; Mode::process prologue) ----- */
+typedef struct {
+  int32_t status;          /* 0 ok, 1 flagged invalid, 2 out of range, 3 zero weight */
+  int32_t nearestsample;
+  int32_t integerdelay;
+  float   weight;          /* dataweight                                        */
+  float   fracsampleerror; /* µs                                                */
+  int32_t intwalltime;
+  double  a;               /* order-1 delay slope over the block (µs)           */
+  double  b;               /* delay at block start minus integerdelay (µs)      */
+  double  fracwalltime;
+} fxg_block_info;
+
+/* ---- result layout (Configuration::populateResultLengths) --------------- */
+typedef struct {
+  int64_t result_length;        /* cf32 elements of the core result buffer     */
+  int64_t device_bytes;         /* HBM footprint of one job (admission)         */
+  int32_t arraystridelen;
+  int32_t xmacstridelen;
+  int32_t rotatestridelen;
+  int32_t bandsperautocorr;
+  int32_t fftchannels;          /* L = 2 * nchan for real sampling             */
+  int32_t chunk_blocks;         /* blocks of spectra resident per pass          */
+} fxg_plan_info;
+
+/* ---- library ------------------------------------------------------------ */
+int         fxg_abi_version(void);
+const char* fxg_last_error(void);
+
+/* ---- host-only planning: no GPU needed ---------------------------------- */
+/* Validates the descriptor and computes the layout and HBM footprint. */
+int fxg_plan(const fxg_job_desc* desc, fxg_plan_info* info);
+/* Offsets into the cf32 result buffer (-1 if the entry does not exist):
+ *   configuration.h getCoreResultBaselineOffset / BWeightOffset /
+ *   AutocorrOffset / ACWeightOffset (configuration.cpp:2322-2433). */
+int fxg_plan_baseline_offset(const fxg_job_desc* desc, int freq, int baseline, int64_t* off);
+int fxg_plan_bweight_offset(const fxg_job_desc* desc, int freq, int baseline, int64_t* off);
+int fxg_plan_autocorr_offset(const fxg_job_desc* desc, int datastream, int64_t* off);
+int fxg_plan_acweight_offset(const fxg_job_desc* desc, int datastream, int64_t* off);
+
+/* ---- context / device ---------------------------------------------------- */
+typedef struct fxg_ctx fxg_ctx;
+typedef struct fxg_job fxg_job;
+
+int fxg_ctx_create(int device, fxg_ctx** ctx);
+int fxg_ctx_destroy(fxg_ctx* ctx);
+/* Device allocations for callers that keep their raw data resident. */
+int fxg_device_alloc(fxg_ctx* ctx, int64_t bytes, void** dptr);
+int fxg_device_free(fxg_ctx* ctx, void* dptr);
+int fxg_memcpy_h2d(fxg_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int fxg_memcpy_d2h(fxg_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
+/* ---- standalone unpack (Mk5Mode::unpack, mk5mode.cpp:104-177) ----------- */
+/* Decodes nsamp sample-times starting at sample offset `sampleoffset` of the
+ * frame stream `data` (device pointer, nbytes long) into out[nrecbands][nsamp]
+ * (device pointer, band-major floats).  Returns the good-sample weight
+ * goodsamples/nsamp in *weight, with the mark5access blanking rules. */
+int fxg_unpack(fxg_ctx* ctx, const fxg_datastream* ds, const uint8_t* data,
+               int64_t nbytes, int64_t sampleoffset, int32_t nsamp,
+               float* out, float* weight);
+
+/* ---- jobs ----------------------------------------------------------------- */
+int fxg_job_create(fxg_ctx* ctx, const fxg_job_desc* desc, fxg_job** job);
+int fxg_job_destroy(fxg_job* job);
+int fxg_job_plan_info(const fxg_job* job, fxg_plan_info* info);
+
+/* Process one subintegration: stations[ndatastreams].  If data_on_device is
+ * nonzero, stations[i].data are device pointers that stay valid until the call
+ * returns; otherwise they are host pointers copied in by the call.  results
+ * receives result_length cf32 values (interleaved re,im floats), zeroed and
+ * accumulated exactly as a Core slot (core.cpp:319, 1048-1108); if
+ * results_on_device is nonzero it is a device pointer. */
+int fxg_subint_process(fxg_job* job, const fxg_subint_desc* sub,
+                       const fxg_station_input* stations, int data_on_device,
+                       float* results, int results_on_device);
+
+/* Debug taps valid after fxg_subint_process with keep_taps set. */
+int fxg_job_set_keep_taps(fxg_job* job, int keep);
+int fxg_tap_block_info(fxg_job* job, int datastream, int block, fxg_block_info* out);
+/* Spectrum of one recorded band for one block (Mode::getFreqs, mode.h:199):
+ * nchan cf32 values as FP32 (converted from the job's spectrum precision). */
+int fxg_tap_spectrum(fxg_job* job, int datastream, int band, int block, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FXCORR_GPU_H */

@@ -1,0 +1,953 @@
+/*
+ * fxoracle.cpp — CPU restatement of the DiFX mpifxcorr FX hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path (libfxcorr_gpu.so) never links or calls it.
+ *
+ * It restates, with GENERIC-architecture semantics (architecture.h.in:404-1075:
+ * libm sinf/cosf, unnormalised forward FFT, truncating int casts, no FMA
+ * contraction), the following reference code:
+ *
+ *   unpack   libraries/mark5access/mark5access/format_vdif.c:74-149 (LUTs),
+ *            :768-1303 (2-bit decoders), :1837-2020 (8-bit decoders),
+ *            :3840-3848 (granularity), :4009-4066 (validate);
+ *            mark5_format_mark5b.c:59-96 (LUT), :1579-1950 (2-bit decoders);
+ *            mark5_stream_unpacker.c:129-148 (unpack_with_offset);
+ *            mark5_stream.c:153-202 (next_frame); blanker_mark5.c:53-165,282-312;
+ *            mpifxcorr/src/mk5mode.cpp:104-177 (Mk5Mode::unpack)
+ *   station  mpifxcorr/src/mode.cpp:43-462 (constructor tables), :649-1397
+ *            (Mode::process, fringe order 1, real sampling)
+ *   model    mpifxcorr/src/model.cpp:229-304 (calculateDelayInterpolator)
+ *   xmac     mpifxcorr/src/core.cpp:667-1122 (processdata), :1159-1369
+ *            (averageAndSendAutocorrs), :1586-1867 (uvshiftAndAverage, single
+ *            phase centre, no channel averaging)
+ *   layout   mpifxcorr/src/configuration.cpp:46-59, 2212-2441, 2464-2546
+ *
+ * Two arithmetic flavours share one code path through a template parameter:
+ *   float  — the reference's own FP32 arithmetic (the drop-in parity target);
+ *   double — the same algorithm in FP64 end to end (the error yardstick for
+ *            the reduced-precision cross-multiply).
+ * The FP64 block geometry (delays, nearest sample, integer delay) is the same
+ * FP64 expression sequence in both.
+ */
+#include "../include/fxcorr_gpu.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+#include <complex>
+#include <algorithm>
+#include <memory>
+
+namespace orc {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559; /* architecture.h.in:435 */
+constexpr double TINY = 0.000000001;                          /* mode.cpp:35 */
+constexpr float HIMAG = 3.3359f;                              /* mark5_stream.h:75 */
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) { g_err = msg; return code; }
+
+/* ------------------------------------------------------------------------ */
+/* Lookup tables                                                            */
+/* ------------------------------------------------------------------------ */
+/* VDIF 2-bit: code k -> {-HiMag, -1, +1, +HiMag}[k] (format_vdif.c:78,104-107) */
+static void lut_vdif2(float* t) { t[0] = -HIMAG; t[1] = -1.0f; t[2] = 1.0f; t[3] = HIMAG; }
+/* VDIF 8-bit: (b-128)/3.3 evaluated in double, stored as float (format_vdif.c:126) */
+static void lut_vdif8(float* t) { for (int b = 0; b < 256; b++) t[b] = (b - 128) / 3.3; }
+/* Mark5B 2-bit: sign bit at even position, magnitude at odd, table
+ * {-HiMag, 1, -1, HiMag}[s + 2m] (mark5_format_mark5b.c:63,80-83) */
+static void lut_mark5b2(float* t) { t[0] = -HIMAG; t[1] = 1.0f; t[2] = -1.0f; t[3] = HIMAG; }
+
+/* ------------------------------------------------------------------------ */
+/* Stride rules (configuration.cpp:46-59, 2464-2511)                          */
+/* ------------------------------------------------------------------------ */
+static int calcstridelength(unsigned int arraylength) {
+  if (arraylength <= 16) return (int)arraylength;
+  for (unsigned int guess = (unsigned int)std::sqrt(arraylength + 0.1);; --guess)
+    if (arraylength % guess == 0) return (int)(arraylength / guess);
+}
+static long gcdl(long a, long b) { while (b) { long t = a % b; a = b; b = t; } return a; }
+
+/* ------------------------------------------------------------------------ */
+/* Frame-stream decoder (mark5access restated)                              */
+/* ------------------------------------------------------------------------ */
+struct StreamFmt {
+  int format, nbits, nchan, framebytes, headerbytes, databytes, blanker;
+  int framesamples;      /* sample-times per frame */
+  int granularity;       /* samplegranularity (format_vdif.c:3845; mark5b.c:2837) */
+  int groupbytes;        /* bytes consumed per decoder iteration */
+  int groupsamples;      /* sample-times produced per decoder iteration */
+  bool noadvance_quirk;  /* 2/3/4-channel 8-bit VDIF decoders do not advance on blank */
+  float lut[256];
+};
+
+static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
+  f->format = ds->format; f->nbits = ds->nbits; f->nchan = ds->nrecbands;
+  f->framebytes = ds->framebytes; f->headerbytes = ds->headerbytes;
+  f->databytes = ds->framebytes - ds->headerbytes; f->blanker = ds->blanker;
+  if (f->nchan <= 0 || (f->nchan & (f->nchan - 1)) != 0)
+    return fail(FXG_EUNSUPPORTED, "nrecbands must be a power of two");
+  int bitsper = f->nchan * f->nbits;
+  if (ds->format == FXG_FORMAT_VDIF) {
+    if (f->nbits == 2) lut_vdif2(f->lut);
+    else if (f->nbits == 8) lut_vdif8(f->lut);
+    else return fail(FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
+  } else if (ds->format == FXG_FORMAT_MARK5B) {
+    if (f->nbits != 2) return fail(FXG_EUNSUPPORTED, "Mark5B nbits must be 2");
+    if (f->databytes != 10000 || f->headerbytes != 16)
+      return fail(FXG_EINVAL, "Mark5B frames are 16 + 10000 bytes");
+    if (bitsper > 32) return fail(FXG_EUNSUPPORTED, "Mark5B bitstreams <= 32");
+    lut_mark5b2(f->lut);
+  } else return fail(FXG_EINVAL, "unknown format");
+  f->granularity = bitsper >= 8 ? 1 : 8 / bitsper;
+  f->framesamples = (int)((int64_t)f->databytes * 8 / bitsper);
+  f->groupbytes = bitsper >= 8 ? bitsper / 8 : 1;
+  f->groupsamples = bitsper >= 8 ? 1 : 8 / bitsper;
+  f->noadvance_quirk = (ds->format == FXG_FORMAT_VDIF && f->nbits == 8 &&
+                        f->nchan >= 2 && f->nchan <= 4);
+  return 0;
+}
+
+/* little-endian 64-bit fill word 0x1122334411223344 (blanker_mark5.c:42-46) */
+static const uint64_t FILL64 = 0x1122334411223344ULL;
+static const uint32_t FILL32 = 0x11223344UL;
+static uint64_t rd64(const uint8_t* p) { uint64_t v; std::memcpy(&v, p, 8); return v; }
+static uint32_t rd32(const uint8_t* p) { uint32_t v; std::memcpy(&v, p, 4); return v; }
+
+/* binary searches of blanker_mark5.c:53-123 (word indices relative to payload) */
+static int findfirstinvalid(const uint8_t* pl, int start, int end) {
+  int n = end - start, i = n / 2, r = n - 1;
+  for (int s = n / 4; s > 0; s >>= 1) {
+    if (rd64(pl + 8 * (size_t)(start + i)) == FILL64) { r = i; i -= s; } else i += s;
+  }
+  r = 8 * (start + r);
+  if (rd32(pl + 4 * (size_t)(r / 4 - 1)) == FILL32) r -= 4;
+  return r;
+}
+static int findfirstvalid(const uint8_t* pl, int start, int end) {
+  int n = end - start, i = n / 2, r = n - 1;
+  for (int s = n / 4; s > 0; s >>= 1) {
+    if (rd64(pl + 8 * (size_t)(start + i)) != FILL64) { r = i; i -= s; } else i += s;
+  }
+  r = 8 * (start + r);
+  if (rd32(pl + 4 * (size_t)(r / 4 - 2)) == FILL32) r += 4;
+  return r;
+}
+
+struct FrameState {
+  const uint8_t* frame;   /* null when the frame is not inside the buffer */
+  int startvalid, endvalid;
+};
+
+/* mark5_stream_next_frame (mark5_stream.c:153-202) for the memory unpacker:
+ * validate, then blank (whole frame) or run the blanker. */
+static void load_frame(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
+                       int64_t frameidx, FrameState* st) {
+  int64_t start = frameidx * (int64_t)f.framebytes;
+  if (frameidx < 0 || start + f.framebytes > nbytes) {
+    /* outside the buffer: the reference reads past its buffer here; this
+     * restatement defines such frames as invalid */
+    st->frame = nullptr; st->startvalid = 1 << 30; st->endvalid = 0; return;
+  }
+  const uint8_t* fr = data + start;
+  st->frame = fr;
+  bool valid = true;
+  if (f.format == FXG_FORMAT_VDIF) {
+    /* mark5_format_vdif_validate (format_vdif.c:4009-4060); the time check is
+     * skipped because the unpacker stream has mjd == 0 (mark5_stream.c:301) */
+    uint32_t w0 = rd32(fr), w2 = rd32(fr + 8);
+    if (w2 == 0) valid = false;
+    if ((w0 >> 31) & 1u) valid = false;
+  }
+  if (!valid) { st->startvalid = 1 << 30; st->endvalid = 0; return; }
+  const uint8_t* pl = fr + f.headerbytes;
+  int nword = f.databytes / 8;
+  if (f.blanker == FXG_BLANKER_NONE) {            /* blanker_none.c:36-43 */
+    st->startvalid = 0; st->endvalid = 1 << 30;
+  } else if (f.format == FXG_FORMAT_VDIF) {        /* blanker_vdif, blanker_mark5.c:282-312 */
+    st->startvalid = 0;
+    if (rd64(pl) == FILL64 || rd64(pl + 8 * (size_t)(nword - 1)) == FILL64) st->endvalid = 0;
+    else st->endvalid = 1 << 30;
+  } else {                                         /* blanker_mark5 zone 0 (one zone: 10000 B < 32 KiB) */
+    int b = 0, e = nword;
+    bool startOK = rd64(pl + 8 * (size_t)b) != FILL64;
+    bool endOK = rd64(pl + 8 * (size_t)(e - 1)) != FILL64;
+    if (startOK && endOK) { st->startvalid = 0; st->endvalid = 1 << 30; }
+    else if (!startOK && !endOK) { st->startvalid = 1 << 30; st->endvalid = 0; }
+    else if (startOK) { st->startvalid = 0; st->endvalid = findfirstinvalid(pl, b, e); }
+    else { st->startvalid = findfirstvalid(pl, b, e); st->endvalid = 1 << 30; }
+  }
+}
+
+/* mark5_unpack_with_offset + the format decoder loop.  out[c*nsamp + o].
+ * Returns goodsamples (the decoder's return value). */
+static int decode(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
+                  int64_t offsetsamples, int nsamp, float* out) {
+  int64_t frameidx = offsetsamples / f.framesamples;
+  FrameState st; load_frame(f, data, nbytes, frameidx, &st);
+  int i = (int)((offsetsamples % f.framesamples) * f.nchan * f.nbits / 8);
+  int nblank = 0;
+  const uint32_t mask = (1u << f.nbits) - 1u;
+  for (int o = 0; o < nsamp; o += f.groupsamples) {
+    bool blank = (i < st.startvalid) || (i >= st.endvalid) || (st.frame == nullptr);
+    if (!blank && f.format == FXG_FORMAT_MARK5B && (st.frame[f.headerbytes - 11] & 0x80)) blank = true;
+    if (blank) {
+      nblank++;
+      for (int k = 0; k < f.groupsamples; k++)
+        for (int c = 0; c < f.nchan; c++)
+          if (o + k < nsamp) out[(size_t)c * nsamp + o + k] = 0.0f;
+      if (!f.noadvance_quirk) i += f.groupbytes;
+    } else {
+      const uint8_t* g = st.frame + f.headerbytes + i;
+      for (int k = 0; k < f.groupsamples; k++)
+        for (int c = 0; c < f.nchan; c++) {
+          int bit = (k * f.nchan + c) * f.nbits;
+          uint32_t code = (g[bit >> 3] >> (bit & 7)) & mask;
+          if (o + k < nsamp) out[(size_t)c * nsamp + o + k] = f.lut[code];
+        }
+      i += f.groupbytes;
+    }
+    if (i >= f.databytes) { frameidx++; load_frame(f, data, nbytes, frameidx, &st); i = 0; }
+  }
+  return nsamp - f.groupsamples * nblank;
+}
+
+/* Mk5Mode::unpack (mk5mode.cpp:104-177) for real sampling.  `work` holds
+ * nchan*(unpacksamples+granularity) floats; on return the L samples starting
+ * at sampleoffset for channel c are at work[c*stride + munged + n]. */
+struct Unpacked { int stride; int munged; float weight; };
+static Unpacked mk5_unpack(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
+                           int64_t sampleoffset, int unpacksamples, std::vector<float>& work) {
+  int samplestounpack = unpacksamples + (f.granularity > 1 ? f.granularity : 0);
+  work.assign((size_t)f.nchan * samplestounpack, 0.0f);
+  int64_t unpackstart = sampleoffset - (sampleoffset % f.granularity);
+  float goodsamples = (float)decode(f, data, nbytes, unpackstart, samplestounpack, work.data());
+  int munged = 0;
+  if (f.granularity > 1) {
+    int erased = 0;
+    munged = (int)(sampleoffset % f.granularity);
+    for (int i = 0; i < munged; i++)
+      for (int b = 0; b < f.nchan; b++) {
+        float& v = work[(size_t)b * samplestounpack + i];
+        if (v != 0.0f) { v = 0.0f; erased++; }
+      }
+    for (int i = unpacksamples + munged; i < samplestounpack; i++)
+      for (int b = 0; b < f.nchan; b++) {
+        float& v = work[(size_t)b * samplestounpack + i];
+        if (v != 0.0f) { v = 0.0f; erased++; }
+      }
+    goodsamples -= erased / (float)f.nchan;
+  }
+  if (goodsamples < 0) goodsamples = 0;
+  Unpacked u; u.stride = samplestounpack; u.munged = munged;
+  u.weight = goodsamples / (float)unpacksamples;
+  return u;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT: iterative radix-2, forward, unnormalised, accurate twiddles         */
+/* ------------------------------------------------------------------------ */
+template <typename T> struct FFT {
+  int n = 0, logn = 0;
+  std::vector<int> rev;
+  std::vector<std::complex<T>> tw;
+  explicit FFT(int L) : n(L) {
+    while ((1 << logn) < n) logn++;
+    rev.resize(n);
+    for (int i = 0; i < n; i++) { int r = 0; for (int b = 0; b < logn; b++) if (i & (1 << b)) r |= 1 << (logn - 1 - b); rev[i] = r; }
+    tw.resize(n / 2);
+    for (int k = 0; k < n / 2; k++) {
+      double ang = -TWO_PI * (double)k / (double)n;
+      tw[k] = std::complex<T>((T)std::cos(ang), (T)std::sin(ang));
+    }
+  }
+  void run(std::complex<T>* a) const {
+    for (int i = 0; i < n; i++) if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+    for (int len = 2; len <= n; len <<= 1) {
+      int half = len >> 1, step = n / len;
+      for (int i = 0; i < n; i += len)
+        for (int j = 0; j < half; j++) {
+          std::complex<T> w = tw[(size_t)j * step];
+          std::complex<T> u = a[i + j];
+          std::complex<T> x = a[i + j + half];
+          T vr = x.real() * w.real() - x.imag() * w.imag();
+          T vi = x.real() * w.imag() + x.imag() * w.real();
+          a[i + j] = std::complex<T>(u.real() + vr, u.imag() + vi);
+          a[i + j + half] = std::complex<T>(u.real() - vr, u.imag() - vi);
+        }
+    }
+  }
+};
+
+/* sin/cos in the flavour's precision: libm sinf/cosf for float (GENERIC
+ * vectorSinCos_f32, architecture.h.in:~950) and sin/cos for double. */
+static inline void sincos_t(float x, float* s, float* c) { *s = sinf(x); *c = cosf(x); }
+static inline void sincos_t(double x, double* s, double* c) { *s = std::sin(x); *c = std::cos(x); }
+
+/* complex helpers written out so the operation order is that of the generic
+ * vector routines (architecture.h.in:580-586, 673-719) */
+template <typename T> struct cpx { T re, im; };
+template <typename T> static inline cpx<T> cmul(cpx<T> a, cpx<T> v) {
+  cpx<T> d; d.re = a.re * v.re - a.im * v.im; d.im = a.re * v.im + a.im * v.re; return d;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Job geometry                                                              */
+/* ------------------------------------------------------------------------ */
+struct DsGeom {
+  StreamFmt fmt;
+  int C, L, s, numfrstrides, nfracsteps;
+  double bw, sampletime;          /* MHz, µs */
+  bool fractionalLoFreq, deltapoloffsets;
+  int bytesperblocknum, bytesperblockden, samplesperblock;
+};
+
+struct Layout {
+  int nfreqs, nds, nbl;
+  std::vector<char> freqused, equivused;
+  std::vector<std::vector<int64_t>> bloff, bwoff;  /* [freq][baseline], -1 if absent */
+  std::vector<int64_t> acoff, acwoff, pcaloff;
+  std::vector<std::vector<int>> blfreq_local;      /* [baseline][freq] -> local index or -1 */
+  int64_t total = 0;
+  int bandsperautocorr = 1;
+  int xmacstridelen = 0, rotatestridelen = 0;
+  std::vector<int> arraystridelen;
+};
+
+static int ds_band_freqindex(const fxg_datastream& d, int band) {
+  if (band < d.nrecbands) return d.recfreq_index[d.recband_freq[band]];
+  return d.zoomband_freqindex[band - d.nrecbands];
+}
+
+static int validate(const fxg_job_desc* j) {
+  if (!j || !j->freqs || !j->datastreams || (j->nbaselines > 0 && !j->baselines))
+    return fail(FXG_EINVAL, "null descriptor");
+  if (j->nfreqs <= 0 || j->nfreqs > FXG_MAX_FREQS) return fail(FXG_EINVAL, "nfreqs");
+  if (j->ndatastreams <= 0) return fail(FXG_EINVAL, "ndatastreams");
+  if (j->blockspersend <= 0) return fail(FXG_EINVAL, "blockspersend");
+  if (j->fringe_order != 1) return fail(FXG_EUNSUPPORTED, "only fringe rotation order 1");
+  for (int f = 0; f < j->nfreqs; f++) {
+    if (j->freqs[f].chans_to_avg != 1) return fail(FXG_EUNSUPPORTED, "chans_to_avg must be 1");
+    if (j->freqs[f].nchan <= 0) return fail(FXG_EINVAL, "nchan");
+  }
+  for (int d = 0; d < j->ndatastreams; d++) {
+    const fxg_datastream& ds = j->datastreams[d];
+    if (ds.nrecfreqs <= 0 || ds.nrecfreqs > FXG_MAX_RECFREQS) return fail(FXG_EINVAL, "nrecfreqs");
+    if (ds.nrecbands <= 0 || ds.nrecbands > FXG_MAX_BANDS) return fail(FXG_EINVAL, "nrecbands");
+    if (ds.nzoombands < 0 || ds.nzoombands > FXG_MAX_ZOOMBANDS) return fail(FXG_EINVAL, "nzoombands");
+    for (int i = 0; i < ds.nrecfreqs; i++)
+      if (ds.recfreq_index[i] < 0 || ds.recfreq_index[i] >= j->nfreqs) return fail(FXG_EINVAL, "recfreq_index");
+    for (int b = 0; b < ds.nrecbands; b++)
+      if (ds.recband_freq[b] < 0 || ds.recband_freq[b] >= ds.nrecfreqs) return fail(FXG_EINVAL, "recband_freq");
+    for (int z = 0; z < ds.nzoombands; z++) {
+      if (ds.zoomband_freqindex[z] < 0 || ds.zoomband_freqindex[z] >= j->nfreqs) return fail(FXG_EINVAL, "zoomband_freqindex");
+      if (ds.zoomband_parent[z] < 0 || ds.zoomband_parent[z] >= ds.nrecbands) return fail(FXG_EINVAL, "zoomband_parent");
+    }
+    const fxg_freq& f0 = j->freqs[ds.recfreq_index[0]];
+    for (int i = 1; i < ds.nrecfreqs; i++) {
+      const fxg_freq& fi = j->freqs[ds.recfreq_index[i]];
+      if (fi.nchan != f0.nchan || fi.bw_mhz != f0.bw_mhz) return fail(FXG_EUNSUPPORTED, "recorded freqs of a datastream must share nchan and bandwidth");
+    }
+  }
+  for (int b = 0; b < j->nbaselines; b++) {
+    const fxg_baseline& bl = j->baselines[b];
+    if (bl.ds1 < 0 || bl.ds1 >= j->ndatastreams || bl.ds2 < 0 || bl.ds2 >= j->ndatastreams) return fail(FXG_EINVAL, "baseline datastream");
+    if (bl.nfreqs < 0 || bl.nfreqs > FXG_MAX_BLFREQS) return fail(FXG_EINVAL, "baseline nfreqs");
+    for (int f = 0; f < bl.nfreqs; f++) {
+      if (bl.freq_index[f] < 0 || bl.freq_index[f] >= j->nfreqs) return fail(FXG_EINVAL, "baseline freq_index");
+      if (bl.npolproducts[f] <= 0 || bl.npolproducts[f] > FXG_MAX_POLPRODUCTS) return fail(FXG_EINVAL, "npolproducts");
+      const fxg_datastream& d1 = j->datastreams[bl.ds1];
+      const fxg_datastream& d2 = j->datastreams[bl.ds2];
+      for (int p = 0; p < bl.npolproducts[f]; p++) {
+        int b1 = bl.band1[f][p], b2 = bl.band2[f][p];
+        if (b1 < 0 || b1 >= d1.nrecbands + d1.nzoombands || b2 < 0 || b2 >= d2.nrecbands + d2.nzoombands) return fail(FXG_EINVAL, "baseline band index");
+        if (ds_band_freqindex(d1, b1) != bl.freq_index[f] || ds_band_freqindex(d2, b2) != bl.freq_index[f]) return fail(FXG_EINVAL, "baseline band frequency mismatch");
+      }
+    }
+  }
+  return 0;
+}
+
+/* populateResultLengths for one config, single phase centre, no pulsar
+ * binning (configuration.cpp:2212-2441), plus the used / equivalent-used
+ * frequency flags (configuration.cpp:325-367) and stride selection. */
+static int make_layout(const fxg_job_desc* j, Layout* L) {
+  int r = validate(j); if (r) return r;
+  L->nfreqs = j->nfreqs; L->nds = j->ndatastreams; L->nbl = j->nbaselines;
+  L->freqused.assign(j->nfreqs, 0); L->equivused.assign(j->nfreqs, 0);
+  L->blfreq_local.assign(j->nbaselines, std::vector<int>(j->nfreqs, -1));
+  int maxproducts = 0;
+  for (int b = 0; b < j->nbaselines; b++)
+    for (int f = 0; f < j->baselines[b].nfreqs; f++) {
+      L->freqused[j->baselines[b].freq_index[f]] = 1;
+      L->blfreq_local[b][j->baselines[b].freq_index[f]] = f;
+      maxproducts = std::max(maxproducts, j->baselines[b].npolproducts[f]);
+    }
+  for (int a = 0; a < j->nfreqs; a++) {
+    if (L->freqused[a]) continue;
+    for (int k = 0; k < j->nfreqs; k++) {
+      const fxg_freq& fa = j->freqs[a]; const fxg_freq& fk = j->freqs[k];
+      double bwdiff = fa.bw_mhz - fk.bw_mhz;
+      double freqdiff = fa.freq_mhz - fk.freq_mhz;
+      if (fa.lsb) freqdiff -= fa.bw_mhz;
+      if (fk.lsb) freqdiff += fk.bw_mhz;
+      /* the reference compares signed differences against TINY (configuration.cpp:356-358) */
+      if (bwdiff < TINY && freqdiff < TINY && fa.nchan == fk.nchan && fa.chans_to_avg == fk.chans_to_avg)
+        if (L->freqused[k]) L->equivused[a] = 1;
+    }
+  }
+  /* strides */
+  L->arraystridelen.assign(j->ndatastreams, 0);
+  for (int d = 0; d < j->ndatastreams; d++) {
+    int nchan = j->freqs[j->datastreams[d].recfreq_index[0]].nchan;
+    L->arraystridelen[d] = j->arraystridelen > 0 ? j->arraystridelen : calcstridelength(nchan);
+  }
+  if (j->xmacstridelen > 0) L->xmacstridelen = j->xmacstridelen;
+  else {
+    long nchangcd = 0;
+    for (int b = 0; b < j->nbaselines; b++)
+      for (int f = 0; f < j->baselines[b].nfreqs; f++) {
+        long nc = j->freqs[j->baselines[b].freq_index[f]].nchan;
+        nchangcd = nchangcd == 0 ? nc : gcdl(nchangcd, nc);
+      }
+    int target = 150, best = 0; double err = 1.0e12;
+    for (long g = nchangcd; g > 0; --g) {
+      if (nchangcd % g) continue;
+      double dd = std::fabs((double)(g - target));
+      if (dd < err) { err = dd; best = (int)g; }
+    }
+    L->xmacstridelen = best > 0 ? best : 1;
+  }
+  L->rotatestridelen = calcstridelength(L->xmacstridelen);
+  for (int f = 0; f < j->nfreqs; f++)
+    if (L->freqused[f] && j->freqs[f].nchan % L->xmacstridelen) return fail(FXG_EINVAL, "xmac stride must divide nchan");
+
+  L->bandsperautocorr = maxproducts > 2 ? 2 : 1;
+  int64_t idx = 0;
+  L->bloff.assign(j->nfreqs, std::vector<int64_t>(j->nbaselines, -1));
+  L->bwoff.assign(j->nfreqs, std::vector<int64_t>(j->nbaselines, -1));
+  for (int f = 0; f < j->nfreqs; f++) {
+    if (!L->freqused[f]) continue;
+    for (int b = 0; b < j->nbaselines; b++) {
+      int lf = L->blfreq_local[b][f];
+      if (lf < 0) continue;
+      L->bloff[f][b] = idx;
+      idx += (int64_t)j->baselines[b].npolproducts[lf] * j->freqs[f].nchan / j->freqs[f].chans_to_avg;
+    }
+  }
+  for (int f = 0; f < j->nfreqs; f++) {
+    if (!L->freqused[f]) continue;
+    for (int b = 0; b < j->nbaselines; b++) {
+      int lf = L->blfreq_local[b][f];
+      if (lf < 0) continue;
+      L->bwoff[f][b] = idx;
+      int np = j->baselines[b].npolproducts[lf];
+      idx += np / 2 + np % 2;
+    }
+  }
+  L->acoff.assign(j->ndatastreams, 0); L->acwoff.assign(j->ndatastreams, 0); L->pcaloff.assign(j->ndatastreams, 0);
+  for (int d = 0; d < j->ndatastreams; d++) {
+    const fxg_datastream& ds = j->datastreams[d];
+    L->acoff[d] = idx;
+    for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+      int fi = ds_band_freqindex(ds, k);
+      if (L->freqused[fi] || L->equivused[fi])
+        idx += (int64_t)L->bandsperautocorr * j->freqs[fi].nchan / j->freqs[fi].chans_to_avg;
+    }
+  }
+  for (int d = 0; d < j->ndatastreams; d++) {
+    const fxg_datastream& ds = j->datastreams[d];
+    L->acwoff[d] = idx;
+    int toadd = 0;
+    for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+      int fi = ds_band_freqindex(ds, k);
+      if (L->freqused[fi] || L->equivused[fi]) toadd += L->bandsperautocorr;
+    }
+    toadd = (toadd + 1) / 2;
+    if (toadd == 0) toadd = 1;
+    idx += toadd;
+  }
+  for (int d = 0; d < j->ndatastreams; d++) L->pcaloff[d] = idx;  /* phase cal off in this build */
+  L->total = idx;
+  return 0;
+}
+
+static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g) {
+  const fxg_datastream& ds = j->datastreams[d];
+  int r = make_fmt(&ds, &g->fmt); if (r) return r;
+  const fxg_freq& f0 = j->freqs[ds.recfreq_index[0]];
+  g->C = f0.nchan; g->L = 2 * g->C; g->bw = f0.bw_mhz;
+  g->s = lay.arraystridelen[d];
+  if (g->L % g->s) return fail(FXG_EINVAL, "array stride must divide fftchannels");
+  if (g->C % g->s) return fail(FXG_EINVAL, "array stride must divide nchan");
+  g->numfrstrides = g->L / g->s; g->nfracsteps = g->numfrstrides / 2;
+  g->sampletime = 1.0 / (2.0 * g->bw);
+  g->fractionalLoFreq = false; g->deltapoloffsets = false;
+  for (int i = 0; i < ds.nrecfreqs; i++) {
+    double loval = j->freqs[ds.recfreq_index[i]].freq_mhz;
+    if (std::fabs(loval - (int)(loval + 0.5)) > TINY) g->fractionalLoFreq = true;
+    if (ds.recfreq_clockoffsetdelta_us[i] != 0.0) g->deltapoloffsets = true;
+    if (ds.recfreq_phaseoffset_deg[i] != 0.0) return fail(FXG_EUNSUPPORTED, "pol phase offsets");
+  }
+  if ((g->L & (g->L - 1)) != 0) return fail(FXG_EUNSUPPORTED, "fftchannels must be a power of two");
+  /* Mk5Mode: blockclock = 2*bw, so samplesperblock = 1 (mode.cpp:129-141) */
+  g->samplesperblock = 1;
+  int numsamplebits = ds.nbits;
+  g->bytesperblocknum = (ds.nrecbands * g->samplesperblock * numsamplebits) / 8;
+  if (g->bytesperblocknum == 0) {
+    g->bytesperblocknum = 1;
+    g->bytesperblockden = 8 / (ds.nrecbands * g->samplesperblock * numsamplebits);
+  } else g->bytesperblockden = 1;
+  if (g->fmt.framebytes <= g->fmt.headerbytes) return fail(FXG_EINVAL, "framebytes");
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mode::process restated (fringe order 1, real sampling)                   */
+/* ------------------------------------------------------------------------ */
+struct BlockGeom {
+  int status; int nearestsample; int integerdelay; float weight; float fracsampleerror;
+  int intwalltime; double a, b, fracwalltime; double starttime, averagedelay;
+};
+
+static bool validbit(const fxg_station_input& in, int index) {
+  if (!in.validflags || in.nvalidflags <= index / FXG_FLAGS_PER_INT) return false;
+  return ((in.validflags[index / FXG_FLAGS_PER_INT] >> (index % FXG_FLAGS_PER_INT)) & 0x01) != 0;
+}
+
+static bool station_usable(const fxg_subint_desc& sub, const fxg_station_input& in) {
+  if (!sub.model_ok) return false;               /* setOffsets !foundok -> INVALID_SUBINT */
+  if (in.sec <= FXG_INVALID_SUBINT) return false;
+  if (in.scan != sub.scan) return false;         /* mode.cpp:1532-1536 torches data */
+  if (in.nbytes <= 1) return false;
+  return true;
+}
+
+/* mode.cpp:686-772 (geometry, range check, unpack, zero-weight check) and
+ * :795-812 (order-1 interpolator).  The unpacked samples go to `work`. */
+static BlockGeom block_geom(const DsGeom& g, const fxg_subint_desc& sub,
+                            const fxg_station_input& in, int index,
+                            std::vector<float>* work, Unpacked* up) {
+  BlockGeom bg; std::memset(&bg, 0, sizeof(bg));
+  const int fftchannels = g.L;
+  if (!station_usable(sub, in) || !validbit(in, index)) { bg.status = 1; return bg; }
+  const double* interp = in.interpolator;
+  double fftcentre = index + 0.5;
+  double averagedelay = interp[0] * fftcentre * fftcentre + interp[1] * fftcentre + interp[2];
+  double fftstartmicrosec = index * fftchannels * g.sampletime;
+  double starttime = (sub.sec - in.sec) * 1000000.0 +
+                     (static_cast<long long>(sub.ns) - static_cast<long long>(in.ns)) / 1000.0 +
+                     fftstartmicrosec - averagedelay;
+  int nearestsample = int(starttime / g.sampletime + 0.5);
+  double walltimesecs = sub.scan_start_sec + sub.sec + sub.ns / 1.0e9 + fftstartmicrosec / 1.0e6;
+  int intwalltime = static_cast<int>(walltimesecs);
+  bg.averagedelay = averagedelay; bg.starttime = starttime;
+  bg.intwalltime = intwalltime; bg.fracwalltime = walltimesecs - intwalltime;
+  bg.nearestsample = nearestsample;
+  if (nearestsample < -1 ||
+      ((((int64_t)nearestsample + fftchannels) / g.samplesperblock) * g.bytesperblocknum) / g.bytesperblockden > in.nbytes) {
+    bg.status = 2; return bg;
+  }
+  if (nearestsample == -1) nearestsample = 0;
+  bg.nearestsample = nearestsample;
+  *up = mk5_unpack(g.fmt, in.data, in.nbytes, nearestsample, g.L, *work);
+  bg.weight = up->weight;
+  if (!(bg.weight > 0.0f)) { bg.status = 3; return bg; }
+  double nearestsampletime = nearestsample * g.sampletime;
+  bg.fracsampleerror = float(starttime - nearestsampletime);
+  double d0 = interp[0] * index * index + interp[1] * index + interp[2];
+  double d1 = interp[0] * (index + 0.5) * (index + 0.5) + interp[1] * (index + 0.5) + interp[2];
+  double d2 = interp[0] * (index + 1) * (index + 1) + interp[1] * (index + 1) + interp[2];
+  double a = d2 - d0;
+  double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+  int integerdelay = static_cast<int>(b);
+  b -= integerdelay;
+  bg.a = a; bg.b = b; bg.integerdelay = integerdelay;
+  bg.status = 0;
+  return bg;
+}
+
+template <typename T> struct StationWork {
+  std::vector<float> unpacked;
+  std::vector<cpx<T>> rot, fracA, fracB, buf;
+  std::vector<std::complex<T>> fftbuf;
+};
+
+/* One block of one station: spectra[nrecbands][C] (zeroed if bailed). */
+template <typename T>
+static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
+                               const fxg_subint_desc& sub, const fxg_station_input& in,
+                               int index, const FFT<T>& fft, StationWork<T>& w,
+                               cpx<T>* spectra /* [nrecbands*C] */) {
+  const fxg_datastream& ds = j->datastreams[d];
+  const int C = g.C, L = g.L, s = g.s;
+  Unpacked up{};
+  BlockGeom bg = block_geom(g, sub, in, index, &w.unpacked, &up);
+  std::memset((void*)spectra, 0, sizeof(cpx<T>) * (size_t)ds.nrecbands * C);
+  if (bg.status != 0) return bg;
+  const bool fp64 = sizeof(T) == 8;
+  w.rot.resize(L); w.fracA.resize(C); w.fracB.resize(C); w.fftbuf.resize(L);
+  std::vector<double> subxval(s), stepxval(g.numfrstrides);
+  for (int k = 0; k < s; k++) subxval[k] = (double(k) / double(L)) * bg.a + bg.b;          /* mode.cpp:804-812 */
+  for (int jj = 0; jj < g.numfrstrides; jj++) stepxval[jj] = (double(jj * s) / double(L)) * bg.a;
+
+  for (int i = 0; i < ds.nrecfreqs; i++) {
+    const fxg_freq& fq = j->freqs[ds.recfreq_index[i]];
+    const double lofreq = fq.freq_mhz;
+    const double looffset = ds.recfreq_looffset_hz[i];
+    bool looff = false, isfraclooffset = false; double fraclooffset = 0.0;
+    if (looffset > 0.0 || looffset < 0.0) {                                               /* mode.cpp:888-896 */
+      looff = true;
+      fraclooffset = std::fabs(looffset) - int(std::fabs(looffset));
+      if (fraclooffset > TINY) isfraclooffset = true;
+      if (looffset < 0) fraclooffset = -fraclooffset;
+    }
+    /* fringe rotator, order 1 (mode.cpp:909-978) */
+    std::vector<cpx<T>> sub_c(s), step_c(g.numfrstrides);
+    for (int k = 0; k < s; k++) {
+      double ph = subxval[k] * lofreq;
+      if (g.fractionalLoFreq) ph += (lofreq - int(lofreq)) * double(bg.integerdelay);
+      if (looff) {
+        ph += (k * g.sampletime / 1e6) * (-looffset);
+        ph += -looffset * bg.fracwalltime;
+        if (isfraclooffset) ph += -fraclooffset * bg.intwalltime;
+      }
+      T arg = fp64 ? (T)(-TWO_PI * (ph - int(ph))) : (T)(float)(-TWO_PI * (ph - int(ph)));
+      T sn, cs; sincos_t(arg, &sn, &cs); sub_c[k].re = cs; sub_c[k].im = sn;
+    }
+    for (int jj = 0; jj < g.numfrstrides; jj++) {
+      double ph = stepxval[jj] * lofreq;
+      if (looff) ph += (jj * s * g.sampletime / 1e6) * (-looffset);
+      T arg = fp64 ? (T)(-TWO_PI * (ph - int(ph))) : (T)(float)(-TWO_PI * (ph - int(ph)));
+      T sn, cs; sincos_t(arg, &sn, &cs); step_c[jj].re = cs; step_c[jj].im = sn;
+    }
+    for (int k = 0; k < s; k++) w.rot[k] = sub_c[k];
+    for (int jj = 1; jj < g.numfrstrides; jj++)
+      for (int k = 0; k < s; k++) w.rot[(size_t)jj * s + k] = cmul(sub_c[k], step_c[jj]);
+
+    /* fractional-sample rotator (mode.cpp:1054-1097, 1100-1146) */
+    for (int pass = 0; pass < (g.deltapoloffsets ? 2 : 1); pass++) {
+      cpx<T>* fr = pass == 0 ? w.fracA.data() : w.fracB.data();
+      double dval = (double)(fp64 ? (double)(bg.starttime - bg.nearestsample * g.sampletime) : (double)bg.fracsampleerror)
+                    - ds.recfreq_clockoffset_us[i]
+                    + (pass == 0 ? 1.0 : -1.0) * ds.recfreq_clockoffsetdelta_us[i] / 2;
+      T val = fp64 ? (T)dval : (T)(float)dval;
+      std::vector<cpx<T>> stepf(g.nfracsteps);
+      for (int k = 0; k < s; k++) {
+        T cf = fp64 ? (T)((TWO_PI * (k) * g.bw) / C) : (T)(float)((TWO_PI * (k) * g.bw) / C);
+        T arg = cf * val; T sn, cs; sincos_t(arg, &sn, &cs); fr[k].re = cs; fr[k].im = sn;
+      }
+      for (int jj = 0; jj < g.nfracsteps; jj++) {
+        double dcf = fq.lsb ? (-TWO_PI * ((g.nfracsteps - jj) * s) * g.bw) / C
+                            : (TWO_PI * jj * s * g.bw) / C;
+        T cf = fp64 ? (T)dcf : (T)(float)dcf;
+        T arg = cf * val; T sn, cs; sincos_t(arg, &sn, &cs); stepf[jj].re = cs; stepf[jj].im = sn;
+      }
+      for (int jj = 1; jj < g.nfracsteps; jj++)
+        for (int k = 0; k < s; k++) fr[(size_t)jj * s + k] = cmul(fr[k], stepf[jj]);
+      for (int k = 0; k < s; k++) fr[k] = cmul(fr[k], stepf[0]);
+    }
+    if (!g.deltapoloffsets) std::copy(w.fracA.begin(), w.fracA.end(), w.fracB.begin());
+
+    /* bands of this frequency (mode.cpp:1148-1310) */
+    for (int jb = 0; jb < ds.nrecbands; jb++) {
+      if (ds.recband_freq[jb] != i) continue;
+      const float* x = w.unpacked.data() + (size_t)jb * up.stride + up.munged;
+      for (int n = 0; n < L; n++) {
+        T xv = (T)x[n];
+        w.fftbuf[n] = std::complex<T>(xv * w.rot[n].re, xv * w.rot[n].im);
+      }
+      fft.run(w.fftbuf.data());
+      const int off = fq.lsb ? C : 0;
+      const cpx<T>* fr = (!g.deltapoloffsets || ds.recband_pol[jb] == 'R') ? w.fracA.data() : w.fracB.data();
+      cpx<T>* out = spectra + (size_t)jb * C;
+      for (int c = 0; c < C; c++) {
+        cpx<T> v; v.re = w.fftbuf[off + c].real(); v.im = w.fftbuf[off + c].imag();
+        cpx<T> o; o.re = v.re * fr[c].re - v.im * fr[c].im; o.im = v.re * fr[c].im + v.im * fr[c].re;
+        out[c] = o;
+      }
+    }
+  }
+  return bg;
+}
+
+/* acc += a * conj(b), the operation order of genericAddProduct_32fc applied
+ * to (a, conj(b)) (architecture.h.in:580-586). */
+template <typename T, typename A>
+static inline void addprod_conj(A* acc, const cpx<T>& a, const cpx<T>& b) {
+  T cre = b.re, cim = -b.im;
+  acc[0] += a.re * cre - a.im * cim;
+  acc[1] += a.re * cim + a.im * cre;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Core::processdata restated                                               */
+/* ------------------------------------------------------------------------ */
+template <typename T>
+struct CoreRun {
+  const fxg_job_desc* j; Layout lay; std::vector<DsGeom> geom;
+  int init() {
+    int r = make_layout(j, &lay); if (r) return r;
+    geom.resize(j->ndatastreams);
+    for (int d = 0; d < j->ndatastreams; d++) { r = make_geom(j, lay, d, &geom[d]); if (r) return r; }
+    return 0;
+  }
+  /* Accumulate blocks [b0, b1) into acc (result layout, T precision). */
+  void run_range(const fxg_subint_desc& sub, const fxg_station_input* st, int b0, int b1, std::vector<T>& acc) {
+    const int nds = j->ndatastreams;
+    std::vector<std::vector<cpx<T>>> spec(nds);
+    std::vector<StationWork<T>> work(nds);
+    std::vector<std::unique_ptr<FFT<T>>> ffts(nds);
+    std::vector<std::vector<T>> ac0(nds), ac1(nds);      /* autocorrelations[2][band][C] as re,im */
+    std::vector<std::vector<T>> w0(nds), w1(nds);
+    for (int d = 0; d < nds; d++) {
+      const fxg_datastream& ds = j->datastreams[d];
+      spec[d].resize((size_t)ds.nrecbands * geom[d].C);
+      ffts[d].reset(new FFT<T>(geom[d].L));
+      ac0[d].assign((size_t)ds.nrecbands * geom[d].C * 2, 0); ac1[d].assign((size_t)ds.nrecbands * geom[d].C * 2, 0);
+      w0[d].assign(ds.nrecbands, 0); w1[d].assign(ds.nrecbands, 0);
+    }
+    std::vector<float> dweight(nds);
+    for (int blk = b0; blk < b1; blk++) {
+      for (int d = 0; d < nds; d++) {
+        const fxg_datastream& ds = j->datastreams[d];
+        BlockGeom bg = station_block<T>(j, geom[d], d, sub, st[d], blk, *ffts[d], work[d], spec[d].data());
+        dweight[d] = bg.status == 0 ? bg.weight : 0.0f;
+        if (bg.status != 0) continue;
+        const int C = geom[d].C;
+        for (int i = 0; i < ds.nrecfreqs; i++) {
+          int idx[FXG_MAX_BANDS]; int count = 0;
+          for (int jb = 0; jb < ds.nrecbands; jb++) {
+            if (ds.recband_freq[jb] != i) continue;
+            idx[count++] = jb;
+            const cpx<T>* X = spec[d].data() + (size_t)jb * C;
+            T* a0 = ac0[d].data() + (size_t)jb * C * 2;
+            for (int c = 0; c < C; c++) addprod_conj<T, T>(a0 + 2 * c, X[c], X[c]);
+            w0[d][jb] += (T)bg.weight;
+          }
+          if (count > 1 && j->crosspol_autocorrs) {            /* mode.cpp:1356-1375 */
+            const cpx<T>* X0 = spec[d].data() + (size_t)idx[0] * C;
+            const cpx<T>* X1 = spec[d].data() + (size_t)idx[1] * C;
+            T* a10 = ac1[d].data() + (size_t)idx[0] * C * 2;
+            T* a11 = ac1[d].data() + (size_t)idx[1] * C * 2;
+            for (int c = 0; c < C; c++) { addprod_conj<T, T>(a10 + 2 * c, X0[c], X1[c]); addprod_conj<T, T>(a11 + 2 * c, X1[c], X0[c]); }
+            w1[d][idx[0]] += (T)bg.weight; w1[d][idx[1]] += (T)bg.weight;
+          }
+        }
+      }
+      /* XMAC + baseline weights (core.cpp:826-975, 998-1045) */
+      for (int f = 0; f < j->nfreqs; f++) {
+        if (!lay.freqused[f]) continue;
+        const int nchan = j->freqs[f].nchan;
+        for (int b = 0; b < j->nbaselines; b++) {
+          int lf = lay.blfreq_local[b][f]; if (lf < 0) continue;
+          const fxg_baseline& bl = j->baselines[b];
+          for (int p = 0; p < bl.npolproducts[lf]; p++) {
+            const cpx<T>* v1 = band_spectrum(bl.ds1, bl.band1[lf][p], spec);
+            const cpx<T>* v2 = band_spectrum(bl.ds2, bl.band2[lf][p], spec);
+            T* out = acc.data() + 2 * (size_t)(lay.bloff[f][b] + (int64_t)p * nchan);
+            for (int c = 0; c < nchan; c++) addprod_conj<T, T>(out + 2 * c, v1[c], v2[c]);
+            T* bw = acc.data() + 2 * (size_t)lay.bwoff[f][b] + p;
+            *bw += (T)(dweight[bl.ds1] * dweight[bl.ds2]);
+          }
+        }
+      }
+    }
+    /* averageAndSendAutocorrs (core.cpp:1299-1369) */
+    for (int d = 0; d < nds; d++) {
+      const fxg_datastream& ds = j->datastreams[d];
+      int64_t ri = lay.acoff[d];
+      for (int pass = 0; pass < lay.bandsperautocorr; pass++) {
+        if (pass == 1 && !j->crosspol_autocorrs) break;
+        for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+          int fi = ds_band_freqindex(ds, k);
+          if (!(lay.freqused[fi] || lay.equivused[fi])) continue;
+          int nch = j->freqs[fi].nchan;
+          int parent = k < ds.nrecbands ? k : ds.zoomband_parent[k - ds.nrecbands];
+          int choff = k < ds.nrecbands ? 0 : ds.zoomband_choffset[k - ds.nrecbands];
+          const T* src = (pass == 0 ? ac0[d] : ac1[d]).data() + ((size_t)parent * geom[d].C + choff) * 2;
+          for (int c = 0; c < 2 * nch; c++) acc[2 * (size_t)ri + c] += src[c];
+          ri += nch;
+        }
+      }
+      int64_t wi = lay.acwoff[d] * 2;
+      for (int pass = 0; pass < lay.bandsperautocorr; pass++) {
+        if (pass == 1 && !j->crosspol_autocorrs) break;
+        for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
+          int fi = ds_band_freqindex(ds, k);
+          if (!(lay.freqused[fi] || lay.equivused[fi])) continue;
+          int parent = k < ds.nrecbands ? k : ds.zoomband_parent[k - ds.nrecbands];
+          acc[wi] += w0[d][parent];   /* getWeight(false, ...) for both passes */
+          wi++;
+        }
+      }
+    }
+  }
+  const cpx<T>* band_spectrum(int d, int band, const std::vector<std::vector<cpx<T>>>& spec) const {
+    const fxg_datastream& ds = j->datastreams[d];
+    if (band < ds.nrecbands) return spec[d].data() + (size_t)band * geom[d].C;
+    int z = band - ds.nrecbands;
+    return spec[d].data() + (size_t)ds.zoomband_parent[z] * geom[d].C + ds.zoomband_choffset[z];
+  }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+int orc_lut(int format, int nbits, float* out) {
+  if (format == FXG_FORMAT_VDIF && nbits == 2) { lut_vdif2(out); return 4; }
+  if (format == FXG_FORMAT_VDIF && nbits == 8) { lut_vdif8(out); return 256; }
+  if (format == FXG_FORMAT_MARK5B && nbits == 2) { lut_mark5b2(out); return 4; }
+  return fail(FXG_EUNSUPPORTED, "lut");
+}
+
+int orc_calcstridelength(int n) { return calcstridelength((unsigned)n); }
+
+/* Mk5Mode::unpack of L = nsamp samples at sampleoffset; out[nrecbands][nsamp]. */
+int orc_unpack(const fxg_datastream* ds, const uint8_t* data, int64_t nbytes,
+               int64_t sampleoffset, int nsamp, float* out, float* weight) {
+  StreamFmt f; int r = make_fmt(ds, &f); if (r) return r;
+  std::vector<float> work;
+  Unpacked u = mk5_unpack(f, data, nbytes, sampleoffset, nsamp, work);
+  for (int c = 0; c < f.nchan; c++)
+    std::memcpy(out + (size_t)c * nsamp, work.data() + (size_t)c * u.stride + u.munged, sizeof(float) * nsamp);
+  *weight = u.weight;
+  return 0;
+}
+
+int orc_plan(const fxg_job_desc* j, fxg_plan_info* info) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  std::memset(info, 0, sizeof(*info));
+  info->result_length = L.total;
+  info->arraystridelen = L.arraystridelen.empty() ? 0 : L.arraystridelen[0];
+  info->xmacstridelen = L.xmacstridelen;
+  info->rotatestridelen = L.rotatestridelen;
+  info->bandsperautocorr = L.bandsperautocorr;
+  info->fftchannels = 2 * j->freqs[j->datastreams[0].recfreq_index[0]].nchan;
+  return 0;
+}
+int orc_plan_baseline_offset(const fxg_job_desc* j, int f, int b, int64_t* off) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (f < 0 || f >= j->nfreqs || b < 0 || b >= j->nbaselines) return fail(FXG_EINVAL, "index");
+  *off = L.bloff[f][b]; return 0;
+}
+int orc_plan_bweight_offset(const fxg_job_desc* j, int f, int b, int64_t* off) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (f < 0 || f >= j->nfreqs || b < 0 || b >= j->nbaselines) return fail(FXG_EINVAL, "index");
+  *off = L.bwoff[f][b]; return 0;
+}
+int orc_plan_autocorr_offset(const fxg_job_desc* j, int d, int64_t* off) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "index");
+  *off = L.acoff[d]; return 0;
+}
+int orc_plan_acweight_offset(const fxg_job_desc* j, int d, int64_t* off) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "index");
+  *off = L.acwoff[d]; return 0;
+}
+
+/* Block geometry of one station/block (the Mode::process prologue). */
+int orc_block_info(const fxg_job_desc* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                   int d, int block, fxg_block_info* out) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "datastream");
+  DsGeom g; r = make_geom(j, L, d, &g); if (r) return r;
+  std::vector<float> work; Unpacked up{};
+  BlockGeom bg = block_geom(g, *sub, st[d], block, &work, &up);
+  std::memset(out, 0, sizeof(*out));
+  out->status = bg.status; out->nearestsample = bg.nearestsample; out->integerdelay = bg.integerdelay;
+  out->weight = bg.status == 0 ? bg.weight : 0.0f; out->fracsampleerror = bg.fracsampleerror;
+  out->intwalltime = bg.intwalltime; out->a = bg.a; out->b = bg.b; out->fracwalltime = bg.fracwalltime;
+  return 0;
+}
+
+/* Spectra of all recorded bands of station d for one block:
+ * out[nrecbands][C] complex, float (fp64 == 0) or double (fp64 != 0). */
+int orc_spectra(const fxg_job_desc* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                int d, int block, int fp64, void* out) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "datastream");
+  DsGeom g; r = make_geom(j, L, d, &g); if (r) return r;
+  if (fp64) {
+    FFT<double> fft(g.L); StationWork<double> w;
+    station_block<double>(j, g, d, *sub, st[d], block, fft, w, (cpx<double>*)out);
+  } else {
+    FFT<float> fft(g.L); StationWork<float> w;
+    station_block<float>(j, g, d, *sub, st[d], block, fft, w, (cpx<float>*)out);
+  }
+  return 0;
+}
+
+/* Core::processdata over blocks [block0, block0+nblocks) with nthreads
+ * threads, each owning a contiguous block range (core.cpp:2091-2095) and its
+ * own accumulators, merged into results in thread order.  results has
+ * result_length complex values: float pairs (fp64 == 0) or double pairs. */
+int orc_process(const fxg_job_desc* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                int fp64, int nthreads, int block0, int nblocks, void* results) {
+  if (nthreads < 1) nthreads = 1;
+  if (block0 < 0 || nblocks < 0 || block0 + nblocks > j->blockspersend) return fail(FXG_EINVAL, "block range");
+  auto go = [&](auto tag) -> int {
+    using T = decltype(tag);
+    CoreRun<T> core; core.j = j; int r = core.init(); if (r) return r;
+    const size_t n = 2 * (size_t)core.lay.total;
+    std::vector<std::vector<T>> accs(nthreads, std::vector<T>(n, (T)0));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) {
+      int s0 = block0 + (int)((int64_t)nblocks * t / nthreads);
+      int s1 = block0 + (int)((int64_t)nblocks * (t + 1) / nthreads);
+      th.emplace_back([&, s0, s1, t]() { core.run_range(*sub, st, s0, s1, accs[t]); });
+    }
+    for (auto& x : th) x.join();
+    T* out = (T*)results;
+    for (size_t i = 0; i < n; i++) out[i] = 0;
+    for (int t = 0; t < nthreads; t++) for (size_t i = 0; i < n; i++) out[i] += accs[t][i];
+    return 0;
+  };
+  return fp64 ? go(double()) : go(float());
+}
+
+/* Forward unnormalised FFT of L complex values (float or double pairs). */
+int orc_fft(int L, int fp64, const void* in, void* out) {
+  if (L <= 0 || (L & (L - 1))) return fail(FXG_EINVAL, "L");
+  if (fp64) {
+    FFT<double> f(L); std::vector<std::complex<double>> a((const std::complex<double>*)in, (const std::complex<double>*)in + L);
+    f.run(a.data()); std::memcpy(out, a.data(), sizeof(double) * 2 * L);
+  } else {
+    FFT<float> f(L); std::vector<std::complex<float>> a((const std::complex<float>*)in, (const std::complex<float>*)in + L);
+    f.run(a.data()); std::memcpy(out, a.data(), sizeof(float) * 2 * L);
+  }
+  return 0;
+}
+
+/* Model::calculateDelayInterpolator, order 2 (model.cpp:229-304) for a
+ * polynomial delay[polyorder+1] (µs, in seconds from the polynomial start):
+ * offsettime/timespan in seconds relative to the polynomial start. */
+int orc_delay_interpolator(const double* poly, int polyorder, double offsettime, double timespan,
+                           int numincrements, double* coeffs) {
+  double tp[16]; double samp[3];
+  if (polyorder < 0 || polyorder > 14) return fail(FXG_EINVAL, "polyorder");
+  auto eval = [&](double dt) {
+    tp[0] = 1.0; for (int i = 0; i < polyorder; i++) tp[i + 1] = tp[i] * dt;
+    double o = 0; for (int i = 0; i <= polyorder; i++) o += tp[i] * poly[i]; return o;
+  };
+  samp[1] = eval(offsettime + timespan / 2.0);
+  samp[0] = eval(offsettime);
+  samp[2] = eval(offsettime + timespan);
+  coeffs[0] = (2.0 * samp[0] - 4.0 * samp[1] + 2.0 * samp[2]) / (numincrements * numincrements);
+  coeffs[1] = (-3.0 * samp[0] + 4.0 * samp[1] - samp[2]) / numincrements;
+  coeffs[2] = samp[0];
+  return 0;
+}
+
+}  // extern "C"

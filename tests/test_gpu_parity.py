@@ -1,0 +1,229 @@
+"""GPU parity: the HIP library against the CPU oracle on the same seeded
+inputs (unpack bit-exact, block geometry exact, spectra and visibilities
+within the stated FP32 tolerance, reduced-precision XMAC within 1e-3 of the
+FP64 oracle)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fxcorr
+from fxcorr import (FXG_BLANKER_NONE, FXG_FORMAT_MARK5B, FXG_FORMAT_VDIF, Datastream, JobSpec, check)
+from fxsynth import make_subint
+from fxtestlib import (GpuJob, oracle_block_info, oracle_results, oracle_spectra, plan_info, rel_err,
+                       region_slices)
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5      # plain FP32 path vs the FP32 oracle, ||dV||/||V||
+# reduced-precision XMAC vs the FP64 oracle, ||dV||/||V||: the spectra are
+# rounded to an 11-bit (fp16) or 8-bit (bf16) significand, so the expected
+# relative visibility error is ~2^-11 (fp16) and ~2^-8 (bf16)
+MIXED_TOL = {"fp16": 1e-3, "bf16": 8e-3}
+
+
+def random_frames(rng, fmt, nchan, nbits, nframes, framebytes, headerbytes, corrupt=()):
+    data = rng.integers(0, 256, size=(nframes, framebytes), dtype=np.uint8)
+    fill = np.frombuffer(np.array([0x11223344, 0x11223344], dtype=np.uint32).tobytes(), dtype=np.uint8)
+    for f in range(nframes):
+        if fmt == FXG_FORMAT_VDIF:
+            w = np.zeros(8, dtype=np.uint32)
+            w[0] = 1000 + f // 100
+            w[2] = (framebytes // 8) | (int(np.log2(nchan)) << 24)
+            w[3] = (nbits - 1) << 26
+            data[f, :32] = w.view(np.uint8)
+        else:
+            w = np.zeros(4, dtype=np.uint32)
+            w[0] = 0xABADDEED
+            w[1] = f & 0x7FFF
+            data[f, :16] = w.view(np.uint8)
+    for f, kind in corrupt:
+        pl = headerbytes
+        if kind == "invalid":
+            if fmt == FXG_FORMAT_VDIF:
+                data[f, 3] |= 0x80
+            else:
+                data[f, 5] |= 0x80
+        elif kind == "zerolen":
+            data[f, 8:12] = 0
+        elif kind == "fillstart":
+            data[f, pl:pl + 8] = fill
+        elif kind == "fillend":
+            data[f, framebytes - 8:framebytes] = fill
+        elif kind == "filltail":       # fill pattern over the last part of the payload
+            n = (framebytes - pl) // 8
+            k = n // 3
+            data[f, framebytes - 8 * k:framebytes] = np.tile(fill, k)
+        elif kind == "fillhead":
+            n = (framebytes - pl) // 8
+            k = n // 4
+            data[f, pl:pl + 8 * k] = np.tile(fill, k)
+    return data.reshape(-1)
+
+
+def ds_desc(fmt, nchan, nbits, framebytes, headerbytes, blanker=0):
+    d = Datastream()
+    d.format = fmt
+    d.nbits = nbits
+    d.framebytes = framebytes
+    d.headerbytes = headerbytes
+    d.blanker = blanker
+    d.nrecfreqs = 1
+    d.nrecbands = nchan
+    d.nzoombands = 0
+    d.recfreq_index[0] = 0
+    return d
+
+
+UNPACK_CASES = [
+    # fmt, nchan, nbits, framebytes, headerbytes
+    (FXG_FORMAT_VDIF, 1, 2, 8032, 32),
+    (FXG_FORMAT_VDIF, 2, 2, 8032, 32),
+    (FXG_FORMAT_VDIF, 4, 2, 1056, 32),
+    (FXG_FORMAT_VDIF, 16, 2, 8032, 32),
+    (FXG_FORMAT_VDIF, 32, 2, 8032, 32),
+    (FXG_FORMAT_VDIF, 1, 8, 1056, 32),
+    (FXG_FORMAT_VDIF, 2, 8, 1056, 32),
+    (FXG_FORMAT_VDIF, 4, 8, 8032, 32),
+    (FXG_FORMAT_VDIF, 16, 8, 8032, 32),
+    (FXG_FORMAT_MARK5B, 4, 2, 10016, 16),
+    (FXG_FORMAT_MARK5B, 8, 2, 10016, 16),
+    (FXG_FORMAT_MARK5B, 16, 2, 10016, 16),
+]
+
+
+@pytest.mark.parametrize("case", UNPACK_CASES, ids=lambda c: f"fmt{c[0]}-{c[1]}ch-{c[2]}bit")
+@pytest.mark.parametrize("blanker", [0, FXG_BLANKER_NONE])
+def test_unpack_bitexact(gpu, oracle, case, blanker):
+    lib, ctx = gpu
+    fmt, nchan, nbits, fb, hb = case
+    rng = np.random.default_rng(nchan * 100 + nbits + 7 * blanker + fmt)
+    nframes = 24
+    corrupt = [(3, "invalid"), (7, "fillstart"), (11, "fillend"), (15, "filltail"), (18, "fillhead")]
+    if fmt == FXG_FORMAT_VDIF:
+        corrupt.append((21, "zerolen"))
+    data = random_frames(rng, fmt, nchan, nbits, nframes, fb, hb, corrupt)
+    d = ds_desc(fmt, nchan, nbits, fb, hb, blanker)
+    framesamples = (fb - hb) * 8 // (nchan * nbits)
+    dptr = C.c_void_p()
+    check(lib, lib.fxg_device_alloc(ctx, data.size, C.byref(dptr)), "alloc")
+    check(lib, lib.fxg_memcpy_h2d(ctx, dptr, data.ctypes.data, data.size), "h2d")
+    total = nframes * framesamples
+    offsets = [0, 1, 5, framesamples - 3, 2 * framesamples + 17, 3 * framesamples - 100,
+               7 * framesamples + 33, 15 * framesamples - 5, 20 * framesamples + 1, total - 600]
+    for nsamp in (512, 4096):
+        outdev = C.c_void_p()
+        check(lib, lib.fxg_device_alloc(ctx, 4 * nchan * nsamp, C.byref(outdev)), "alloc")
+        try:
+            for off in offsets:
+                if off < 0:
+                    continue
+                ref = np.zeros(nchan * nsamp, dtype=np.float32)
+                wref = C.c_float()
+                check(oracle, oracle.orc_unpack(C.byref(d), data.ctypes.data, data.size, off, nsamp,
+                                                ref.ctypes.data, C.byref(wref)), "orc_unpack")
+                w = C.c_float()
+                check(lib, lib.fxg_unpack(ctx, C.byref(d), dptr, data.size, off, nsamp, outdev, C.byref(w)),
+                      "fxg_unpack")
+                got = np.zeros_like(ref)
+                check(lib, lib.fxg_memcpy_d2h(ctx, got.ctypes.data, outdev, got.nbytes), "d2h")
+                assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), f"unpack mismatch at offset {off}"
+                assert w.value == wref.value, f"weight {w.value} != {wref.value} at offset {off}"
+        finally:
+            lib.fxg_device_free(ctx, outdev)
+    lib.fxg_device_free(ctx, dptr)
+
+
+def c1_spec(**kw):
+    base = dict(nstations=4, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=48)
+    base.update(kw)
+    return JobSpec(**base)
+
+
+def c2_spec(**kw):
+    base = dict(nstations=10, nsubbands=8, npol=2, bw_mhz=32.0, nchan=1024, blockspersend=6,
+                base_freq_mhz=6000.0)
+    base.update(kw)
+    return JobSpec(**base)
+
+
+def make_data(spec, seed=5, rho=0.05, **kw):
+    rng = np.random.default_rng(seed + 1000)
+    delays = rng.uniform(-20.0, 20.0, spec.nstations)
+    rates = rng.uniform(-1.0, 1.0, spec.nstations)
+    return make_subint(spec, seed=seed, rho=rho, delays_us=delays, rates_us_per_s=rates, **kw)
+
+
+def test_block_geometry_exact(gpu, oracle):
+    lib, ctx = gpu
+    spec = c1_spec()
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={1: [2, 3]}, invalid_blocks={2: [5, 6, 40]})
+    g = GpuJob(lib, ctx, job)
+    g.process(sd)
+    for ds in range(spec.nstations):
+        for blk in range(spec.blockspersend):
+            a = g.block_info(ds, blk)
+            b = oracle_block_info(oracle, job, sd, ds, blk)
+            for f in ("status", "nearestsample", "integerdelay", "intwalltime"):
+                assert getattr(a, f) == getattr(b, f), (ds, blk, f)
+            if b.status == 0:
+                for f in ("weight", "fracsampleerror", "a", "b", "fracwalltime"):
+                    assert getattr(a, f) == getattr(b, f), (ds, blk, f, getattr(a, f), getattr(b, f))
+    g.close()
+
+
+@pytest.mark.parametrize("mk", ["c1", "c2"])
+def test_spectra_fp32(gpu, oracle, mk):
+    lib, ctx = gpu
+    spec = c1_spec() if mk == "c1" else c2_spec()
+    job, keep = spec.build()
+    sd = make_data(spec)
+    g = GpuJob(lib, ctx, job)
+    g.process(sd, keep_taps=True)
+    chunk = g.info.chunk_blocks
+    last0 = ((spec.blockspersend - 1) // chunk) * chunk
+    for ds in range(spec.nstations):
+        for blk in (last0, spec.blockspersend - 1):
+            ref = oracle_spectra(oracle, job, sd, ds, blk, spec.nrecbands, spec.nchan)
+            for band in range(spec.nrecbands):
+                got = g.spectrum(ds, band, blk, spec.nchan)
+                assert rel_err(got, ref[band]) < FP32_TOL, (ds, band, blk, rel_err(got, ref[band]))
+    g.close()
+
+
+@pytest.mark.parametrize("mk", ["c1", "c2"])
+def test_visibilities_fp32(gpu, oracle, mk):
+    lib, ctx = gpu
+    spec = c1_spec() if mk == "c1" else c2_spec()
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [1]}, invalid_blocks={1: [2]})
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd)
+    info = plan_info(oracle, "orc_", job)
+    assert g.info.result_length == info.result_length
+    xend, a0, a1 = region_slices(oracle, job, info)
+    assert rel_err(got[:xend], ref[:xend]) < FP32_TOL
+    assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL
+    # weights are sums of the same FP32 terms in block order
+    assert np.array_equal(got[xend:a0].view(np.float32), ref[xend:a0].view(np.float32))
+    assert np.array_equal(got[a1:].view(np.float32), ref[a1:].view(np.float32))
+    g.close()
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_visibilities_mixed_vs_fp64(gpu, oracle, prec):
+    lib, ctx = gpu
+    spec = c1_spec(xmac=prec)
+    job, keep = spec.build()
+    sd = make_data(spec)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd, fp64=True)
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    e = rel_err(got[:xend], ref[:xend])
+    assert e < MIXED_TOL[prec], e
+    assert rel_err(got[a0:a1], ref[a0:a1]) < MIXED_TOL[prec]
+    g.close()
