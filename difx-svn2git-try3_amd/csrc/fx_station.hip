@@ -81,8 +81,10 @@ __global__ void k_frames(const DevStation* __restrict__ st, const DevStationSub*
       if (ld32(pl + 4LL * (r / 4 - 2)) == kFill32) r += 4;
       z = make_int2(r, kBig);
     }
-    if (fr[S.headerbytes - 11] & 0x80) z = make_int2(kBig, 0);  // per-group header check, whole frame
   }
+  // Mark5B decoders test the header's invalid flag for every group, whatever
+  // the blanker (mark5_format_mark5b.c:1591-1593)
+  if (S.format == FXG_FORMAT_MARK5B && (fr[S.headerbytes - 11] & 0x80)) z = make_int2(kBig, 0);
   ftab[Q.fbase + f] = z;
 }
 
