@@ -1,0 +1,323 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X FX correlator on the 32-station
+mixed-precision workload.
+
+Workload (one subintegration = one step): 32 stations x 2 pol x 16 subbands
+x 64 MHz (128 Msamp/s real), 2-bit VDIF (8032-byte frames), 4096 channels
+per subband (8192-point FFTs), 2000 FFT blocks (128 ms), every baseline with
+RR/RL/LR/LL products plus auto and cross-pol autocorrelations.  Inputs are
+seeded synthetic data (common Gaussian sky signal per subband, correlation
+log-uniform in [1e-4, 1e-1], independent station noise) generated on the GPU,
+so every timed step starts with its raw frames resident in HBM.
+
+Prints one JSON line: input Gsamples/s of the mixed-precision (fp16 XMAC)
+path, the plain FP32 path beside it, visibility error of every precision
+against the FP64 oracle on a short sample of the same workload, the
+roofline of the dominant kernel (live HIP-event timing) and the CPU
+restatement timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "difx-svn2git-try3_amd"))
+
+METRIC = "input Gsamples/s, mixed-precision vs plain FP32 at ≤1e-3 vis err; p99 latency"
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c3_spec(xmac: str, blocks: int):
+    from fxcorr import JobSpec
+    return JobSpec(nstations=32, nsubbands=16, npol=2, bw_mhz=64.0, nchan=4096, nbits=2,
+                   blockspersend=blocks, crosspol=True, xmac=xmac, framebytes=8032,
+                   base_freq_mhz=8200.0)
+
+
+def rhos_for(nsb, seed):
+    rng = np.random.default_rng(seed)
+    return 10.0 ** rng.uniform(-4.0, -1.0, nsb)
+
+
+class Job:
+    def __init__(self, lib, ctx, spec):
+        from fxcorr import PlanInfo, check
+        self.lib, self.spec = lib, spec
+        self.job, self.keep = spec.build()
+        self.h = C.c_void_p()
+        check(lib, lib.fxg_job_create(ctx, C.byref(self.job), C.byref(self.h)), "fxg_job_create")
+        self.info = PlanInfo()
+        check(lib, lib.fxg_job_plan_info(self.h, C.byref(self.info)), "plan_info")
+
+    def run(self, sub, st, dres):
+        from fxcorr import check
+        check(self.lib, self.lib.fxg_subint_process(self.h, C.byref(sub), st, 1, C.c_void_p(dres), 1),
+              "fxg_subint_process")
+
+    def stage_times(self, reset=True):
+        ms = (C.c_double * 8)()
+        n = (C.c_int64 * 8)()
+        self.lib.fxg_job_stage_times(self.h, ms, n, 1 if reset else 0)
+        return list(ms), list(n)
+
+    def close(self):
+        self.lib.fxg_job_destroy(self.h)
+
+
+def time_job(lib, ctx, spec, sd, raws, steps, warmup, torch):
+    from fxsynth import station_inputs_device
+    job = Job(lib, ctx, spec)
+    sub = sd.subint_desc()
+    st, keep = station_inputs_device(sd, raws)
+    dres = torch.empty(2 * job.info.result_length, dtype=torch.float32, device="cuda:0")
+    for _ in range(warmup):
+        job.run(sub, st, dres.data_ptr())
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        job.run(sub, st, dres.data_ptr())      # synchronous: returns when the subint is done
+        lat.append(time.perf_counter() - a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # one extra profiled step for the per-stage device times
+    lib.fxg_job_set_profiling(job.h, 1)
+    job.stage_times(reset=True)
+    job.run(sub, st, dres.data_ptr())
+    ms, n = job.stage_times(reset=True)
+    lib.fxg_job_set_profiling(job.h, 0)
+    info = job.info
+    job.close()
+    del dres
+    return el, lat, ms, n, info
+
+
+def vis_errors(lib, ctx, orc, seed, blocks, threads, torch):
+    """Visibility error of every XMAC precision against the FP64 oracle on a
+    short (blocks-long) subintegration of the same workload."""
+    from fxsynth import make_subint_device, station_inputs_device
+    from fxcorr import check
+    spec64 = c3_spec("fp32", blocks)
+    rng = np.random.default_rng(seed)
+    delays = [int(v) for v in rng.integers(-1500, 1500, spec64.nstations)]
+    sd, raws = make_subint_device(spec64, seed=seed, rho=rhos_for(spec64.nsubbands, seed), delay_samples=delays)
+    host = [r.cpu().numpy() for r in raws]
+    for s, h in zip(sd.stations, host):
+        s.raw = h
+    job64, keep = spec64.build()
+    from fxcorr import PlanInfo
+    info = PlanInfo()
+    check(orc, orc.orc_plan(C.byref(job64), C.byref(info)), "orc_plan")
+    ref = np.zeros(2 * info.result_length, dtype=np.float64)
+    sub = sd.subint_desc()
+    st, k = sd.station_inputs()
+    t = time.perf_counter()
+    check(orc, orc.orc_process(C.byref(job64), C.byref(sub), st, 1, threads, 0, blocks, ref.ctypes.data), "orc64")
+    log(f"fp64 oracle on {blocks} blocks: {time.perf_counter() - t:.1f}s")
+    ref = ref.view(np.complex128)
+    off = C.c_int64()
+    check(orc, orc.orc_plan_bweight_offset(C.byref(job64), 0, 0, C.byref(off)), "bw")
+    xend = off.value
+    check(orc, orc.orc_plan_autocorr_offset(C.byref(job64), 0, C.byref(off)), "ac")
+    a0 = off.value
+    # per-(datastream, band) parallel-hand autocorrelation spectra of the oracle
+    nb, nch = spec64.nrecbands, spec64.nchan
+    acs = []
+    for d in range(spec64.nstations):
+        check(orc, orc.orc_plan_autocorr_offset(C.byref(job64), d, C.byref(off)), "ac")
+        acs.append(ref[off.value: off.value + nb * nch].real.reshape(nb, nch))
+    out = {}
+    for prec in ("fp16", "bf16", "fp32"):
+        spec = c3_spec(prec, blocks)
+        job = Job(lib, ctx, spec)
+        dres = torch.empty(2 * job.info.result_length, dtype=torch.float32, device="cuda:0")
+        sti, keep2 = station_inputs_device(sd, raws)
+        job.run(sub, sti, dres.data_ptr())
+        got = dres.cpu().numpy().view(np.complex64).astype(np.complex128)
+        job.close()
+        d = got[:xend] - ref[:xend]
+        rel = float(np.linalg.norm(d) / np.linalg.norm(ref[:xend]))
+        # correlation-coefficient units: |dV| / sqrt(A1 A2) per channel
+        coef = 0.0
+        bl = 0
+        pos = 0
+        for f in range(spec.nsubbands):
+            for i in range(spec.nstations):
+                for j in range(i + 1, spec.nstations):
+                    for (p1, p2) in ((0, 0), (0, 1), (1, 0), (1, 1)):
+                        a1 = acs[i][f * 2 + p1]
+                        a2 = acs[j][f * 2 + p2]
+                        seg = d[pos: pos + nch]
+                        coef = max(coef, float(np.max(np.abs(seg) / np.sqrt(a1 * a2))))
+                        pos += nch
+        out[prec] = {"rel": rel, "coef_max": coef}
+        del dres
+    return out
+
+
+def cpu_baseline(orc, sd_host, spec, threads, nblocks):
+    from fxcorr import PlanInfo, check
+    job, keep = spec.build()
+    info = PlanInfo()
+    check(orc, orc.orc_plan(C.byref(job), C.byref(info)), "orc_plan")
+    out = np.zeros(2 * info.result_length, dtype=np.float32)
+    sub = sd_host.subint_desc()
+    st, k = sd_host.station_inputs()
+    t = time.perf_counter()
+    check(orc, orc.orc_process(C.byref(job), C.byref(sub), st, 0, threads, 0, nblocks, out.ctypes.data), "orc")
+    el = time.perf_counter() - t
+    samples = spec.nstations * spec.nrecbands * nblocks * spec.fftchannels
+    return samples / el / 1e9, el, samples
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--blocks", type=int, default=2000, help="FFT blocks per subintegration")
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32"])
+    ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison")
+    ap.add_argument("--no-verify", action="store_true", help="skip the FP64-oracle error check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-blocks", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import fxcorr
+    from fxcorr import check
+    from fxsynth import make_subint_device
+
+    lib = fxcorr.load_gpu_lib()
+    ctx = C.c_void_p()
+    check(lib, lib.fxg_ctx_create(0, C.byref(ctx)), "fxg_ctx_create")
+
+    spec = c3_spec(args.precision, args.blocks)
+    rng = np.random.default_rng(args.seed)
+    delays = [int(v) for v in rng.integers(-1500, 1500, spec.nstations)]
+    log("generating synthetic subintegration on the GPU ...")
+    t = time.perf_counter()
+    sd, raws = make_subint_device(spec, seed=args.seed, rho=rhos_for(spec.nsubbands, args.seed),
+                                  delay_samples=delays)
+    torch.cuda.synchronize()
+    log(f"  {sum(r.numel() for r in raws) / 1e9:.2f} GB of VDIF in {time.perf_counter() - t:.1f}s")
+    samples = spec.nstations * spec.nrecbands * spec.blockspersend * spec.fftchannels
+
+    log(f"timing {args.precision}: {args.warmup} warmup + {args.steps} steps")
+    el, lat, ms, n, info = time_job(lib, ctx, spec, sd, raws, args.steps, args.warmup, torch)
+    value = samples * args.steps / el / 1e9
+    log(f"  {value:.1f} Gsamples/s  ({el / args.steps * 1e3:.1f} ms/subint)")
+    stage = {fxcorr.STAGE_NAMES[i]: {"ms": ms[i], "launches": n[i]} for i in range(8) if n[i]}
+
+    fp32 = None
+    if not args.no_fp32 and args.precision != "fp32":
+        log("timing plain fp32")
+        spec32 = c3_spec("fp32", args.blocks)
+        el32, lat32, ms32, n32, info32 = time_job(lib, ctx, spec32, sd, raws, max(1, args.steps // 2), 1, torch)
+        v32 = samples * max(1, args.steps // 2) / el32 / 1e9
+        fp32 = {"value": v32, "ms_per_step": el32 / max(1, args.steps // 2) * 1e3,
+                "stages_ms": {fxcorr.STAGE_NAMES[i]: ms32[i] for i in range(8) if n32[i]}}
+        log(f"  fp32 {v32:.1f} Gsamples/s")
+
+    # ---- roofline of the dominant kernel (from the profiled step)
+    C_, L = spec.nchan, spec.fftchannels
+    nin = spec.nstations * spec.npol
+    specbytes = 8 if args.precision == "fp32" else 4
+    dom = max(("fstage", "xmac"), key=lambda k: stage.get(k, {"ms": 0})["ms"])
+    if dom == "fstage":
+        per_launch_blocks = info.chunk_blocks
+        # algorithmic bytes: 2-bit input read once + spectra written once
+        bytes_per_launch = (spec.nstations * spec.nrecbands * per_launch_blocks * L * 0.25 +
+                            spec.nstations * spec.nrecbands * per_launch_blocks * C_ * specbytes)
+        avg_s = stage["fstage"]["ms"] / stage["fstage"]["launches"] / 1e3
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"kernel": "k_fstage<8192>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "flops_per_launch": 5.0 * L * np.log2(L) * spec.nstations * spec.nrecbands * per_launch_blocks,
+                "avg_launch_ms": avg_s * 1e3}
+        roof["fp32_tflops"] = roof["flops_per_launch"] / avg_s / 1e12
+    else:
+        # algorithmic flops: Hermitian pairs incl. autos, 8 flops per complex MAC
+        per_launch_blocks = info.chunk_blocks
+        flops = 8.0 * nin * (nin + 1) / 2 * C_ * per_launch_blocks
+        avg_s = stage["xmac"]["ms"] / stage["xmac"]["launches"] / 1e3
+        achieved = flops / avg_s / 1e12
+        peak = PEAK_TFLOPS[args.precision]
+        roof = {"kernel": "k_xmac", "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3}
+
+    errors = None
+    if not args.no_verify:
+        log("visibility error vs the FP64 oracle on a 6-block sample")
+        orc = fxcorr.load_oracle()
+        errors = vis_errors(lib, ctx, orc, args.seed + 1, 6, min(16, os.cpu_count() or 1), torch)
+        log(f"  {errors}")
+
+    cpu = None
+    if not args.no_cpu:
+        orc = fxcorr.load_oracle()
+        threads = min(16, os.cpu_count() or 1)
+        nb = args.cpu_blocks or 2 * threads
+        host = [r.cpu().numpy() for r in raws]
+        for s, h in zip(sd.stations, host):
+            s.raw = h
+        spec_cpu = c3_spec("fp32", args.blocks)
+        log(f"CPU baseline: oracle FP32, {threads} threads, {nb} blocks")
+        v, el_cpu, ns = cpu_baseline(orc, sd, spec_cpu, threads, nb)
+        cpu = {"value": v, "unit": "Gsamples/s", "cores": threads, "kind": "port",
+               "sample": f"first {nb} of {args.blocks} FFT blocks of the same subintegration "
+                         f"({ns / 1e9:.3f} Gsamples, {el_cpu:.1f} s wall)"}
+        log(f"  {v:.4f} Gsamples/s")
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (seeded Gaussian sky + station noise, 2-bit VDIF generated on the GPU)",
+        "config": {"workload": "32 stations x 2 pol x 16 subbands x 64 MHz, 2-bit VDIF, 4096 ch, "
+                               f"{args.blocks} FFT blocks of 8192 per subint ({args.blocks * 8192 / 128e6 * 1e3:.0f} ms)",
+                   "stations": spec.nstations, "subbands": spec.nsubbands, "pols": spec.npol,
+                   "channels": spec.nchan, "blocks_per_subint": spec.blockspersend,
+                   "xmac": args.precision, "fft": "fp32"},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "stages_ms": {k: v["ms"] for k, v in stage.items()},
+        "plain_fp32": fp32,
+        "mixed_vs_fp32": (value / fp32["value"]) if fp32 else None,
+        "vis_error_vs_fp64": errors,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    lib.fxg_ctx_destroy(ctx)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

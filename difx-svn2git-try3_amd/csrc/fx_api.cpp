@@ -80,6 +80,40 @@ struct fxg_job {
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
+  // stage profiling
+  int profiling = 0;
+  std::vector<hipEvent_t> evpool;
+  std::vector<std::pair<int, int>> evused;   // (stage, pool index of the start event)
+  double stage_ms[FXG_NSTAGES] = {0};
+  long long stage_n[FXG_NSTAGES] = {0};
+  int ev_begin(int stage) {
+    if (!profiling) return 0;
+    size_t need = 2 * (evused.size() + 1);
+    while (evpool.size() < need) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return set_err(FXG_EHIP, "hipEventCreate");
+      evpool.push_back(e);
+    }
+    int idx = (int)(2 * evused.size());
+    evused.push_back({stage, idx});
+    if (hipEventRecord(evpool[idx], stream) != hipSuccess) return set_err(FXG_EHIP, "hipEventRecord");
+    return 0;
+  }
+  int ev_end() {
+    if (!profiling) return 0;
+    if (hipEventRecord(evpool[evused.back().second + 1], stream) != hipSuccess) return set_err(FXG_EHIP, "hipEventRecord");
+    return 0;
+  }
+  void ev_collect() {
+    for (auto& u : evused) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, evpool[u.second], evpool[u.second + 1]) == hipSuccess) {
+        stage_ms[u.first] += ms;
+        stage_n[u.first] += 1;
+      }
+    }
+    evused.clear();
+  }
 };
 
 extern "C" {
@@ -264,6 +298,8 @@ static void job_free(fxg_job* j) {
                   &j->d_spec, &j->d_V};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
+  for (hipEvent_t e : j->evpool) hipEventDestroy(e);
+  j->evpool.clear();
   if (j->stream) hipStreamDestroy(j->stream);
 }
 
@@ -389,7 +425,9 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
     const unsigned char* dp = in.data;
     if (!data_on_device && in.nbytes > 0) {
       if ((rc = j->d_raw[d].ensure((size_t)in.nbytes))) return rc;
+      if ((rc = j->ev_begin(FXG_STAGE_H2D))) return rc;
       HIPCHK(hipMemcpyAsync(j->d_raw[d].p, in.data, (size_t)in.nbytes, hipMemcpyHostToDevice, s));
+      if ((rc = j->ev_end())) return rc;
       dp = j->d_raw[d].as<unsigned char>();
     }
     q.data = dp;
@@ -413,10 +451,14 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   HIPCHK(hipMemcpyAsync(j->d_ss.p, ss.data(), sizeof(DevStationSub) * ss.size(), hipMemcpyHostToDevice, s));
   if (!flags.empty()) HIPCHK(hipMemcpyAsync(j->d_flags.p, flags.data(), sizeof(int) * flags.size(), hipMemcpyHostToDevice, s));
 
+  if ((rc = j->ev_begin(FXG_STAGE_FRAMES))) return rc;
   HIPCHK(launch_frames(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ftab.as<int2>(), P.nds, maxframes, s));
+  if ((rc = j->ev_end())) return rc;
   DevSubint dsub{sub->sec, sub->ns, sub->scan_start_sec, nblocks};
+  if ((rc = j->ev_begin(FXG_STAGE_PROLOGUE))) return rc;
   HIPCHK(launch_prologue(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_flags.as<int>(),
                          j->d_ftab.as<int2>(), dsub, j->d_blocks.as<DevBlock>(), P.nds, s));
+  if ((rc = j->ev_end())) return rc;
 
   // ---- F and X stages, chunk by chunk
   const int prec = D.xmac_precision;
@@ -435,7 +477,9 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   for (int c0 = 0; c0 < nblocks; c0 += P.chunk_blocks) {
     int cn = std::min(P.chunk_blocks, nblocks - c0);
     fa.chunk0 = c0;
+    if ((rc = j->ev_begin(FXG_STAGE_FSTAGE))) return rc;
     HIPCHK(launch_fstage(P.L, fa, P.nbands_total, cn, s));
+    if ((rc = j->ev_end())) return rc;
     for (size_t x = 0; x < P.xf.size(); x++) {
       const XmacFreq& xf = P.xf[x];
       XmacArgs xa{};
@@ -448,22 +492,49 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
       xmac_shape(xa.nin, xa.nchan, &xa);
       xa.V = j->d_V.as<float2>() + xf.voff;
       xa.accumulate = c0 > 0;
+      if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
       HIPCHK(launch_xmac_f32(xa, prec, s));
+      if ((rc = j->ev_end())) return rc;
     }
     j->tap_chunk0 = c0;
     j->tap_chunkn = cn;
   }
+  if ((rc = j->ev_begin(FXG_STAGE_WEIGHTS))) return rc;
   HIPCHK(launch_weights(j->d_blocks.as<DevBlock>(), nblocks, j->d_blpairs.as<int2>(), P.nbl, P.nds,
                         j->d_weights.as<float>(), s));
+  if ((rc = j->ev_end())) return rc;
   const size_t resbytes = sizeof(float) * 2 * (size_t)P.info.result_length;
   float* dres = results_on_device ? results : j->d_results.as<float>();
   HIPCHK(hipMemsetAsync(dres, 0, resbytes, s));
   float vscale = 1.0f / (j->scale * j->scale);
+  if ((rc = j->ev_begin(FXG_STAGE_SCATTER))) return rc;
   HIPCHK(launch_scatter(j->d_segs.as<DevSeg>(), (int)P.segs.size(), P.maxseglen, j->d_V.as<float2>(),
                         j->d_weights.as<float>(), vscale, dres, s));
-  if (!results_on_device) HIPCHK(hipMemcpyAsync(results, dres, resbytes, hipMemcpyDeviceToHost, s));
+  if ((rc = j->ev_end())) return rc;
+  if (!results_on_device) {
+    if ((rc = j->ev_begin(FXG_STAGE_D2H))) return rc;
+    HIPCHK(hipMemcpyAsync(results, dres, resbytes, hipMemcpyDeviceToHost, s));
+    if ((rc = j->ev_end())) return rc;
+  }
   HIPCHK(hipStreamSynchronize(s));
+  j->ev_collect();
   j->have_subint = 1;
+  return 0;
+}
+
+int fxg_job_set_profiling(fxg_job* j, int on) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  j->profiling = on ? 1 : 0;
+  return 0;
+}
+
+int fxg_job_stage_times(fxg_job* j, double* ms, int64_t* launches, int reset) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  for (int i = 0; i < FXG_NSTAGES; i++) {
+    if (ms) ms[i] = j->stage_ms[i];
+    if (launches) launches[i] = j->stage_n[i];
+    if (reset) { j->stage_ms[i] = 0; j->stage_n[i] = 0; }
+  }
   return 0;
 }
 

@@ -33,6 +33,7 @@ FXG_XMAC_FP16 = 1
 FXG_XMAC_BF16 = 2
 FXG_INVALID_SUBINT = -99999999
 FXG_FLAGS_PER_INT = 30
+STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h"]
 
 XMAC_NAMES = {"fp32": FXG_XMAC_FP32, "fp16": FXG_XMAC_FP16, "bf16": FXG_XMAC_BF16}
 
@@ -295,6 +296,8 @@ def load_gpu_lib():
     lib.fxg_subint_process.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int,
                                        C.c_void_p, C.c_int]
     lib.fxg_job_set_keep_taps.argtypes = [C.c_void_p, C.c_int]
+    lib.fxg_job_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]
     lib.fxg_tap_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     _gpu_lib = lib

@@ -163,6 +163,8 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
     maxd = max(abs(d) for d in dsamp) + 2 * spec.fftchannels
     if guard_frames is None:
         guard_frames = (maxd + fs - 1) // fs + 1
+        while abs(guard_frames * fs * ts * 1000.0 - round(guard_frames * fs * ts * 1000.0)) > 1e-6:
+            guard_frames += 1  # the data start must fall on a whole nanosecond
     G = guard_frames * fs
     nsamp_sub = spec.blockspersend * spec.fftchannels
     nframes = (nsamp_sub + 2 * G + fs - 1) // fs + extra_frames
@@ -204,3 +206,112 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
                                     validflags=vf.astype(np.int32), interpolator=interp,
                                     delay_samples=dsamp[s]))
     return SubintData(spec=spec, sub_sec=sub_sec, sub_ns=sub_ns, scan_start_sec=0, stations=stations)
+
+
+# ---------------------------------------------------------------------------
+# Device-side generation for large jobs (same model, generated with torch on
+# the GPU so that multi-GB subintegrations start resident in HBM)
+# ---------------------------------------------------------------------------
+def make_subint_device(spec: JobSpec, seed: int = 1, rho=0.05, delay_samples=None,
+                       rates_us_per_s=None, sub_sec: int = 10, device: str = "cuda:0"):
+    """Returns (SubintData with raw=None, list of torch uint8 tensors on device).
+
+    Delays are whole samples (delay_samples per station); the model carries
+    the same delay.  Only VDIF 2-bit/8-bit is generated here.
+    """
+    import torch
+    if spec.fmt != FXG_FORMAT_VDIF:
+        raise ValueError("device generator writes VDIF")
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    ts = spec.sampletime_us
+    nst, nsb, npol = spec.nstations, spec.nsubbands, spec.npol
+    fs = spec.framesamples
+    frames_per_sec = int(round(1e6 / ts)) // fs
+    if delay_samples is None:
+        rng = np.random.default_rng(seed)
+        delay_samples = [int(v) for v in rng.integers(-2000, 2000, nst)]
+    if rates_us_per_s is None:
+        rates_us_per_s = np.zeros(nst)
+    maxd = max(abs(d) for d in delay_samples) + 2 * spec.fftchannels
+    guard_frames = (maxd + fs - 1) // fs + 1
+    while abs(guard_frames * fs * ts * 1000.0 - round(guard_frames * fs * ts * 1000.0)) > 1e-6:
+        guard_frames += 1      # the data start must fall on a whole nanosecond
+    G = guard_frames * fs
+    nsamp_sub = spec.blockspersend * spec.fftchannels
+    nframes = (nsamp_sub + 2 * G + fs - 1) // fs + 1
+    T = nframes * fs
+    rhos = np.broadcast_to(np.asarray(rho, dtype=np.float64), (nsb,))
+    start_ns_total = sub_sec * 1_000_000_000 - G * ts * 1000.0
+    if abs(start_ns_total - round(start_ns_total)) > 1e-6:
+        raise ValueError("guard does not land on an integer ns")
+    datasec, datans = divmod(int(round(start_ns_total)), 1_000_000_000)
+    first_frameno = int(round((datans / 1000.0) / ts)) // fs
+    lo, hi = min(delay_samples), max(delay_samples)
+    nchan = spec.nrecbands
+    databytes = spec.framebytes - spec.headerbytes
+    # common signal per subband, kept on device
+    common = [torch.randn(T + hi - lo, generator=g, device=device) for _ in range(nsb)]
+    raws = []
+    stations = []
+    nflags = (spec.blockspersend + FXG_FLAGS_PER_INT - 1) // FXG_FLAGS_PER_INT
+    for s in range(nst):
+        codes = torch.empty((T, nchan), dtype=torch.uint8, device=device)
+        for sb in range(nsb):
+            r = float(rhos[sb])
+            gs = common[sb][delay_samples[s] - lo: delay_samples[s] - lo + T]
+            for p in range(npol):
+                x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * torch.randn(T, generator=g, device=device)
+                if spec.nbits == 2:
+                    q = (x >= -0.98).to(torch.uint8) + (x >= 0).to(torch.uint8) + (x >= 0.98).to(torch.uint8)
+                else:
+                    q = torch.clamp(torch.round(x * 8.0) + 128, 0, 255).to(torch.uint8)
+                codes[:, sb * npol + p] = q
+                del x, q
+        flat = codes.reshape(-1)
+        if spec.nbits == 2:
+            f4 = flat.view(-1, 4)
+            payload = f4[:, 0] | (f4[:, 1] << 2) | (f4[:, 2] << 4) | (f4[:, 3] << 6)
+        else:
+            payload = flat
+        del codes, flat
+        out = torch.empty((nframes, spec.framebytes), dtype=torch.uint8, device=device)
+        out[:, spec.headerbytes:] = payload.view(nframes, databytes)
+        hdr = np.empty((nframes, spec.headerbytes), dtype=np.uint8)
+        for f in range(nframes):
+            fn = first_frameno + f
+            hdr[f] = vdif_header(datasec + fn // frames_per_sec, fn % frames_per_sec, spec.framebytes,
+                                 nchan, spec.nbits, s)[: spec.headerbytes]
+        out[:, : spec.headerbytes] = torch.from_numpy(hdr).to(device)
+        raws.append(out.view(-1))
+        vf = np.zeros(nflags, dtype=np.int64)
+        for b in range(spec.blockspersend):
+            vf[b // FXG_FLAGS_PER_INT] |= 1 << (b % FXG_FLAGS_PER_INT)
+        blk_s = spec.fftchannels * ts * 1e-6
+        interp = np.array([0.0, rates_us_per_s[s] * blk_s, delay_samples[s] * ts], dtype=np.float64)
+        stations.append(StationData(raw=np.empty(0, np.uint8), datasec=int(datasec), datans=int(datans),
+                                    validflags=vf.astype(np.int32), interpolator=interp,
+                                    delay_samples=delay_samples[s]))
+    del common
+    sd = SubintData(spec=spec, sub_sec=sub_sec, sub_ns=0, scan_start_sec=0, stations=stations)
+    return sd, raws
+
+
+def station_inputs_device(sd: SubintData, raws):
+    """StationInput array pointing at device tensors."""
+    arr = (StationInput * len(sd.stations))()
+    keep = []
+    for i, st in enumerate(sd.stations):
+        a = arr[i]
+        a.data = raws[i].data_ptr()
+        a.nbytes = raws[i].numel()
+        a.scan = 0
+        a.sec = st.datasec
+        a.ns = st.datans
+        vf = np.ascontiguousarray(st.validflags, dtype=np.int32)
+        keep.append(vf)
+        a.nvalidflags = vf.size
+        a.validflags = vf.ctypes.data_as(C.POINTER(C.c_int32))
+        for k in range(3):
+            a.interpolator[k] = float(st.interpolator[k])
+    return arr, keep

@@ -238,6 +238,22 @@ int fxg_subint_process(fxg_job* job, const fxg_subint_desc* sub,
                        const fxg_station_input* stations, int data_on_device,
                        float* results, int results_on_device);
 
+/* Per-stage device time, measured with HIP events on the job's stream
+ * around every launch while profiling is on. */
+#define FXG_STAGE_FRAMES    0   /* frame validity                              */
+#define FXG_STAGE_PROLOGUE  1   /* FP64 block geometry                         */
+#define FXG_STAGE_FSTAGE    2   /* unpack + rotate + FFT + frac rotation       */
+#define FXG_STAGE_XMAC      3   /* cross-multiply-accumulate                    */
+#define FXG_STAGE_WEIGHTS   4
+#define FXG_STAGE_SCATTER   5   /* copy into the result layout                 */
+#define FXG_STAGE_H2D       6   /* host -> device copies of raw data           */
+#define FXG_STAGE_D2H       7   /* device -> host copy of results              */
+#define FXG_NSTAGES         8
+int fxg_job_set_profiling(fxg_job* job, int on);
+/* ms[FXG_NSTAGES] accumulated milliseconds, launches[FXG_NSTAGES] counts;
+ * reset != 0 clears the accumulators after reading. */
+int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
+
 /* Debug taps valid after fxg_subint_process with keep_taps set. */
 int fxg_job_set_keep_taps(fxg_job* job, int keep);
 int fxg_tap_block_info(fxg_job* job, int datastream, int block, fxg_block_info* out);
