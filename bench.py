@@ -39,11 +39,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def c3_spec(xmac: str, blocks: int):
+def c3_spec(xmac: str, blocks: int, chunk: int = 0):
     from fxcorr import JobSpec
     return JobSpec(nstations=32, nsubbands=16, npol=2, bw_mhz=64.0, nchan=4096, nbits=2,
                    blockspersend=blocks, crosspol=True, xmac=xmac, framebytes=8032,
-                   base_freq_mhz=8200.0)
+                   base_freq_mhz=8200.0, max_chunk_blocks=chunk)
 
 
 def rhos_for(nsb, seed):
@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--no-verify", action="store_true", help="skip the FP64-oracle error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-blocks", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=2000,
+                    help="FFT blocks of spectra resident in HBM at once (default: the whole subint)")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
 
@@ -211,7 +213,7 @@ def main():
     ctx = C.c_void_p()
     check(lib, lib.fxg_ctx_create(0, C.byref(ctx)), "fxg_ctx_create")
 
-    spec = c3_spec(args.precision, args.blocks)
+    spec = c3_spec(args.precision, args.blocks, args.chunk)
     rng = np.random.default_rng(args.seed)
     delays = [int(v) for v in rng.integers(-1500, 1500, spec.nstations)]
     log("generating synthetic subintegration on the GPU ...")
@@ -231,7 +233,7 @@ def main():
     fp32 = None
     if not args.no_fp32 and args.precision != "fp32":
         log("timing plain fp32")
-        spec32 = c3_spec("fp32", args.blocks)
+        spec32 = c3_spec("fp32", args.blocks, args.chunk)
         el32, lat32, ms32, n32, info32 = time_job(lib, ctx, spec32, sd, raws, max(1, args.steps // 2), 1, torch)
         v32 = samples * max(1, args.steps // 2) / el32 / 1e9
         fp32 = {"value": v32, "ms_per_step": el32 / max(1, args.steps // 2) * 1e3,

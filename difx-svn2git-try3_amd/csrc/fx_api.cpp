@@ -73,7 +73,9 @@ struct fxg_job {
   // device tables
   DBuf d_st, d_bands, d_segs, d_blpairs, d_inoff, d_tw, d_lut8;
   // per-subint
-  DBuf d_ss, d_flags, d_ftab, d_blocks, d_weights, d_results;
+  DBuf d_ss, d_flags, d_ftab, d_blocks, d_weights, d_results, d_ct;
+  DBuf d_bandlist;                // global bands grouped by sample size
+  int nb2 = 0, nb8 = 0;           // bands of 2-bit and 8-bit datastreams
   std::vector<DBuf> d_raw;        // staged host data per datastream
   // big buffers
   DBuf d_spec, d_V;
@@ -295,7 +297,7 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
-                  &j->d_spec, &j->d_V};
+                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -366,6 +368,14 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
       (inoff.size() && hipMemcpy(j->d_inoff.p, inoff.data(), sizeof(long long) * inoff.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(j->d_tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_err(FXG_EHIP, "table upload failed"));
+  // band lists per sample size (one F-stage launch per size)
+  std::vector<int> bl2, bl8;
+  for (size_t g = 0; g < P.bands.size(); g++) (P.st[P.bands[g].station].nbits == 8 ? bl8 : bl2).push_back((int)g);
+  j->nb2 = (int)bl2.size(); j->nb8 = (int)bl8.size();
+  std::vector<int> bl = bl2; bl.insert(bl.end(), bl8.begin(), bl8.end());
+  if ((rc = j->d_bandlist.ensure(sizeof(int) * std::max<size_t>(1, bl.size())))) return fail(rc);
+  if (hipMemcpy(j->d_bandlist.p, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(FXG_EHIP, "band list upload failed"));
   j->d_raw.resize(P.nds);
   *out = j;
   return 0;
@@ -417,7 +427,8 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   // ---- per-datastream subint table
   std::vector<DevStationSub> ss(P.nds);
   std::vector<int> flags;
-  long long fbase = 0, maxframes = 0;
+  long long fbase = 0, maxframes = 0, ctbytes = 0;
+  int maxdatabytes = 0;
   for (int d = 0; d < P.nds; d++) {
     const fxg_station_input& in = st[d];
     DevStationSub& q = ss[d];
@@ -435,6 +446,10 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
     q.nframes = in.nbytes / P.st[d].framebytes;
     q.fbase = fbase;
     fbase += q.nframes;
+    q.ctbase = ctbytes;
+    q.ctstride = ((q.nframes * P.st[d].bpf + 3) / 4) * 4;
+    ctbytes += q.ctstride * P.st[d].nchan;
+    maxdatabytes = std::max(maxdatabytes, P.st[d].databytes);
     maxframes = std::max(maxframes, q.nframes);
     q.datasec = in.sec; q.datans = in.ns;
     // Mode::setOffsets / setData guards (mode.cpp:1525-1573, 671)
@@ -446,13 +461,16 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   }
   if ((rc = j->d_ss.ensure(sizeof(DevStationSub) * ss.size())) ||
       (rc = j->d_flags.ensure(sizeof(int) * std::max<size_t>(1, flags.size()))) ||
-      (rc = j->d_ftab.ensure(sizeof(int2) * (size_t)std::max<long long>(1, fbase))))
+      (rc = j->d_ftab.ensure(sizeof(int2) * (size_t)std::max<long long>(1, fbase))) ||
+      (rc = j->d_ct.ensure((size_t)std::max<long long>(4, ctbytes))))
     return rc;
   HIPCHK(hipMemcpyAsync(j->d_ss.p, ss.data(), sizeof(DevStationSub) * ss.size(), hipMemcpyHostToDevice, s));
   if (!flags.empty()) HIPCHK(hipMemcpyAsync(j->d_flags.p, flags.data(), sizeof(int) * flags.size(), hipMemcpyHostToDevice, s));
 
   if ((rc = j->ev_begin(FXG_STAGE_FRAMES))) return rc;
   HIPCHK(launch_frames(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ftab.as<int2>(), P.nds, maxframes, s));
+  HIPCHK(launch_cornerturn(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ct.as<unsigned char>(),
+                           P.nds, maxframes, maxdatabytes, s));
   if ((rc = j->ev_end())) return rc;
   DevSubint dsub{sub->sec, sub->ns, sub->scan_start_sec, nblocks};
   if ((rc = j->ev_begin(FXG_STAGE_PROLOGUE))) return rc;
@@ -470,6 +488,7 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   fa.ftab = j->d_ftab.as<int2>();
   fa.lut8 = j->ctx->lut8.as<float>();
   fa.tw = j->d_tw.as<float2>();
+  fa.ct = j->d_ct.as<unsigned char>();
   fa.spectra = j->d_spec.p;
   fa.prec = prec;
   fa.scale = j->scale;
@@ -478,7 +497,14 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
     int cn = std::min(P.chunk_blocks, nblocks - c0);
     fa.chunk0 = c0;
     if ((rc = j->ev_begin(FXG_STAGE_FSTAGE))) return rc;
-    HIPCHK(launch_fstage(P.L, fa, P.nbands_total, cn, s));
+    if (j->nb2) {
+      fa.bandlist = j->d_bandlist.as<int>();
+      HIPCHK(launch_fstage(P.L, 2, fa, j->nb2, cn, s));
+    }
+    if (j->nb8) {
+      fa.bandlist = j->d_bandlist.as<int>() + j->nb2;
+      HIPCHK(launch_fstage(P.L, 8, fa, j->nb8, cn, s));
+    }
     if ((rc = j->ev_end())) return rc;
     for (size_t x = 0; x < P.xf.size(); x++) {
       const XmacFreq& xf = P.xf[x];
@@ -493,7 +519,8 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
       xa.V = j->d_V.as<float2>() + xf.voff;
       xa.accumulate = c0 > 0;
       if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
-      HIPCHK(launch_xmac_f32(xa, prec, s));
+      if (prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan)) HIPCHK(launch_xmac_mfma(xa, prec, s));
+      else HIPCHK(launch_xmac_f32(xa, prec, s));
       if ((rc = j->ev_end())) return rc;
     }
     j->tap_chunk0 = c0;

@@ -21,7 +21,8 @@ struct DevStation {
   int bpbnum, bpbden;
   int fractionalLo, deltapol;
   int band0;            // global index of recorded band 0 of this datastream
-  int pad;
+  int slog;             // log2(s) when s is a power of two, else -1
+  int bpf;              // bytes per band per frame in the corner-turned streams
   double sampletime, bw;
   float lut[4];         // 2-bit level table (VDIF or Mark5B order)
 };
@@ -44,12 +45,16 @@ struct DevStationSub {
   int datasec, datans, usable, nflags;
   int flagoff;          // offset of this station's valid flags
   int pad;
+  long long ctbase;     // byte offset of band 0's stream in the corner-turn buffer
+  long long ctstride;   // bytes between consecutive bands' streams (multiple of 4)
   double interp[3];
 };
 
 // ---- per block, per datastream (Mode::process prologue result) -----------
 struct DevBlock {
   int status, nearestsample, integerdelay, intwall;
+  int allvalid;         // every sample of the block decodes from a valid group
+  int pad;
   float weight, fse;
   long long validend;   // samples at or after this index decode as zero
   double a, b, fracwall;

@@ -14,6 +14,8 @@ struct FStageArgs {
   const int2* ftab;
   const float* lut8;
   const float2* tw;           // e^{-2 pi i k / L}, k < L
+  const unsigned char* ct;    // corner-turned per-band sample streams
+  const int* bandlist;        // global band per grid row (null: row = band)
   void* spectra;
   int prec;                   // FXG_XMAC_*
   float scale;                // applied before reduced-precision stores
@@ -33,13 +35,17 @@ struct XmacArgs {
 
 hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
                          long long maxframes, hipStream_t stream);
+hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,
+                             long long maxframes, int maxdatabytes, hipStream_t stream);
 hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
                            const int2* ftab, DevSubint sub, DevBlock* out, int nst, hipStream_t stream);
-hipError_t launch_fstage(int L, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
+hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
 hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const int2* ftab, const float* lut8,
                          long long first, int nsamp, int nchan, long long validend, float* out,
                          hipStream_t stream);
 hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream);
+bool xmac_mfma_supported(int nin, int nchan);
+hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream);
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream);
 hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w,

@@ -251,6 +251,10 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     REQUIRE(st.C % st.s == 0 && st.s <= 256 && P.L / st.s <= 256, FXG_EINVAL,
             "array stride must divide nchan and give at most 256 sub/step points");
     st.nfrsteps = P.L / st.s; st.nfracsteps = st.C / st.s;
+    st.slog = -1;
+    if (power_of_two(st.s)) { st.slog = 0; while ((1 << st.slog) < st.s) st.slog++; }
+    REQUIRE(st.databytes % ds.nrecbands == 0, FXG_EUNSUPPORTED, "payload must split evenly over the bands");
+    st.bpf = st.databytes / ds.nrecbands;
     st.bpbnum = bitsper / 8;
     st.bpbden = 1;
     if (st.bpbnum == 0) { st.bpbnum = 1; st.bpbden = 8 / bitsper; }

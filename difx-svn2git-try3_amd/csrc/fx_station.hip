@@ -114,7 +114,7 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   const DevStation S = st[s];
   const DevStationSub Q = ss[s];
   DevBlock g;
-  g.status = 0; g.nearestsample = 0; g.integerdelay = 0; g.intwall = 0;
+  g.status = 0; g.nearestsample = 0; g.integerdelay = 0; g.intwall = 0; g.allvalid = 0; g.pad = 0;
   g.weight = 0.0f; g.fse = 0.0f; g.validend = 0x7fffffffffffffffLL; g.a = 0; g.b = 0; g.fracwall = 0;
   bool ok = Q.usable != 0;
   if (ok) {
@@ -167,6 +167,7 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
     t = fe;
   }
   g.weight = (float)good / (float)S.L;
+  g.allvalid = good == S.L ? 1 : 0;
   if (!(g.weight > 0.0f)) { g.status = 3; out[(long long)s * sub.nblocks + index] = g; return; }
   double nearestsampletime = nearestsample * S.sampletime;
   g.fse = (float)(starttime - nearestsampletime);
@@ -285,22 +286,66 @@ template <int L> struct FftShape {
   static constexpr int NPASS16 = LOG / 4 - ((LOG % 4) ? 0 : 1);  // radix-16 passes after the first
   static constexpr int NTF = L / 16;                              // FFT threads
   static constexpr int NT = NTF < 64 ? 64 : NTF;                  // block size
+  static constexpr int PAD = L + L / 16;                          // padded LDS length
 };
 
+// LDS index with one pad slot per 16 complex values: spreads the strided
+// Stockham writes of the early passes over the banks
+__device__ __forceinline__ int pidx(int a) { return a + (a >> 4); }
 
+// ---------------------------------------------------------------------------
+// k_cornerturn: one workgroup per (frame, datastream).  De-interleaves the
+// frame payload (sample-time major, channel minor) into one contiguous
+// sample stream per band, nbits per sample, LSB first, so that the F-stage
+// of one band reads L*nbits/8 contiguous bytes per FFT block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cornerturn(const DevStation* __restrict__ st,
+                                                    const DevStationSub* __restrict__ ss,
+                                                    unsigned char* __restrict__ ct) {
+  extern __shared__ unsigned char pay[];
+  const int s = blockIdx.y;
+  const DevStation S = st[s];
+  const DevStationSub Q = ss[s];
+  const long long f = blockIdx.x;
+  if (f >= Q.nframes) return;
+  const unsigned char* src = Q.data + f * S.framebytes + S.headerbytes;
+  if ((((unsigned long long)Q.data | (unsigned long long)S.framebytes | (unsigned long long)S.headerbytes) & 3) == 0) {
+    const unsigned* s32 = (const unsigned*)src;
+    unsigned* p32 = (unsigned*)pay;
+    for (int i = threadIdx.x; i < S.databytes / 4; i += blockDim.x) p32[i] = s32[i];
+  } else {
+    for (int i = threadIdx.x; i < S.databytes; i += blockDim.x) pay[i] = src[i];
+  }
+  __syncthreads();
+  const int nbits = S.nbits, nchan = S.nchan, bpf = S.bpf;
+  const int per = 8 / nbits;                   // samples per output byte
+  const unsigned mask = (1u << nbits) - 1u;
+  for (int ob = threadIdx.x; ob < nchan * bpf; ob += blockDim.x) {
+    const int c = ob / bpf, i = ob - c * bpf;
+    unsigned v = 0;
+    for (int k = 0; k < per; k++) {
+      const int bit = ((i * per + k) * nchan + c) * nbits;
+      v |= ((pay[bit >> 3] >> (bit & 7)) & mask) << (k * nbits);
+    }
+    ct[Q.ctbase + (long long)c * Q.ctstride + f * bpf + i] = (unsigned char)v;
+  }
+}
 
-template <int L>
+template <int L, int NBITS>
 __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   using Sh = FftShape<L>;
   constexpr int NT = Sh::NT, NTF = Sh::NTF, R0 = Sh::R0;
-  __shared__ float2 buf[L];
+  constexpr int RAWW = (L * NBITS) / 32 + 2;    // staged stream words of one block
+  __shared__ float2 buf[Sh::PAD];
   __shared__ float2 rsub[256], rstep[256], fsub[256], fstep[256];
-  __shared__ float lut8s[256];
+  __shared__ unsigned raw[RAWW];
+  __shared__ float lut8s[NBITS == 8 ? 256 : 1];
 
   const int tid = threadIdx.x;
   const int cb = blockIdx.x;                      // block within chunk
   const int blk = A.chunk0 + cb;
-  const DevBand B = A.bands[blockIdx.y];
+  const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
+  const DevBand B = A.bands[gband];
   const DevStation S = A.st[B.station];
   const DevStationSub Q = A.ss[B.station];
   const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
@@ -316,6 +361,17 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       for (int c = tid; c < C; c += NT) o[c] = 0u;
     }
     return;
+  }
+
+  const long long t0 = G.nearestsample;
+  const bool fast = G.allvalid != 0;
+  // ---- stage the block's samples of this band (corner-turned stream)
+  long long bit0 = t0 * NBITS;
+  if (fast) {
+    const long long w0 = bit0 >> 5;
+    const unsigned* src = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
+    const long long nwords = Q.ctstride >> 2;
+    for (int i = tid; i < RAWW; i += NT) raw[i] = (w0 + i < nwords) ? src[w0 + i] : 0u;
   }
 
   // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
@@ -363,13 +419,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       float sn, cs; sincosf(cf * val, &sn, &cs);
       fstep[j] = make_float2(cs, sn);
     }
-    if (S.nbits == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
+    if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
   }
   __syncthreads();
 
-  float lut2[4] = {S.lut[0], S.lut[1], S.lut[2], S.lut[3]};
-  const int s = S.s;
-  const long long t0 = G.nearestsample;
+  const float lut2[4] = {S.lut[0], S.lut[1], S.lut[2], S.lut[3]};
+  const int s = S.s, slog = S.slog;
+  const int bitoff = (int)(bit0 & 31);
 
   // ---- first pass (Ns = 1): decode, rotate, radix-R0 DFT, scatter to LDS
   if (tid < NTF) {
@@ -380,13 +436,22 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
 #pragma unroll
       for (int r = 0; r < R0; r++) {
         const int n = j + r * (L / R0);
-        float x = sample_value(S, Q, A.ftab, lut8s, lut2, t0 + n, G.validend, B.band);
-        float2 rot = n < s ? rsub[n] : cmulf(rsub[n % s], rstep[n / s]);
+        float x;
+        if (fast) {
+          const int bp = bitoff + n * NBITS;
+          const unsigned code = (raw[bp >> 5] >> (bp & 31)) & ((1u << NBITS) - 1u);
+          x = NBITS == 8 ? lut8s[code] : lut2[code & 3];
+        } else {
+          x = sample_value(S, Q, A.ftab, NBITS == 8 ? lut8s : A.lut8, lut2, t0 + n, G.validend, B.band);
+        }
+        float2 rot;
+        if (slog >= 0) rot = n < s ? rsub[n] : cmulf(rsub[n & (s - 1)], rstep[n >> slog]);
+        else rot = n < s ? rsub[n] : cmulf(rsub[n % s], rstep[n / s]);
         v[r] = make_float2(x * rot.x, x * rot.y);
       }
       DFT<R0>::run(v);
 #pragma unroll
-      for (int r = 0; r < R0; r++) buf[j * R0 + r] = v[r];
+      for (int r = 0; r < R0; r++) buf[pidx(j * R0 + r)] = v[r];
     }
   }
   __syncthreads();
@@ -398,21 +463,21 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     float2 v[16];
     const int j = tid;
     if (tid < NTF) {
-      const int k = j % Ns;
+      const int k = j & (Ns - 1);
       const int twstep = L / (Ns * 16);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        float2 x = buf[j + r * (L / 16)];
-        v[r] = r == 0 ? x : cmulf(x, A.tw[(r * k * twstep) & (L - 1)]);
+        float2 x = buf[pidx(j + r * (L / 16))];
+        v[r] = r == 0 ? x : cmulf(x, A.tw[r * k * twstep]);
       }
       DFT<16>::run(v);
     }
     __syncthreads();
     if (tid < NTF) {
-      const int k = j % Ns;
+      const int k = j & (Ns - 1);
       const int idxD = (j / Ns) * Ns * 16 + k;
 #pragma unroll
-      for (int r = 0; r < 16; r++) buf[idxD + r * Ns] = v[r];
+      for (int r = 0; r < 16; r++) buf[pidx(idxD + r * Ns)] = v[r];
     }
     __syncthreads();
     Ns *= 16;
@@ -424,7 +489,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     const int j = tid;                     // j < Ns, so k = j and the bin is j + r*L/16
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      float2 x = buf[j + r * (L / 16)];
+      float2 x = buf[pidx(j + r * (L / 16))];
       v[r] = r == 0 ? x : cmulf(x, A.tw[r * j]);
     }
     DFT<16>::run(v);
@@ -433,7 +498,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     for (int q = 0; q < 8; q++) {
       const int c = j + q * (L / 16);      // output channel
       float2 x = v[rbase + q];
-      float2 fr = cmulf(fsub[c % s], fstep[c / s]);
+      float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
       float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
       if (A.prec == FXG_XMAC_FP32) {
         ((float2*)A.spectra)[outbase + c] = y;
@@ -448,7 +513,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   }
 }
 
-#define INSTANTIATE(Lv) template __global__ void k_fstage<Lv>(FStageArgs);
+#define INSTANTIATE(Lv) template __global__ void k_fstage<Lv, 2>(FStageArgs); \
+                        template __global__ void k_fstage<Lv, 8>(FStageArgs);
 INSTANTIATE(128) INSTANTIATE(256) INSTANTIATE(512) INSTANTIATE(1024)
 INSTANTIATE(2048) INSTANTIATE(4096) INSTANTIATE(8192)
 
@@ -470,14 +536,26 @@ hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const 
   return hipGetLastError();
 }
 
-hipError_t launch_fstage(int L, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream) {
+hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream) {
   dim3 grid(nblocks, nbands);
+  if (nbits != 2 && nbits != 8) return hipErrorInvalidValue;
   switch (L) {
-#define CASE(Lv) case Lv: hipLaunchKernelGGL(k_fstage<Lv>, grid, dim3(FftShape<Lv>::NT), 0, stream, a); break;
+#define CASE(Lv) case Lv: \
+      if (nbits == 2) hipLaunchKernelGGL((k_fstage<Lv, 2>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
+      else hipLaunchKernelGGL((k_fstage<Lv, 8>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
+      break;
     CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192)
 #undef CASE
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,
+                             long long maxframes, int maxdatabytes, hipStream_t stream) {
+  if (maxframes <= 0) return hipSuccess;
+  dim3 grid((unsigned)maxframes, nst);
+  hipLaunchKernelGGL(k_cornerturn, grid, dim3(256), (size_t)maxdatabytes, stream, st, ss, ct);
   return hipGetLastError();
 }
 

@@ -1,0 +1,138 @@
+// fx_xmac_mfma.hip — reduced-precision cross-multiply-accumulate on the
+// gfx950 matrix cores (v_mfma_f32_32x32x16_{f16,bf16}, FP32 accumulation).
+//
+// For one frequency and one channel c, with X_a(t) the spectrum of input a
+// (datastream band) at block t, the visibility matrix
+//     V[a][b] = sum_t X_a(t) conj(X_b(t))                 (core.cpp:960)
+// is two real products over K = 2T, k = 2t + {0 re, 1 im}:
+//     Re V = A A^T,  A[a][2t] = re X_a(t),  A[a][2t+1] = im X_a(t)
+//     Im V = Ah A^T, Ah[a][2t] = im X_a(t), Ah[a][2t+1] = -re X_a(t)
+// The interleaved (re, im) half2 spectrum is the A operand as stored, and Ah
+// is a half swap with one sign flip.  Re V is symmetric and Im V
+// antisymmetric, so only the tiles (I <= J) of the 32x32 tile grid are
+// accumulated.  One wavefront owns one channel; a workgroup of 16 wavefronts
+// owns 16 consecutive channels and stages 32 blocks of every input through
+// LDS, transposed to [channel][input][block] so that each lane's operand is
+// one 16-byte LDS read.
+#include <hip/hip_runtime.h>
+#include "fx_kernels.h"
+
+namespace fxg {
+
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+typedef float f16_t __attribute__((ext_vector_type(16)));
+
+template <bool BF>
+__device__ __forceinline__ f16_t mfma(uint4 a, uint4 b, f16_t c) {
+  if constexpr (BF) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8_t, a), __builtin_bit_cast(b8_t, b), c, 0, 0, 0);
+  } else {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8_t, a), __builtin_bit_cast(h8_t, b), c, 0, 0, 0);
+  }
+}
+
+// (re, im) -> (im, -re) on each packed 16-bit pair
+__device__ __forceinline__ unsigned hat(unsigned v) { return (v >> 16) | ((v ^ 0x8000u) << 16); }
+
+template <int T, bool BF>
+__global__ __launch_bounds__(1024) void k_xmac_mfma(XmacArgs A) {
+  constexpr int CG = 16;            // channels (wavefronts) per workgroup
+  constexpr int TT = 32;            // blocks staged per pass
+  constexpr int TP = TT + 4;        // padded row length in dwords
+  constexpr int NR = 32 * T;        // inputs, padded to the tile size
+  constexpr int NTILE = T * (T + 1) / 2;
+  __shared__ unsigned lds[CG * NR * TP];
+
+  const unsigned* spec = (const unsigned*)A.spec;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int c0 = blockIdx.x * CG;
+  const int half = l >> 5, row = l & 31;
+
+  f16_t accR[NTILE], accI[NTILE];
+#pragma unroll
+  for (int q = 0; q < NTILE; q++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) { accR[q][e] = 0.0f; accI[q][e] = 0.0f; }
+
+  for (int t0 = 0; t0 < A.nblocks; t0 += TT) {
+    // ---- stage: NR inputs x TT blocks x CG channels, 16 bytes (4 channels) per item
+    constexpr int NITEM = NR * TT * (CG / 4);
+    for (int it = tid; it < NITEM; it += 1024) {
+      const int q = it & 3;
+      const int t = (it >> 2) % TT;
+      const int a = it / (4 * TT);
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (a < A.nin && t0 + t < A.nblocks) {
+        const long long e = A.in_off[a] + (long long)(t0 + t) * A.block_stride + c0 + 4 * q;
+        if ((e & 3) == 0) {
+          v = *(const uint4*)(spec + e);
+        } else {
+          v.x = spec[e]; v.y = spec[e + 1]; v.z = spec[e + 2]; v.w = spec[e + 3];
+        }
+      }
+      unsigned* d = lds + ((4 * q) * NR + a) * TP + t;
+      d[0] = v.x; d[NR * TP] = v.y; d[2 * NR * TP] = v.z; d[3 * NR * TP] = v.w;
+    }
+    __syncthreads();
+    // ---- K steps of 8 blocks (16 real k values) each
+    const unsigned* base = lds + (w * NR) * TP;
+#pragma unroll
+    for (int ks = 0; ks < TT / 8; ks++) {
+      uint4 f[T], fh[T];
+#pragma unroll
+      for (int I = 0; I < T; I++) {
+        f[I] = *(const uint4*)(base + (32 * I + row) * TP + 8 * ks + 4 * half);
+        fh[I] = make_uint4(hat(f[I].x), hat(f[I].y), hat(f[I].z), hat(f[I].w));
+      }
+      int q = 0;
+#pragma unroll
+      for (int I = 0; I < T; I++)
+#pragma unroll
+        for (int J = I; J < T; J++, q++) {
+          accR[q] = mfma<BF>(f[I], f[J], accR[q]);
+          accI[q] = mfma<BF>(fh[I], f[J], accI[q]);
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- write V[a][b][c] for this wavefront's channel
+  const int c = c0 + w;
+  if (c >= A.nchan) return;
+  int q = 0;
+#pragma unroll
+  for (int I = 0; I < T; I++)
+#pragma unroll
+    for (int J = I; J < T; J++, q++) {
+      const int b = 32 * J + row;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int a = 32 * I + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (a >= A.nin || b >= A.nin) continue;
+        float2* p = A.V + ((long long)a * A.nin + b) * A.nchan + c;
+        float2 v = make_float2(accR[q][r], accI[q][r]);
+        if (A.accumulate) { float2 o = *p; v.x += o.x; v.y += o.y; }
+        *p = v;
+      }
+    }
+}
+
+bool xmac_mfma_supported(int nin, int nchan) { return nin <= 64 && nchan % 16 == 0; }
+
+hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream) {
+  if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(a.nchan / 16));
+  const bool bf = prec == FXG_XMAC_BF16;
+  if (a.nin <= 32) {
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma<1, true>), grid, dim3(1024), 0, stream, a);
+    else hipLaunchKernelGGL((k_xmac_mfma<1, false>), grid, dim3(1024), 0, stream, a);
+  } else {
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma<2, true>), grid, dim3(1024), 0, stream, a);
+    else hipLaunchKernelGGL((k_xmac_mfma<2, false>), grid, dim3(1024), 0, stream, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fxg

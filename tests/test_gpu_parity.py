@@ -212,10 +212,18 @@ def test_visibilities_fp32(gpu, oracle, mk):
     g.close()
 
 
+def wide_spec(**kw):
+    """40 inputs (20 stations x 2 pol): two 32-row tiles on the matrix cores."""
+    base = dict(nstations=20, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=40)
+    base.update(kw)
+    return JobSpec(**base)
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_visibilities_mixed_vs_fp64(gpu, oracle, prec):
+@pytest.mark.parametrize("mk", ["c1", "c2", "wide"])
+def test_visibilities_mixed_vs_fp64(gpu, oracle, prec, mk):
     lib, ctx = gpu
-    spec = c1_spec(xmac=prec)
+    spec = {"c1": c1_spec, "c2": c2_spec, "wide": wide_spec}[mk](xmac=prec)
     job, keep = spec.build()
     sd = make_data(spec)
     g = GpuJob(lib, ctx, job)
