@@ -197,7 +197,20 @@ def main():
     ap.add_argument("--chunk", type=int, default=2000,
                     help="FFT blocks of spectra resident in HBM at once (default: the whole subint)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = 1 << 30
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(7)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 3}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",

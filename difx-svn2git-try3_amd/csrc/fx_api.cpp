@@ -313,6 +313,16 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   int rc = build_plan(desc, &j->plan, &g_err);
   if (rc) { delete j; return rc; }
   Plan& P = j->plan;
+  {
+    // device-side limits of this build
+    std::string why;
+    if (!P.uniform) why = "all datastreams of a job must share nchan";
+    else if (!(P.L >= 128 && P.L <= 8192 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 8192]";
+    for (const DevStation& st : P.st)
+      if (why.empty() && 2 * st.s + st.L / st.s + st.C / st.s > 512)
+        why = "array stride gives more than 512 rotator sub/step points";
+    if (!why.empty()) { delete j; return set_err(FXG_EUNSUPPORTED, why); }
+  }
   // own copies of the descriptor tables
   j->desc = *desc;
   j->freqs.assign(desc->freqs, desc->freqs + desc->nfreqs);
@@ -342,11 +352,16 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   }
   std::vector<int2> blp(std::max(1, P.nbl));
   for (int b = 0; b < P.nbl; b++) blp[b] = make_int2(desc->baselines[b].ds1, desc->baselines[b].ds2);
-  // twiddles e^{-2 pi i k/L} in double, rounded once
-  std::vector<float2> tw(P.L);
-  for (int k = 0; k < P.L; k++) {
+  // split twiddle tables e^{-2 pi i m/L} = hi[m>>6] * lo[m&63], each entry
+  // evaluated in double and rounded once: lo[64] then hi[L/64]
+  std::vector<float2> tw(64 + P.L / 64);
+  for (int k = 0; k < 64; k++) {
     double ang = -kTwoPi * (double)k / (double)P.L;
     tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  for (int h = 0; h < P.L / 64; h++) {
+    double ang = -kTwoPi * (double)(64 * h) / (double)P.L;
+    tw[64 + h] = make_float2((float)std::cos(ang), (float)std::sin(ang));
   }
   size_t specbytes = (size_t)P.nbands_total * chunk * P.C * (prec == FXG_XMAC_FP32 ? 8 : 4);
   if ((rc = j->d_st.ensure(sizeof(DevStation) * P.st.size())) ||

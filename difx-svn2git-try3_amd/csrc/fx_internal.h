@@ -95,7 +95,8 @@ struct Plan {
   std::vector<long long> acoff, acwoff;
   std::vector<int> arraystride;
   int xmacstride = 0, rotatestride = 0, bandsperac = 1;
-  int C = 0, L = 0;                                    // common to all datastreams
+  int C = 0, L = 0;                                    // of datastream 0
+  bool uniform = true;                                 // every datastream has nchan == C
   int nbands_total = 0;                                // recorded bands over all datastreams
   std::vector<DevStation> st;
   std::vector<DevBand> bands;

@@ -234,9 +234,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       REQUIRE(fi.nchan == f0.nchan && fi.bw_mhz == f0.bw_mhz, FXG_EUNSUPPORTED,
               "recorded freqs of a datastream must share nchan and bandwidth");
     }
-    REQUIRE(f0.nchan == P.C, FXG_EUNSUPPORTED, "all datastreams of a job must share nchan");
-    REQUIRE(power_of_two(P.L) && P.L >= 128 && P.L <= 8192, FXG_EUNSUPPORTED,
-            "FFT length must be a power of two in [128, 8192]");
+    if (f0.nchan != P.C) P.uniform = false;
     DevStation st{};
     st.format = ds.format; st.nbits = ds.nbits; st.nchan = ds.nrecbands;
     st.framebytes = ds.framebytes; st.headerbytes = ds.headerbytes;
@@ -247,10 +245,9 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     st.groupbytes = bitsper >= 8 ? bitsper / 8 : 1;
     st.groupsamples = bitsper >= 8 ? 1 : 8 / bitsper;
     st.quirk8 = (ds.format == FXG_FORMAT_VDIF && ds.nbits == 8 && ds.nrecbands >= 2 && ds.nrecbands <= 4);
-    st.C = P.C; st.L = P.L; st.s = P.arraystride[s];
-    REQUIRE(st.C % st.s == 0 && st.s <= 256 && P.L / st.s <= 256, FXG_EINVAL,
-            "array stride must divide nchan and give at most 256 sub/step points");
-    st.nfrsteps = P.L / st.s; st.nfracsteps = st.C / st.s;
+    st.C = f0.nchan; st.L = 2 * f0.nchan; st.s = P.arraystride[s];
+    REQUIRE(st.C % st.s == 0, FXG_EINVAL, "array stride must divide nchan");
+    st.nfrsteps = st.L / st.s; st.nfracsteps = st.C / st.s;
     st.slog = -1;
     if (power_of_two(st.s)) { st.slog = 0; while ((1 << st.slog) < st.s) st.slog++; }
     REQUIRE(st.databytes % ds.nrecbands == 0, FXG_EUNSUPPORTED, "payload must split evenly over the bands");

@@ -289,6 +289,27 @@ template <int L> struct FftShape {
   static constexpr int PAD = L + L / 16;                          // padded LDS length
 };
 
+// twiddle e^{-2 pi i m/L} from split tables (m < L): one complex product
+__device__ __forceinline__ float2 tw_of(const float2* lo, const float2* hi, int m) {
+  return cmulf(hi[m >> 6], lo[m & 63]);
+}
+
+// w[r] = e^{-2 pi i r m/L} for r < 16 (r*m < L): w1, w2, w4, w8 from the
+// tables and the rest as products of at most three of them
+__device__ __forceinline__ void twiddles16(const float2* lo, const float2* hi, int m, float2* w) {
+  w[0] = make_float2(1.f, 0.f);
+  w[1] = tw_of(lo, hi, m);
+  w[2] = tw_of(lo, hi, 2 * m);
+  w[4] = tw_of(lo, hi, 4 * m);
+  w[8] = tw_of(lo, hi, 8 * m);
+  w[3] = cmulf(w[1], w[2]);
+  w[5] = cmulf(w[1], w[4]);
+  w[6] = cmulf(w[2], w[4]);
+  w[7] = cmulf(w[3], w[4]);
+#pragma unroll
+  for (int r = 1; r < 8; r++) w[8 + r] = cmulf(w[r], w[8]);
+}
+
 // LDS index with one pad slot per 16 complex values: spreads the strided
 // Stockham writes of the early passes over the banks
 __device__ __forceinline__ int pidx(int a) { return a + (a >> 4); }
@@ -337,7 +358,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   constexpr int NT = Sh::NT, NTF = Sh::NTF, R0 = Sh::R0;
   constexpr int RAWW = (L * NBITS) / 32 + 2;    // staged stream words of one block
   __shared__ float2 buf[Sh::PAD];
-  __shared__ float2 rsub[256], rstep[256], fsub[256], fstep[256];
+  __shared__ float2 tabs[512];                  // rotator sub/step tables, packed
+  __shared__ float2 twlo[64], twhi[L / 64];     // e^{-2 pi i m/L}, e^{-2 pi i 64h/L}
   __shared__ unsigned raw[RAWW];
   __shared__ float lut8s[NBITS == 8 ? 256 : 1];
 
@@ -373,6 +395,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     const long long nwords = Q.ctstride >> 2;
     for (int i = tid; i < RAWW; i += NT) raw[i] = (w0 + i < nwords) ? src[w0 + i] : 0u;
   }
+
+  float2* rsub = tabs;
+  float2* rstep = tabs + S.s;
+  float2* fsub = rstep + S.nfrsteps;
+  float2* fstep = fsub + S.s;
+  for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
+  for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
 
   // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
   {
@@ -465,10 +494,12 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     if (tid < NTF) {
       const int k = j & (Ns - 1);
       const int twstep = L / (Ns * 16);
+      float2 w[16];
+      twiddles16(twlo, twhi, k * twstep, w);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
         float2 x = buf[pidx(j + r * (L / 16))];
-        v[r] = r == 0 ? x : cmulf(x, A.tw[r * k * twstep]);
+        v[r] = r == 0 ? x : cmulf(x, w[r]);
       }
       DFT<16>::run(v);
     }
@@ -487,10 +518,12 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   if (tid < NTF) {
     float2 v[16];
     const int j = tid;                     // j < Ns, so k = j and the bin is j + r*L/16
+    float2 w[16];
+    twiddles16(twlo, twhi, j, w);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       float2 x = buf[pidx(j + r * (L / 16))];
-      v[r] = r == 0 ? x : cmulf(x, A.tw[r * j]);
+      v[r] = r == 0 ? x : cmulf(x, w[r]);
     }
     DFT<16>::run(v);
     const int rbase = B.lsb ? 8 : 0;       // USB keeps bins [0,C), real LSB keeps [C,2C)

@@ -20,7 +20,7 @@ REPO = os.path.dirname(HERE)
 FXG_MAX_FREQS = 64
 FXG_MAX_RECFREQS = 32
 FXG_MAX_BANDS = 64
-FXG_MAX_ZOOMBANDS = 32
+FXG_MAX_ZOOMBANDS = 128
 FXG_MAX_BLFREQS = 48
 FXG_MAX_POLPRODUCTS = 4
 

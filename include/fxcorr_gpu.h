@@ -38,7 +38,7 @@ extern "C" {
 #define FXG_MAX_FREQS        64   /* frequency table entries per job          */
 #define FXG_MAX_RECFREQS     32   /* recorded freqs per datastream            */
 #define FXG_MAX_BANDS        64   /* recorded bands per datastream            */
-#define FXG_MAX_ZOOMBANDS    32   /* zoom bands per datastream                */
+#define FXG_MAX_ZOOMBANDS   128   /* zoom bands per datastream                */
 #define FXG_MAX_BLFREQS      48   /* frequencies per baseline                 */
 #define FXG_MAX_POLPRODUCTS   4   /* pol products per baseline frequency      */
 

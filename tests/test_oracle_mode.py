@@ -1,0 +1,144 @@
+"""Pins the oracle's station processing (Mode::process, mode.cpp:649-1397)
+and cross-multiplication (Core::processdata, core.cpp:826-975) against
+closed forms and physics, independently of the sub/step rotator structure:
+
+  - block geometry: nearest sample rounds the model start time to the sample
+    grid and the fractional-sample error stays within half a sample;
+  - one station block: FFT of x[n] exp(-2 pi i (lofreq (a n/L + b) -
+    looffset t_n)) evaluated directly, times the fractional-sample rotation
+    exp(2 pi i (c - C lsb) BW/C (fse - clock)), in FP64;
+  - autocorrelation Parseval identity with a zero delay model;
+  - two stations with identical data whose models differ by delta: the
+    cross spectrum's phase is 2 pi (sky frequency) delta, channel by channel
+    (sign and scale of fringe and fractional-sample corrections and the
+    conj(second station) order of the product);
+  - FP32 flavour against the FP64 flavour.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fxcorr import JobSpec, check
+from fxsynth import make_subint
+from fxtestlib import oracle_block_info, oracle_results, oracle_spectra, plan_info, rel_err
+
+
+def unpacked(orc, spec, job, sd, ds, ns):
+    d = job.datastreams[ds]
+    out = np.zeros(spec.nrecbands * spec.fftchannels, dtype=np.float32)
+    w = C.c_float()
+    raw = sd.stations[ds].raw
+    check(orc, orc.orc_unpack(C.byref(d), raw.ctypes.data, raw.size, ns, spec.fftchannels, out.ctypes.data,
+                              C.byref(w)), "unpack")
+    return out.reshape(spec.nrecbands, spec.fftchannels).astype(np.float64)
+
+
+def model_starttime(spec, sd, ds, index):
+    st = sd.stations[ds]
+    ip = st.interpolator
+    fc = index + 0.5
+    delay = ip[0] * fc * fc + ip[1] * fc + ip[2]
+    ts = spec.sampletime_us
+    return (sd.sub_sec - st.datasec) * 1e6 + (sd.sub_ns - st.datans) / 1e3 + index * spec.fftchannels * ts - delay
+
+
+def test_block_geometry_properties(oracle):
+    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=64)
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=2, delays_us=[0.37, -12.81, 44.449], rates_us_per_s=[0.9, -0.7, 0.33])
+    ts = spec.sampletime_us
+    for ds in range(3):
+        for blk in range(0, 64, 7):
+            bi = oracle_block_info(oracle, job, sd, ds, blk)
+            assert bi.status == 0
+            t = model_starttime(spec, sd, ds, blk)
+            assert bi.nearestsample == int(np.floor(t / ts + 0.5))
+            assert abs(bi.fracsampleerror - np.float32(t - bi.nearestsample * ts)) == 0.0
+            assert abs(bi.fracsampleerror) <= ts / 2 + 1e-6
+            ip = sd.stations[ds].interpolator
+            d0 = ip[0] * blk * blk + ip[1] * blk + ip[2]
+            d2 = ip[0] * (blk + 1) ** 2 + ip[1] * (blk + 1) + ip[2]
+            assert abs(bi.a - (d2 - d0)) < 1e-12
+            assert bi.integerdelay == int(np.trunc(bi.b + bi.integerdelay))
+
+
+@pytest.mark.parametrize("lsb", [0, 1])
+@pytest.mark.parametrize("looff", [0.0, 1234.567])
+def test_station_spectrum_closed_form(oracle, lsb, looff):
+    spec = JobSpec(nstations=2, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=16,
+                   base_freq_mhz=8400.123, lsb=[lsb, lsb], clockoffset_us=[0.013, -0.021],
+                   looffset_hz=[looff, -looff])
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=7, delays_us=[3.2111, -7.777], rates_us_per_s=[0.8, -0.3])
+    L, Cn, ts, bw = spec.fftchannels, spec.nchan, spec.sampletime_us, spec.bw_mhz
+    for ds in range(2):
+        for blk in (0, 9):
+            bi = oracle_block_info(oracle, job, sd, ds, blk)
+            assert bi.status == 0
+            x = unpacked(oracle, spec, job, sd, ds, bi.nearestsample)
+            got = oracle_spectra(oracle, job, sd, ds, blk, spec.nrecbands, Cn, fp64=True)
+            fse = model_starttime(spec, sd, ds, blk) - bi.nearestsample * ts
+            walltime = bi.intwalltime + bi.fracwalltime
+            n = np.arange(L)
+            c = np.arange(Cn)
+            lo_off = looff if ds == 0 else -looff
+            clk = 0.013 if ds == 0 else -0.021
+            for band in range(spec.nrecbands):
+                sb = band // spec.npol
+                lofreq = spec.base_freq_mhz + sb * bw
+                phase = lofreq * (bi.a * n / L + bi.b + bi.integerdelay) - lo_off * (n * ts * 1e-6 + walltime)
+                X = np.fft.fft(x[band] * np.exp(-2j * np.pi * phase))[c + Cn * lsb]
+                theta = 2 * np.pi * (c - Cn * lsb) * bw / Cn * (fse - clk)
+                exp = X * np.exp(1j * theta)
+                assert rel_err(got[band], exp) < 1e-9, (ds, blk, band, rel_err(got[band], exp))
+
+
+def test_autocorrelation_parseval(oracle):
+    spec = JobSpec(nstations=2, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=8)
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=9, delays_us=[0.0, 0.0])
+    res = oracle_results(oracle, job, sd, fp64=True)
+    off = C.c_int64()
+    check(oracle, oracle.orc_plan_autocorr_offset(C.byref(job), 0, C.byref(off)), "ac")
+    L, Cn = spec.fftchannels, spec.nchan
+    for band in range(spec.nrecbands):
+        ac = res[off.value + band * Cn: off.value + (band + 1) * Cn]
+        tot = 0.0
+        for blk in range(spec.blockspersend):
+            bi = oracle_block_info(oracle, job, sd, 0, blk)
+            x = unpacked(oracle, spec, job, sd, 0, bi.nearestsample)[band]
+            x0, xc = x.sum(), (x * (-1.0) ** np.arange(L)).sum()
+            tot += (L * np.sum(x * x) + x0 * x0 - xc * xc) / 2
+        assert abs(ac.real.sum() - tot) < 1e-10 * tot
+        assert np.max(np.abs(ac.imag)) == 0.0
+
+
+@pytest.mark.parametrize("lsb", [0, 1])
+def test_cross_phase_equals_sky_frequency_times_delay(oracle, lsb):
+    spec = JobSpec(nstations=2, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=8,
+                   base_freq_mhz=8400.0, lsb=[lsb])
+    job, keep = spec.build()
+    ts = spec.sampletime_us
+    delta = 0.3 * ts
+    sd = make_subint(spec, seed=10, delays_us=[0.0, 0.0])
+    sd.stations[1].raw = sd.stations[0].raw.copy()            # identical voltages
+    sd.stations[1].interpolator = np.array([0.0, 0.0, delta])  # model differs by delta
+    res = oracle_results(oracle, job, sd, fp64=True)
+    Cn, bw = spec.nchan, spec.bw_mhz
+    v = res[0:Cn]                                              # baseline 0, RR
+    c = np.arange(Cn)
+    lowest = spec.base_freq_mhz - (bw if lsb else 0.0)          # sky frequency of channel 0
+    sky = lowest + c * bw / Cn
+    want = np.exp(2j * np.pi * sky * delta)
+    got = v / np.abs(v)
+    assert np.max(np.abs(got - want)) < 1e-9
+
+
+def test_fp32_matches_fp64(oracle):
+    spec = JobSpec(nstations=4, nsubbands=2, npol=2, bw_mhz=32.0, nchan=512, blockspersend=12)
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=12, rho=0.05, delays_us=[1.1, -3.3, 5.5, 0.0], rates_us_per_s=[0.1, 0.2, -0.3, 0])
+    r32 = oracle_results(oracle, job, sd, fp64=False)
+    r64 = oracle_results(oracle, job, sd, fp64=True)
+    assert rel_err(r32, r64) < 1e-6
