@@ -35,6 +35,19 @@ HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/c3_hbm_traffic.txt), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_hbm_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_dispatch = "):
+                return json.loads(line.split("=", 1)[1]).get(kernel)
+    except OSError:
+        return None
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -265,8 +278,10 @@ def main():
                             spec.nstations * spec.nrecbands * per_launch_blocks * C_ * specbytes)
         avg_s = stage["fstage"]["ms"] / stage["fstage"]["launches"] / 1e3
         achieved = bytes_per_launch / avg_s / 1e9
-        roof = {"kernel": "k_fstage<8192>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        kname = f"fxg::k_fstage<{L}, 2>"
+        roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kname),
+                "algorithmic_bytes_per_launch": bytes_per_launch,
                 "flops_per_launch": 5.0 * L * np.log2(L) * spec.nstations * spec.nrecbands * per_launch_blocks,
                 "avg_launch_ms": avg_s * 1e3}
         roof["fp32_tflops"] = roof["flops_per_launch"] / avg_s / 1e12
@@ -277,8 +292,9 @@ def main():
         avg_s = stage["xmac"]["ms"] / stage["xmac"]["launches"] / 1e3
         achieved = flops / avg_s / 1e12
         peak = PEAK_TFLOPS[args.precision]
-        roof = {"kernel": "k_xmac", "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "avg_launch_ms": avg_s * 1e3}
+        kname = "fxg::k_xmac_mfma<2, false>" if args.precision == "fp16" else "fxg::k_xmac_mfma<2, true>"
+        roof = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": profiled_traffic(kname), "avg_launch_ms": avg_s * 1e3}
 
     errors = None
     if not args.no_verify:

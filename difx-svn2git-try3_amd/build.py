@@ -77,6 +77,25 @@ def build_host_shim(verbose: bool = False) -> str:
     return lib
 
 
+def build_host_tests(verbose: bool = False) -> list:
+    """C++ programs that drive the host shim (tests/cpp/*.cpp)."""
+    tdir = os.path.join(REPO, "tests", "cpp")
+    outs = []
+    if not os.path.isdir(tdir):
+        return outs
+    for f in sorted(os.listdir(tdir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(OUT, f[:-4])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(REPO, 'include')}", f"-I{HOST}",
+              "-o", exe, os.path.join(tdir, f), f"-L{OUT}", "-lfxcorr_host", "-lfxcorr_gpu",
+              f"-Wl,-rpath,{OUT}"])
+        outs.append(exe)
+        if verbose:
+            print("built", exe)
+    return outs
+
+
 def build_oracle(verbose: bool = False) -> str:
     """The CPU oracle (test infrastructure), via its own Makefile."""
     r = subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "-B"], capture_output=True, text=True)
@@ -88,6 +107,7 @@ def build_oracle(verbose: bool = False) -> str:
 def build_all(verbose: bool = False):
     build_gpu_lib(verbose)
     build_host_shim(verbose)
+    build_host_tests(verbose)
     build_oracle(verbose)
 
 

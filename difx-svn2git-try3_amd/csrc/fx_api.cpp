@@ -82,6 +82,9 @@ struct fxg_job {
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
+  // the subintegration whose frames / corner turn / geometry are resident
+  std::vector<unsigned char> prepared_key;
+  DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
   // stage profiling
   int profiling = 0;
   std::vector<hipEvent_t> evpool;
@@ -297,7 +300,7 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
-                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist};
+                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -391,6 +394,11 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   if ((rc = j->d_bandlist.ensure(sizeof(int) * std::max<size_t>(1, bl.size())))) return fail(rc);
   if (hipMemcpy(j->d_bandlist.p, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_err(FXG_EHIP, "band list upload failed"));
+  std::vector<int> iota(P.bands.size());
+  for (size_t g = 0; g < iota.size(); g++) iota[g] = (int)g;
+  if ((rc = j->d_iota.ensure(sizeof(int) * std::max<size_t>(1, iota.size())))) return fail(rc);
+  if (hipMemcpy(j->d_iota.p, iota.data(), sizeof(int) * iota.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(FXG_EHIP, "band index upload failed"));
   j->d_raw.resize(P.nds);
   *out = j;
   return 0;
@@ -430,13 +438,13 @@ static void xmac_shape(int nin, int nchan, XmacArgs* a) {
   a->TB = std::min(tb, 16);
 }
 
-int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
-                       int data_on_device, float* results, int results_on_device) {
-  if (!j || !sub || !st || !results) return set_err(FXG_EINVAL, "null argument");
+// Frame validity, corner turn and FP64 block geometry of one subintegration
+// (everything before the F stage).
+static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                          int data_on_device) {
   Plan& P = j->plan;
   const fxg_job_desc& D = j->desc;
   hipStream_t s = j->stream;
-  HIPCHK(hipSetDevice(j->ctx->device));
   int rc;
   const int nblocks = D.blockspersend;
   // ---- per-datastream subint table
@@ -492,22 +500,57 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   HIPCHK(launch_prologue(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_flags.as<int>(),
                          j->d_ftab.as<int2>(), dsub, j->d_blocks.as<DevBlock>(), P.nds, s));
   if ((rc = j->ev_end())) return rc;
+  return 0;
+}
+
+static void fill_fstage_args(fxg_job* j, FStageArgs* fa) {
+  *fa = FStageArgs{};
+  fa->st = j->d_st.as<DevStation>();
+  fa->bands = j->d_bands.as<DevBand>();
+  fa->ss = j->d_ss.as<DevStationSub>();
+  fa->blocks = j->d_blocks.as<DevBlock>();
+  fa->ftab = j->d_ftab.as<int2>();
+  fa->lut8 = j->ctx->lut8.as<float>();
+  fa->tw = j->d_tw.as<float2>();
+  fa->ct = j->d_ct.as<unsigned char>();
+  fa->spectra = j->d_spec.p;
+  fa->prec = j->desc.xmac_precision;
+  fa->scale = j->scale;
+  fa->nblocks_subint = j->desc.blockspersend;
+}
+
+// identity of a subintegration's inputs: the prepared state is reused while it matches
+static std::vector<unsigned char> subint_key(const fxg_job* j, const fxg_subint_desc* sub,
+                                             const fxg_station_input* st, int data_on_device) {
+  std::vector<unsigned char> k;
+  auto put = [&](const void* p, size_t n) { const unsigned char* b = (const unsigned char*)p; k.insert(k.end(), b, b + n); };
+  put(sub, sizeof(*sub));
+  put(&data_on_device, sizeof(int));
+  for (int d = 0; d < j->plan.nds; d++) {
+    put(&st[d].data, sizeof(st[d].data)); put(&st[d].nbytes, sizeof(st[d].nbytes));
+    put(&st[d].scan, 3 * sizeof(int32_t)); put(st[d].interpolator, sizeof(st[d].interpolator));
+    put(&st[d].nvalidflags, sizeof(int32_t));
+    if (st[d].validflags) put(st[d].validflags, sizeof(int32_t) * (size_t)std::max(0, st[d].nvalidflags));
+  }
+  return k;
+}
+
+int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                       int data_on_device, float* results, int results_on_device) {
+  if (!j || !sub || !st || !results) return set_err(FXG_EINVAL, "null argument");
+  Plan& P = j->plan;
+  const fxg_job_desc& D = j->desc;
+  hipStream_t s = j->stream;
+  HIPCHK(hipSetDevice(j->ctx->device));
+  int rc;
+  const int nblocks = D.blockspersend;
+  j->prepared_key.clear();
+  if ((rc = prepare_subint(j, sub, st, data_on_device))) return rc;
 
   // ---- F and X stages, chunk by chunk
   const int prec = D.xmac_precision;
-  FStageArgs fa{};
-  fa.st = j->d_st.as<DevStation>();
-  fa.bands = j->d_bands.as<DevBand>();
-  fa.ss = j->d_ss.as<DevStationSub>();
-  fa.blocks = j->d_blocks.as<DevBlock>();
-  fa.ftab = j->d_ftab.as<int2>();
-  fa.lut8 = j->ctx->lut8.as<float>();
-  fa.tw = j->d_tw.as<float2>();
-  fa.ct = j->d_ct.as<unsigned char>();
-  fa.spectra = j->d_spec.p;
-  fa.prec = prec;
-  fa.scale = j->scale;
-  fa.nblocks_subint = nblocks;
+  FStageArgs fa;
+  fill_fstage_args(j, &fa);
   for (int c0 = 0; c0 < nblocks; c0 += P.chunk_blocks) {
     int cn = std::min(P.chunk_blocks, nblocks - c0);
     fa.chunk0 = c0;
@@ -561,6 +604,34 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
   HIPCHK(hipStreamSynchronize(s));
   j->ev_collect();
   j->have_subint = 1;
+  return 0;
+}
+
+int fxg_station_process_block(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                              int data_on_device, int ds, int block, float* spectra, fxg_block_info* info) {
+  if (!j || !sub || !st || !spectra) return set_err(FXG_EINVAL, "null argument");
+  const Plan& P = j->plan;
+  if (ds < 0 || ds >= P.nds || block < 0 || block >= j->desc.blockspersend) return set_err(FXG_EINVAL, "index");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  int rc;
+  std::vector<unsigned char> key = subint_key(j, sub, st, data_on_device);
+  if (key != j->prepared_key) {
+    if ((rc = prepare_subint(j, sub, st, data_on_device))) { j->prepared_key.clear(); return rc; }
+    j->prepared_key = key;
+  }
+  FStageArgs fa;
+  fill_fstage_args(j, &fa);
+  fa.chunk0 = block;
+  fa.bandlist = j->d_iota.as<int>() + P.st[ds].band0;
+  HIPCHK(launch_fstage(P.L, P.st[ds].nbits, fa, P.st[ds].nchan, 1, j->stream));
+  HIPCHK(hipStreamSynchronize(j->stream));
+  j->ev_collect();
+  j->have_subint = 1;
+  j->tap_chunk0 = block;
+  j->tap_chunkn = 1;
+  for (int b = 0; b < P.st[ds].nchan; b++)
+    if ((rc = fxg_tap_spectrum(j, ds, b, block, spectra + 2 * (size_t)b * P.C))) return rc;
+  if (info) if ((rc = fxg_tap_block_info(j, ds, block, info))) return rc;
   return 0;
 }
 

@@ -238,6 +238,17 @@ int fxg_subint_process(fxg_job* job, const fxg_subint_desc* sub,
                        const fxg_station_input* stations, int data_on_device,
                        float* results, int results_on_device);
 
+/* Mode::process for one block of one datastream (mode.cpp:649-1397): the
+ * spectra of all its recorded bands (nrecbands x nchan cf32 as float pairs,
+ * host pointer, band-major) and the block's geometry and data weight.  The
+ * frame validity, corner turn and FP64 geometry of the whole subintegration
+ * are computed on the first call and reused while sub/stations describe the
+ * same subintegration (same pointers, lengths, times and delay models). */
+int fxg_station_process_block(fxg_job* job, const fxg_subint_desc* sub,
+                              const fxg_station_input* stations, int data_on_device,
+                              int datastream, int block, float* spectra,
+                              fxg_block_info* info);
+
 /* Per-stage device time, measured with HIP events on the job's stream
  * around every launch while profiling is on. */
 #define FXG_STAGE_FRAMES    0   /* frame validity                              */
