@@ -450,8 +450,9 @@ static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
   // ---- per-datastream subint table
   std::vector<DevStationSub> ss(P.nds);
   std::vector<int> flags;
-  long long fbase = 0, maxframes = 0, ctbytes = 0;
+  long long fbase = 0, maxframes = 0, ctbytes = 0, maxsamples = 0;
   int maxdatabytes = 0;
+  int fastW = -1;   // 2-bit register corner turn: W words per sample-time for every datastream, else 0
   for (int d = 0; d < P.nds; d++) {
     const fxg_station_input& in = st[d];
     DevStationSub& q = ss[d];
@@ -474,6 +475,15 @@ static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
     ctbytes += q.ctstride * P.st[d].nchan;
     maxdatabytes = std::max(maxdatabytes, P.st[d].databytes);
     maxframes = std::max(maxframes, q.nframes);
+    {
+      const DevStation& S = P.st[d];
+      const int W = S.nbits == 2 && S.nchan % 16 == 0 ? S.nchan / 16 : 0;
+      const bool ok = (W == 1 || W == 2 || W == 4) &&
+                      (((uintptr_t)q.data | (uintptr_t)S.framebytes | (uintptr_t)S.headerbytes |
+                        (uintptr_t)S.databytes) % (4 * W)) == 0;
+      if (q.nframes > 0) fastW = (ok && (fastW < 0 || fastW == W)) ? W : 0;
+      if (W) maxsamples = std::max(maxsamples, q.nframes * (S.databytes / (4 * W)));
+    }
     q.datasec = in.sec; q.datans = in.ns;
     // Mode::setOffsets / setData guards (mode.cpp:1525-1573, 671)
     q.usable = (sub->model_ok && in.sec > FXG_INVALID_SUBINT && in.scan == sub->scan && in.nbytes > 1) ? 1 : 0;
@@ -493,7 +503,7 @@ static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
   if ((rc = j->ev_begin(FXG_STAGE_FRAMES))) return rc;
   HIPCHK(launch_frames(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ftab.as<int2>(), P.nds, maxframes, s));
   HIPCHK(launch_cornerturn(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ct.as<unsigned char>(),
-                           P.nds, maxframes, maxdatabytes, s));
+                           P.nds, maxframes, maxdatabytes, fastW > 0 ? fastW : 0, maxsamples, s));
   if ((rc = j->ev_end())) return rc;
   DevSubint dsub{sub->sec, sub->ns, sub->scan_start_sec, nblocks};
   if ((rc = j->ev_begin(FXG_STAGE_PROLOGUE))) return rc;

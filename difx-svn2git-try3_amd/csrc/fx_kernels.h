@@ -36,7 +36,8 @@ struct XmacArgs {
 hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
                          long long maxframes, hipStream_t stream);
 hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,
-                             long long maxframes, int maxdatabytes, hipStream_t stream);
+                             long long maxframes, int maxdatabytes, int fastW, long long maxsamples,
+                             hipStream_t stream);
 hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
                            const int2* ftab, DevSubint sub, DevBlock* out, int nst, hipStream_t stream);
 hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);

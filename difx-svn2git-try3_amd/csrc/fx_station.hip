@@ -11,6 +11,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 #include "fx_kernels.h"
+#include "fx_bits.h"
+#include <type_traits>
 
 namespace fxg {
 
@@ -185,9 +187,17 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
 // ---------------------------------------------------------------------------
 // sample decoding
 // ---------------------------------------------------------------------------
+// 2-bit level lookup by selects (a dynamically indexed local array would live
+// in scratch memory)
+__device__ __forceinline__ float pick4(const float4& l, unsigned c) {
+  const float lo = (c & 1u) ? l.y : l.x;
+  const float hi = (c & 1u) ? l.w : l.z;
+  return (c & 2u) ? hi : lo;
+}
+
 __device__ __forceinline__ float sample_value(const DevStation& S, const DevStationSub& Q,
                                               const int2* __restrict__ ftab, const float* lut8,
-                                              const float lut2[4], long long t, long long validend, int band) {
+                                              const float4& lut2, long long t, long long validend, int band) {
   if (t >= validend || t < 0) return 0.0f;
   long long f = t / S.framesamples;
   if (f >= Q.nframes) return 0.0f;
@@ -199,7 +209,7 @@ __device__ __forceinline__ float sample_value(const DevStation& S, const DevStat
   int bit = ((tin - grp * S.groupsamples) * S.nchan + band) * S.nbits;
   unsigned char byte = Q.data[f * S.framebytes + S.headerbytes + ib + (bit >> 3)];
   if (S.nbits == 8) return lut8[byte];
-  return lut2[(byte >> (bit & 7)) & 3];
+  return pick4(lut2, (byte >> (bit & 7)) & 3u);
 }
 
 // standalone decode of nsamp sample-times from `first` into out[nchan][nsamp]
@@ -211,7 +221,8 @@ __global__ void k_unpack(const DevStation* __restrict__ st, const DevStationSub*
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   int c = blockIdx.y;
   if (n >= nsamp) return;
-  out[(long long)c * nsamp + n] = sample_value(S, Q, ftab, lut8, S.lut, first + n, validend, c);
+  const float4 l2 = make_float4(S.lut[0], S.lut[1], S.lut[2], S.lut[3]);
+  out[(long long)c * nsamp + n] = sample_value(S, Q, ftab, lut8, l2, first + n, validend, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,6 +363,52 @@ __global__ __launch_bounds__(256) void k_cornerturn(const DevStation* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_cornerturn2<W>: the same corner turn for 2-bit frames whose sample-times
+// are W whole 32-bit words (16 W bands), used when every datastream has that
+// shape and 4W-byte aligned frames.  Each thread turns 16 consecutive
+// sample-times of all bands with a 2-bit transpose in registers: W vector
+// loads per sample-time, one coalesced word store per band.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restrict__ st,
+                                                     const DevStationSub* __restrict__ ss,
+                                                     unsigned char* __restrict__ ct) {
+  using VW = typename std::conditional<W == 1, unsigned, typename std::conditional<W == 2, uint2, uint4>::type>::type;
+  const int s = blockIdx.y;
+  const DevStation S = st[s];
+  const DevStationSub Q = ss[s];
+  const long long spf = S.databytes / (4 * W);          // sample-times per frame
+  const long long total = Q.nframes * spf;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long tau0 = g * 16;
+  if (tau0 >= total) return;
+  long long f = tau0 / spf;
+  long long off = tau0 - f * spf;
+  uint32_t w[W][16];
+#pragma unroll
+  for (int t = 0; t < 16; t++) {
+    VW v;
+    if (tau0 + t < total) {
+      const VW* p = (const VW*)(Q.data + f * S.framebytes + S.headerbytes);
+      v = p[off];
+    } else {
+      memset(&v, 0, sizeof v);
+    }
+    const uint32_t* pv = (const uint32_t*)&v;
+#pragma unroll
+    for (int c = 0; c < W; c++) w[c][t] = pv[c];
+    if (++off == spf) { off = 0; f++; }
+  }
+#pragma unroll
+  for (int c = 0; c < W; c++) {
+    transpose16x2(w[c]);
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      ((uint32_t*)(ct + Q.ctbase + (long long)(16 * c + k) * Q.ctstride))[g] = w[c][k];
+  }
+}
+
 template <int L, int NBITS>
 __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   using Sh = FftShape<L>;
@@ -452,7 +509,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   }
   __syncthreads();
 
-  const float lut2[4] = {S.lut[0], S.lut[1], S.lut[2], S.lut[3]};
+  const float4 lut2 = make_float4(S.lut[0], S.lut[1], S.lut[2], S.lut[3]);
   const int s = S.s, slog = S.slog;
   const int bitoff = (int)(bit0 & 31);
 
@@ -469,7 +526,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
         if (fast) {
           const int bp = bitoff + n * NBITS;
           const unsigned code = (raw[bp >> 5] >> (bp & 31)) & ((1u << NBITS) - 1u);
-          x = NBITS == 8 ? lut8s[code] : lut2[code & 3];
+          x = NBITS == 8 ? lut8s[code] : pick4(lut2, code);
         } else {
           x = sample_value(S, Q, A.ftab, NBITS == 8 ? lut8s : A.lut8, lut2, t0 + n, G.validend, B.band);
         }
@@ -585,8 +642,17 @@ hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int 
 }
 
 hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,
-                             long long maxframes, int maxdatabytes, hipStream_t stream) {
+                             long long maxframes, int maxdatabytes, int fastW, long long maxsamples,
+                             hipStream_t stream) {
   if (maxframes <= 0) return hipSuccess;
+  if (fastW) {
+    const long long groups = (maxsamples + 15) / 16;
+    dim3 grid((unsigned)((groups + 255) / 256), nst);
+    if (fastW == 1) hipLaunchKernelGGL(k_cornerturn2<1>, grid, dim3(256), 0, stream, st, ss, ct);
+    else if (fastW == 2) hipLaunchKernelGGL(k_cornerturn2<2>, grid, dim3(256), 0, stream, st, ss, ct);
+    else hipLaunchKernelGGL(k_cornerturn2<4>, grid, dim3(256), 0, stream, st, ss, ct);
+    return hipGetLastError();
+  }
   dim3 grid((unsigned)maxframes, nst);
   hipLaunchKernelGGL(k_cornerturn, grid, dim3(256), (size_t)maxdatabytes, stream, st, ss, ct);
   return hipGetLastError();

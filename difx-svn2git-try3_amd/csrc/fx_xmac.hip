@@ -104,32 +104,52 @@ __global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// weights: one thread per baseline (sum over blocks of w1*w2) and per
-// datastream (sum of w), in block order
+// weights: per baseline the sum over blocks of w1*w2, per datastream the sum
+// of w, each added in block order
 // ---------------------------------------------------------------------------
-__global__ void k_weights(const DevBlock* __restrict__ blocks, int nblocks, const int2* __restrict__ bl,
-                          int nbl, int nds, float* __restrict__ wout) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nbl) {
-    int d1 = bl[i].x, d2 = bl[i].y;
-    float acc = 0.0f;
-    for (int t = 0; t < nblocks; t++) {
-      const DevBlock& g1 = blocks[(long long)d1 * nblocks + t];
-      const DevBlock& g2 = blocks[(long long)d2 * nblocks + t];
-      float w1 = g1.status == 0 ? g1.weight : 0.0f;
-      float w2 = g2.status == 0 ? g2.weight : 0.0f;
-      acc += w1 * w2;
+__global__ __launch_bounds__(256) void k_weights(const DevBlock* __restrict__ blocks, int nblocks,
+                                                 const int2* __restrict__ bl, int nbl, int nds,
+                                                 float* __restrict__ wout) {
+  // one workgroup per baseline, then one per datastream: the per-block terms
+  // are gathered in parallel into LDS and one lane adds them in block order
+  constexpr int CH = 4096;
+  __shared__ float4 term4[CH / 4];
+  float* term = (float*)term4;
+  const int i = blockIdx.x;
+  if (i >= nbl + nds) return;
+  const bool isbl = i < nbl;
+  const int d1 = isbl ? bl[i].x : i - nbl;
+  const int d2 = isbl ? bl[i].y : d1;
+  float acc = 0.0f;
+  for (int t0 = 0; t0 < nblocks; t0 += CH) {
+    const int n = min(CH, nblocks - t0);
+    for (int k = threadIdx.x; k < CH; k += blockDim.x) {
+      float v = 0.0f;
+      if (k < n) {
+        const DevBlock& g1 = blocks[(long long)d1 * nblocks + t0 + k];
+        const float w1 = g1.status == 0 ? g1.weight : 0.0f;
+        if (isbl) {
+          const DevBlock& g2 = blocks[(long long)d2 * nblocks + t0 + k];
+          v = w1 * (g2.status == 0 ? g2.weight : 0.0f);
+        } else {
+          v = w1;
+        }
+      }
+      term[k] = v;
     }
-    wout[i] = acc;
-  } else if (i < nbl + nds) {
-    int d = i - nbl;
-    float acc = 0.0f;
-    for (int t = 0; t < nblocks; t++) {
-      const DevBlock& g = blocks[(long long)d * nblocks + t];
-      acc += g.status == 0 ? g.weight : 0.0f;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < (n + 3) / 4; k++) {
+        const float4 q = term4[k];
+        acc += q.x;
+        acc += q.y;
+        acc += q.z;
+        acc += q.w;
+      }
     }
-    wout[i] = acc;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) wout[i] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -168,7 +188,8 @@ hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream) {
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream) {
   int n = nbl + nds;
-  hipLaunchKernelGGL(k_weights, dim3((n + 127) / 128), dim3(128), 0, stream, blocks, nblocks, bl, nbl, nds, w);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_weights, dim3(n), dim3(256), 0, stream, blocks, nblocks, bl, nbl, nds, w);
   return hipGetLastError();
 }
 

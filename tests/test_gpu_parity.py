@@ -147,6 +147,17 @@ def c2_spec(**kw):
     return JobSpec(**base)
 
 
+def bands_spec(nbands, **kw):
+    """Frames of 32 or 64 bands: 2 or 4 words per sample-time (register corner turn)."""
+    base = dict(nstations=3, nsubbands=nbands // 2, npol=2, bw_mhz=16.0, nchan=128, blockspersend=10)
+    base.update(kw)
+    return JobSpec(**base)
+
+
+SPECS = {"c1": c1_spec, "c2": c2_spec, "b32": lambda **kw: bands_spec(32, **kw),
+         "b64": lambda **kw: bands_spec(64, **kw)}
+
+
 def make_data(spec, seed=5, rho=0.05, **kw):
     rng = np.random.default_rng(seed + 1000)
     delays = rng.uniform(-20.0, 20.0, spec.nstations)
@@ -192,10 +203,10 @@ def test_spectra_fp32(gpu, oracle, mk):
     g.close()
 
 
-@pytest.mark.parametrize("mk", ["c1", "c2"])
+@pytest.mark.parametrize("mk", ["c1", "c2", "b32", "b64"])
 def test_visibilities_fp32(gpu, oracle, mk):
     lib, ctx = gpu
-    spec = c1_spec() if mk == "c1" else c2_spec()
+    spec = SPECS[mk]()
     job, keep = spec.build()
     sd = make_data(spec, invalid_frames={0: [1]}, invalid_blocks={1: [2]})
     g = GpuJob(lib, ctx, job)
