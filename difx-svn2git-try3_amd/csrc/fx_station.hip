@@ -290,14 +290,20 @@ template <> struct DFT<16> {
   }
 };
 
+// L = 2^LOG = 16^a * 2^q.  Pass 1 is radix 16 (Ns = 1, straight from the
+// decoder), then NMID radix-16 passes, then a last pass of radix RL = 2^q (16
+// when q = 0) that computes only the half of the bins the sideband keeps.
+// Every FFT thread holds 16 points in every pass.
 template <int L> struct FftShape {
   static constexpr int LOG = (L == 128) ? 7 : (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10
                            : (L == 2048) ? 11 : (L == 4096) ? 12 : 13;
-  static constexpr int R0 = (LOG % 4) ? (1 << (LOG % 4)) : 16;   // first-pass radix
-  static constexpr int NPASS16 = LOG / 4 - ((LOG % 4) ? 0 : 1);  // radix-16 passes after the first
+  static constexpr int Q = LOG % 4;
+  static constexpr int RL = Q ? (1 << Q) : 16;                    // last-pass radix
+  static constexpr int NMID = Q ? LOG / 4 - 1 : LOG / 4 - 2;      // radix-16 passes between
   static constexpr int NTF = L / 16;                              // FFT threads
   static constexpr int NT = NTF < 64 ? 64 : NTF;                  // block size
   static constexpr int PAD = L + L / 16;                          // padded LDS length
+  static constexpr int NSL = L / RL;                              // Ns of the last pass
 };
 
 // twiddle e^{-2 pi i m/L} from split tables (m < L): one complex product
@@ -319,6 +325,23 @@ __device__ __forceinline__ void twiddles16(const float2* lo, const float2* hi, i
   w[7] = cmulf(w[3], w[4]);
 #pragma unroll
   for (int r = 1; r < 8; r++) w[8 + r] = cmulf(w[r], w[8]);
+}
+
+// w[r] = e^{-2 pi i r m/L} for r < R (R = 2, 4, 8)
+template <int R>
+__device__ __forceinline__ void twiddlesR(const float2* lo, const float2* hi, int m, float2* w) {
+  w[0] = make_float2(1.f, 0.f);
+  w[1] = tw_of(lo, hi, m);
+  if (R >= 4) {
+    w[2] = tw_of(lo, hi, 2 * m);
+    w[3] = cmulf(w[1], w[2]);
+  }
+  if (R >= 8) {
+    w[4] = tw_of(lo, hi, 4 * m);
+    w[5] = cmulf(w[1], w[4]);
+    w[6] = cmulf(w[2], w[4]);
+    w[7] = cmulf(w[3], w[4]);
+  }
 }
 
 // LDS index with one pad slot per 16 complex values: spreads the strided
@@ -412,7 +435,7 @@ __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restric
 template <int L, int NBITS>
 __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   using Sh = FftShape<L>;
-  constexpr int NT = Sh::NT, NTF = Sh::NTF, R0 = Sh::R0;
+  constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
   constexpr int RAWW = (L * NBITS) / 32 + 2;    // staged stream words of one block
   __shared__ float2 buf[Sh::PAD];
   __shared__ float2 tabs[512];                  // rotator sub/step tables, packed
@@ -512,16 +535,39 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   const float4 lut2 = make_float4(S.lut[0], S.lut[1], S.lut[2], S.lut[3]);
   const int s = S.s, slog = S.slog;
   const int bitoff = (int)(bit0 & 31);
+  // Strides between a thread's points (j + r L/16) in the padded LDS layout
+  // and in the staged stream words; both constant when L/16 is a multiple
+  // of 16 (points) and of 32 bits (samples).
+  constexpr bool kStride16 = NTF % 16 == 0;
+  constexpr int RSTR = NTF + NTF / 16;
+  constexpr bool kWordStep = (NBITS * NTF) % 32 == 0;
+  constexpr int WSTEP = (NBITS * NTF) / 32;
+  // n & (s-1) is the same for all 16 points of a thread: one sub-rotator
+  // value per thread, a step-rotator value per point
+  const bool rotfast = slog >= 0 && (NTF & (s - 1)) == 0;
 
-  // ---- first pass (Ns = 1): decode, rotate, radix-R0 DFT, scatter to LDS
+  // ---- pass 1 (Ns = 1): decode, fringe-rotate, radix-16 DFT, to LDS
   if (tid < NTF) {
+    const int j = tid;
+    float2 v[16];
+    if (kWordStep && fast && rotfast) {
+      const int bp = bitoff + NBITS * j;
+      const unsigned* wp = raw + (bp >> 5);
+      const int sh = bp & 31;
+      const float2 r0 = rsub[j & (s - 1)];
+      const float2* stp = rstep + (j >> slog);
+      const int sstep = NTF >> slog;
 #pragma unroll
-    for (int u = 0; u < 16 / R0; u++) {
-      const int j = tid + u * NTF;
-      float2 v[R0];
+      for (int r = 0; r < 16; r++) {
+        const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
+        const float x = NBITS == 8 ? lut8s[code] : pick4(lut2, code);
+        const float2 rot = cmulf(r0, stp[r * sstep]);   // sub[k] * step[j], mode.cpp:1160-1170
+        v[r] = make_float2(x * rot.x, x * rot.y);
+      }
+    } else {
 #pragma unroll
-      for (int r = 0; r < R0; r++) {
-        const int n = j + r * (L / R0);
+      for (int r = 0; r < 16; r++) {
+        const int n = j + r * NTF;
         float x;
         if (fast) {
           const int bp = bitoff + n * NBITS;
@@ -535,27 +581,28 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
         else rot = n < s ? rsub[n] : cmulf(rsub[n % s], rstep[n / s]);
         v[r] = make_float2(x * rot.x, x * rot.y);
       }
-      DFT<R0>::run(v);
-#pragma unroll
-      for (int r = 0; r < R0; r++) buf[pidx(j * R0 + r)] = v[r];
     }
+    DFT<16>::run(v);
+    float2* o = buf + 17 * j;                // pidx(16 j + r) = 17 j + r
+#pragma unroll
+    for (int r = 0; r < 16; r++) o[r] = v[r];
   }
   __syncthreads();
 
   // ---- middle radix-16 passes: LDS -> registers -> LDS
-  int Ns = R0;
 #pragma unroll
-  for (int p = 0; p < Sh::NPASS16 - 1; p++) {
+  for (int p = 0; p < Sh::NMID; p++) {
+    const int Ns = 16 << (4 * p);
     float2 v[16];
     const int j = tid;
     if (tid < NTF) {
       const int k = j & (Ns - 1);
-      const int twstep = L / (Ns * 16);
       float2 w[16];
-      twiddles16(twlo, twhi, k * twstep, w);
+      twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
+      const float2* ip = buf + pidx(j);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        float2 x = buf[pidx(j + r * (L / 16))];
+        const float2 x = kStride16 ? ip[r * RSTR] : buf[pidx(j + r * NTF)];
         v[r] = r == 0 ? x : cmulf(x, w[r]);
       }
       DFT<16>::run(v);
@@ -564,40 +611,48 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     if (tid < NTF) {
       const int k = j & (Ns - 1);
       const int idxD = (j / Ns) * Ns * 16 + k;
+      float2* op = buf + pidx(idxD);           // Ns is a multiple of 16
 #pragma unroll
-      for (int r = 0; r < 16; r++) buf[pidx(idxD + r * Ns)] = v[r];
+      for (int r = 0; r < 16; r++) op[r * (Ns + Ns / 16)] = v[r];
     }
     __syncthreads();
-    Ns *= 16;
   }
 
-  // ---- last pass (Ns = L/16): LDS -> registers -> sideband half -> frac rotation -> HBM
+  // ---- last pass (radix RL, Ns = L/RL): only the bins of the kept sideband
+  //      (USB [0, C), real LSB [C, 2C)), then the fractional-sample rotation
   if (tid < NTF) {
-    float2 v[16];
-    const int j = tid;                     // j < Ns, so k = j and the bin is j + r*L/16
-    float2 w[16];
-    twiddles16(twlo, twhi, j, w);
+    const int j = tid;
+    const int mh = B.lsb ? RL / 2 : 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      float2 x = buf[pidx(j + r * (L / 16))];
-      v[r] = r == 0 ? x : cmulf(x, w[r]);
-    }
-    DFT<16>::run(v);
-    const int rbase = B.lsb ? 8 : 0;       // USB keeps bins [0,C), real LSB keeps [C,2C)
+    for (int u = 0; u < 16 / RL; u++) {
+      const int k = j + u * NTF;             // butterfly, k < L/RL
+      float2 v[RL], w[RL];
+      if (RL == 16) twiddles16(twlo, twhi, k, w);
+      else twiddlesR<RL>(twlo, twhi, k, w);
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int c = j + q * (L / 16);      // output channel
-      float2 x = v[rbase + q];
-      float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
-      float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
-      if (A.prec == FXG_XMAC_FP32) {
-        ((float2*)A.spectra)[outbase + c] = y;
-      } else if (A.prec == FXG_XMAC_FP16) {
-        __half2 h = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
-        ((__half2*)A.spectra)[outbase + c] = h;
+      for (int r = 0; r < RL; r++) {
+        const float2 x = buf[pidx(k + NSL * r)];
+        v[r] = r == 0 ? x : cmulf(x, w[r]);
+      }
+      if (RL == 2) {
+        v[0] = mh ? sub2(v[0], v[1]) : add2(v[0], v[1]);
       } else {
-        __hip_bfloat162 h = __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
-        ((__hip_bfloat162*)A.spectra)[outbase + c] = h;
+        DFT<RL>::run(v);
+      }
+#pragma unroll
+      for (int q = 0; q < (RL == 2 ? 1 : RL / 2); q++) {
+        const int c = k + NSL * q;             // output channel
+        const float2 x = RL == 2 ? v[0] : v[mh + q];
+        const float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
+        const float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
+        if (A.prec == FXG_XMAC_FP32) {
+          ((float2*)A.spectra)[outbase + c] = y;
+        } else if (A.prec == FXG_XMAC_FP16) {
+          ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
+        } else {
+          ((__hip_bfloat162*)A.spectra)[outbase + c] =
+              __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
+        }
       }
     }
   }
