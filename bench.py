@@ -292,7 +292,7 @@ def main():
         avg_s = stage["xmac"]["ms"] / stage["xmac"]["launches"] / 1e3
         achieved = flops / avg_s / 1e12
         peak = PEAK_TFLOPS[args.precision]
-        kname = "fxg::k_xmac_mfma<2, false>" if args.precision == "fp16" else "fxg::k_xmac_mfma<2, true>"
+        kname = "fxg::k_xmac_mfma<2, false, 16>" if args.precision == "fp16" else "fxg::k_xmac_mfma<2, true, 16>"
         roof = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": profiled_traffic(kname), "avg_launch_ms": avg_s * 1e3}
 

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -37,6 +38,21 @@ def _run(cmd):
     return r
 
 
+def _check_no_scratch(r, src):
+    """Kernels must not use scratch (private) memory: a spilled struct or a
+    dynamically indexed local array costs a memory round trip per access."""
+    fn, bad = None, []
+    for line in r.stderr.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            fn = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) > 0:
+            bad.append(f"{fn}: {m.group(1)} bytes/lane")
+    if bad:
+        raise RuntimeError(f"{src}: kernels using scratch memory:\n  " + "\n  ".join(bad))
+
+
 def build_gpu_lib(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     objs = []
@@ -47,10 +63,16 @@ def build_gpu_lib(verbose: bool = False) -> str:
             continue
         obj = os.path.join(OUT, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"]
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
+        remarks = ["-Rpass-analysis=kernel-resource-usage"] if src.endswith(".hip") else []
+        cmds.append(([HIPCC, *HIP_FLAGS, *remarks, *lang, "-c", path, "-o", obj], src))
         objs.append(obj)
+
+    def compile_one(job):
+        cmd, src = job
+        _check_no_scratch(_run(cmd), src)
+
     with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
-        list(ex.map(_run, cmds))
+        list(ex.map(compile_one, cmds))
     lib = os.path.join(OUT, "libfxcorr_gpu.so")
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

@@ -432,6 +432,45 @@ __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restric
   }
 }
 
+__device__ __forceinline__ void put_spec(float2* o, int c, float2 y) { o[c] = y; }
+__device__ __forceinline__ void put_spec(__half2* o, int c, float2 y) { o[c] = __floats2half2_rn(y.x, y.y); }
+__device__ __forceinline__ void put_spec(__hip_bfloat162* o, int c, float2 y) { o[c] = __float22bfloat162_rn(y); }
+
+// Last pass of radix 2 (L = 2 * 16^a) for thread j: butterflies k = j + u L/16,
+// u < 8, with W_L^k = W_L^j W_16^u; only the kept sideband's output of each
+// butterfly; channel c = k has c mod s = j mod s (one fsub value per thread,
+// the power-of-two spectrum scale folded into it exactly); then the
+// fractional-sample rotation fsub[c mod s] * fstep[c / s] (mode.cpp:1228-1245).
+template <int L, typename TO>
+__device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* twlo, const float2* twhi,
+                                             const float2* fsub, const float2* fstep, int s, int slog,
+                                             float scale, bool lsb, TO* out, int j) {
+  constexpr int NTF = L / 16, NSL = L / 2, RSTR = NTF + NTF / 16;
+  const float2 wj = tw_of(twlo, twhi, j);
+  float2 f0 = fsub[j & (s - 1)];
+  f0 = make_float2(f0.x * scale, f0.y * scale);
+  const float2* fsp = fstep + (j >> slog);
+  const int fss = NTF >> slog;
+  const float2* ip = buf + pidx(j);
+  constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const float2 x0 = ip[u * RSTR];
+    float2 t = cmulf(ip[u * RSTR + NSL + NSL / 16], wj);
+    if (u == 1) t = cmulf(t, make_float2(c1, -s1));          // * W_16^u
+    if (u == 2) t = cmulf(t, make_float2(h, -h));
+    if (u == 3) t = cmulf(t, make_float2(s1, -c1));
+    if (u == 4) t = mul_mi(t);
+    if (u == 5) t = cmulf(t, make_float2(-s1, -c1));
+    if (u == 6) t = cmulf(t, make_float2(-h, -h));
+    if (u == 7) t = cmulf(t, make_float2(-c1, -s1));
+    const float2 x = lsb ? sub2(x0, t) : add2(x0, t);
+    const float2 fr = cmulf(f0, *fsp);
+    fsp += fss;
+    put_spec(out, j + u * NTF, make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x));
+  }
+}
+
 template <int L, int NBITS>
 __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   using Sh = FftShape<L>;
@@ -441,7 +480,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   __shared__ float2 tabs[512];                  // rotator sub/step tables, packed
   __shared__ float2 twlo[64], twhi[L / 64];     // e^{-2 pi i m/L}, e^{-2 pi i 64h/L}
   __shared__ unsigned raw[RAWW];
-  __shared__ float lut8s[NBITS == 8 ? 256 : 1];
+  __shared__ float lut8s[NBITS == 8 ? 256 : 4];  // level table indexed by the sample code
 
   const int tid = threadIdx.x;
   const int cb = blockIdx.x;                      // block within chunk
@@ -529,6 +568,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       fstep[j] = make_float2(cs, sn);
     }
     if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
+    else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];   // (S.lut[tid] would put S in scratch)
   }
   __syncthreads();
 
@@ -560,8 +600,9 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
 #pragma unroll
       for (int r = 0; r < 16; r++) {
         const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
-        const float x = NBITS == 8 ? lut8s[code] : pick4(lut2, code);
-        const float2 rot = cmulf(r0, stp[r * sstep]);   // sub[k] * step[j], mode.cpp:1160-1170
+        const float x = lut8s[code];
+        const float2 rot = cmulf(r0, *stp);     // sub[k] * step[j], mode.cpp:1160-1170
+        stp += sstep;
         v[r] = make_float2(x * rot.x, x * rot.y);
       }
     } else {
@@ -572,7 +613,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
         if (fast) {
           const int bp = bitoff + n * NBITS;
           const unsigned code = (raw[bp >> 5] >> (bp & 31)) & ((1u << NBITS) - 1u);
-          x = NBITS == 8 ? lut8s[code] : pick4(lut2, code);
+          x = lut8s[code];
         } else {
           x = sample_value(S, Q, A.ftab, NBITS == 8 ? lut8s : A.lut8, lut2, t0 + n, G.validend, B.band);
         }
@@ -620,7 +661,15 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
 
   // ---- last pass (radix RL, Ns = L/RL): only the bins of the kept sideband
   //      (USB [0, C), real LSB [C, 2C)), then the fractional-sample rotation
-  if (tid < NTF) {
+  if (RL == 2 && tid < NTF && rotfast) {
+    if (A.prec == FXG_XMAC_FP32)
+      last_pass_r2<L, float2>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0, (float2*)A.spectra + outbase, tid);
+    else if (A.prec == FXG_XMAC_FP16)
+      last_pass_r2<L, __half2>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0, (__half2*)A.spectra + outbase, tid);
+    else
+      last_pass_r2<L, __hip_bfloat162>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0,
+                                       (__hip_bfloat162*)A.spectra + outbase, tid);
+  } else if (tid < NTF) {
     const int j = tid;
     const int mh = B.lsb ? RL / 2 : 0;
 #pragma unroll

@@ -10,8 +10,8 @@
 // The interleaved (re, im) half2 spectrum is the A operand as stored, and Ah
 // is a half swap with one sign flip.  Re V is symmetric and Im V
 // antisymmetric, so only the tiles (I <= J) of the 32x32 tile grid are
-// accumulated.  One wavefront owns one channel; a workgroup of 16 wavefronts
-// owns 16 consecutive channels and stages 32 blocks of every input through
+// accumulated.  One wavefront owns one channel; a workgroup of CG wavefronts
+// owns CG consecutive channels and stages 32 blocks of every input through
 // LDS, transposed to [channel][input][block] so that each lane's operand is
 // one 16-byte LDS read.
 #include <hip/hip_runtime.h>
@@ -35,19 +35,30 @@ __device__ __forceinline__ f16_t mfma(uint4 a, uint4 b, f16_t c) {
 // (re, im) -> (im, -re) on each packed 16-bit pair
 __device__ __forceinline__ unsigned hat(unsigned v) { return (v >> 16) | ((v ^ 0x8000u) << 16); }
 
-template <int T, bool BF>
-__global__ __launch_bounds__(1024) void k_xmac_mfma(XmacArgs A) {
-  constexpr int CG = 16;            // channels (wavefronts) per workgroup
+template <int T, bool BF, int CG>
+__global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
+  // CG channels (wavefronts) per workgroup
+  constexpr int NTH = 64 * CG;
   constexpr int TT = 32;            // blocks staged per pass
   constexpr int TP = TT + 4;        // padded row length in dwords
   constexpr int NR = 32 * T;        // inputs, padded to the tile size
   constexpr int NTILE = T * (T + 1) / 2;
-  __shared__ unsigned lds[CG * NR * TP];
+  static_assert(CG % 4 == 0 && (NR * TT * (CG / 4)) % NTH == 0, "staging items must tile the workgroup");
+  // per-channel stride: a multiple of 4 dwords that is 4 mod 16, so the four
+  // channel rows one staging lane writes fall in different banks
+  constexpr int CHS = NR * TP + 4;
+  __shared__ unsigned lds[CG * CHS];
 
   const unsigned* spec = (const unsigned*)A.spec;
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63;
-  const int c0 = blockIdx.x * CG;
+  // XCD-aware channel groups: workgroups are dealt to the 8 XCDs round-robin,
+  // so group g = (bid mod 8) * (n/8) + bid/8 puts the two groups that share
+  // each 128-byte line of a spectrum row on the same XCD (same L2)
+  const int nwg = gridDim.x;
+  const int bid = blockIdx.x;
+  const int grp = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;
+  const int c0 = grp * CG;
   const int half = l >> 5, row = l & 31;
 
   f16_t accR[NTILE], accI[NTILE];
@@ -59,10 +70,11 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma(XmacArgs A) {
   for (int t0 = 0; t0 < A.nblocks; t0 += TT) {
     // ---- stage: NR inputs x TT blocks x CG channels, 16 bytes (4 channels) per item
     constexpr int NITEM = NR * TT * (CG / 4);
-    for (int it = tid; it < NITEM; it += 1024) {
-      const int q = it & 3;
-      const int t = (it >> 2) % TT;
-      const int a = it / (4 * TT);
+    for (int it = tid; it < NITEM; it += NTH) {
+      constexpr int QN = CG / 4;      // 4-channel quads per staged row
+      const int q = it % QN;
+      const int t = (it / QN) % TT;
+      const int a = it / (QN * TT);
       uint4 v = make_uint4(0u, 0u, 0u, 0u);
       if (a < A.nin && t0 + t < A.nblocks) {
         const long long e = A.in_off[a] + (long long)(t0 + t) * A.block_stride + c0 + 4 * q;
@@ -72,12 +84,12 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma(XmacArgs A) {
           v.x = spec[e]; v.y = spec[e + 1]; v.z = spec[e + 2]; v.w = spec[e + 3];
         }
       }
-      unsigned* d = lds + ((4 * q) * NR + a) * TP + t;
-      d[0] = v.x; d[NR * TP] = v.y; d[2 * NR * TP] = v.z; d[3 * NR * TP] = v.w;
+      unsigned* d = lds + (4 * q) * CHS + a * TP + t;
+      d[0] = v.x; d[CHS] = v.y; d[2 * CHS] = v.z; d[3 * CHS] = v.w;
     }
     __syncthreads();
     // ---- K steps of 8 blocks (16 real k values) each
-    const unsigned* base = lds + (w * NR) * TP;
+    const unsigned* base = lds + w * CHS;
 #pragma unroll
     for (int ks = 0; ks < TT / 8; ks++) {
       uint4 f[T], fh[T];
@@ -123,14 +135,15 @@ bool xmac_mfma_supported(int nin, int nchan) { return nin <= 64 && nchan % 16 ==
 
 hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream) {
   if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
-  dim3 grid((unsigned)(a.nchan / 16));
+  constexpr int CG = 16;      // 64-byte row segments; one workgroup (148 KB of LDS) per CU
+  dim3 grid((unsigned)(a.nchan / CG));
   const bool bf = prec == FXG_XMAC_BF16;
   if (a.nin <= 32) {
-    if (bf) hipLaunchKernelGGL((k_xmac_mfma<1, true>), grid, dim3(1024), 0, stream, a);
-    else hipLaunchKernelGGL((k_xmac_mfma<1, false>), grid, dim3(1024), 0, stream, a);
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma<1, true, CG>), grid, dim3(64 * CG), 0, stream, a);
+    else hipLaunchKernelGGL((k_xmac_mfma<1, false, CG>), grid, dim3(64 * CG), 0, stream, a);
   } else {
-    if (bf) hipLaunchKernelGGL((k_xmac_mfma<2, true>), grid, dim3(1024), 0, stream, a);
-    else hipLaunchKernelGGL((k_xmac_mfma<2, false>), grid, dim3(1024), 0, stream, a);
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma<2, true, CG>), grid, dim3(64 * CG), 0, stream, a);
+    else hipLaunchKernelGGL((k_xmac_mfma<2, false, CG>), grid, dim3(64 * CG), 0, stream, a);
   }
   return hipGetLastError();
 }
