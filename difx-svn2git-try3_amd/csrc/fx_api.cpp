@@ -322,7 +322,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     if (!P.uniform) why = "all datastreams of a job must share nchan";
     else if (!(P.L >= 128 && P.L <= 8192 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 8192]";
     for (const DevStation& st : P.st)
-      if (why.empty() && 2 * st.s + st.L / st.s + st.C / st.s > 512)
+      if (why.empty() && (st.order == 2 ? 3 : 2) * st.s + st.L / st.s + st.C / st.s > 512)
         why = "array stride gives more than 512 rotator sub/step points";
     if (!why.empty()) { delete j; return set_err(FXG_EUNSUPPORTED, why); }
   }
@@ -672,7 +672,7 @@ int fxg_tap_block_info(fxg_job* j, int ds, int block, fxg_block_info* out) {
   std::memset(out, 0, sizeof(*out));
   out->status = g.status; out->nearestsample = g.nearestsample; out->integerdelay = g.integerdelay;
   out->weight = g.status == 0 ? g.weight : 0.0f; out->fracsampleerror = g.fse; out->intwalltime = g.intwall;
-  out->a = g.a; out->b = g.b; out->fracwalltime = g.fracwall;
+  out->a = g.a; out->b = g.b; out->fracwalltime = g.fracwall; out->c = g.c;
   return 0;
 }
 

@@ -23,6 +23,7 @@ struct DevStation {
   int band0;            // global index of recorded band 0 of this datastream
   int slog;             // log2(s) when s is a power of two, else -1
   int bpf;              // bytes per band per frame in the corner-turned streams
+  int order;            // fringe rotation order 0 (post-F), 1 (linear), 2 (quadratic)
   double sampletime, bw;
   float lut[4];         // 2-bit level table (VDIF or Mark5B order)
 };
@@ -58,6 +59,8 @@ struct DevBlock {
   float weight, fse;
   long long validend;   // samples at or after this index decode as zero
   double a, b, fracwall;
+  double c;             // order 2: constant term minus integerdelay; order 0: delay at the block centre
+  double wall;          // walltimesecs of the block start
 };
 
 // ---- subint-wide scalars ---------------------------------------------------

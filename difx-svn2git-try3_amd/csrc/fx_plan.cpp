@@ -71,7 +71,7 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
   REQUIRE(d->ndatastreams > 0, FXG_EINVAL, "ndatastreams must be positive");
   REQUIRE(d->nbaselines >= 0, FXG_EINVAL, "nbaselines negative");
   REQUIRE(d->blockspersend > 0, FXG_EINVAL, "blockspersend must be positive");
-  REQUIRE(d->fringe_order == 1, FXG_EUNSUPPORTED, "only fringe rotation order 1 is built");
+  REQUIRE(d->fringe_order >= 0 && d->fringe_order <= 2, FXG_EUNSUPPORTED, "fringe rotation order must be 0, 1 or 2");
   REQUIRE(d->xmac_precision >= FXG_XMAC_FP32 && d->xmac_precision <= FXG_XMAC_BF16, FXG_EINVAL,
           "xmac_precision");
   for (int f = 0; f < d->nfreqs; f++) {
@@ -252,6 +252,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     if (power_of_two(st.s)) { st.slog = 0; while ((1 << st.slog) < st.s) st.slog++; }
     REQUIRE(st.databytes % ds.nrecbands == 0, FXG_EUNSUPPORTED, "payload must split evenly over the bands");
     st.bpf = st.databytes / ds.nrecbands;
+    st.order = d->fringe_order;
     st.bpbnum = bitsper / 8;
     st.bpbden = 1;
     if (st.bpbnum == 0) { st.bpbnum = 1; st.bpbden = 8 / bitsper; }

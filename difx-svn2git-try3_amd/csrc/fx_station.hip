@@ -118,6 +118,7 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   DevBlock g;
   g.status = 0; g.nearestsample = 0; g.integerdelay = 0; g.intwall = 0; g.allvalid = 0; g.pad = 0;
   g.weight = 0.0f; g.fse = 0.0f; g.validend = 0x7fffffffffffffffLL; g.a = 0; g.b = 0; g.fracwall = 0;
+  g.c = 0; g.wall = 0;
   bool ok = Q.usable != 0;
   if (ok) {
     int w = index / FXG_FLAGS_PER_INT;
@@ -135,6 +136,7 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   double walltimesecs = (sub.scanstart + sub.offsec) + sub.offns / 1.0e9 + fftstartmicrosec / 1.0e6;
   g.intwall = (int)walltimesecs;
   g.fracwall = walltimesecs - g.intwall;
+  g.wall = walltimesecs;
   g.nearestsample = nearestsample;
   if (nearestsample < -1 ||
       ((((long long)nearestsample + fftchannels) / 1) * S.bpbnum) / S.bpbden > Q.nbytes) {
@@ -173,14 +175,26 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   if (!(g.weight > 0.0f)) { g.status = 3; out[(long long)s * sub.nblocks + index] = g; return; }
   double nearestsampletime = nearestsample * S.sampletime;
   g.fse = (float)(starttime - nearestsampletime);
-  double d0 = ip[0] * index * index + ip[1] * index + ip[2];
-  double d1 = ip[0] * (index + 0.5) * (index + 0.5) + ip[1] * (index + 0.5) + ip[2];
-  double d2 = ip[0] * (index + 1) * (index + 1) + ip[1] * (index + 1) + ip[2];
-  double a = d2 - d0;
-  double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
-  int integerdelay = (int)b;
-  b -= integerdelay;
-  g.a = a; g.b = b; g.integerdelay = integerdelay;
+  if (S.order == 0) {                        // post-F (mode.cpp:791-793)
+    g.integerdelay = (int)averagedelay;
+    g.c = averagedelay;
+  } else if (S.order == 1) {                 // linear (mode.cpp:794-813)
+    double d0 = ip[0] * index * index + ip[1] * index + ip[2];
+    double d1 = ip[0] * (index + 0.5) * (index + 0.5) + ip[1] * (index + 0.5) + ip[2];
+    double d2 = ip[0] * (index + 1) * (index + 1) + ip[1] * (index + 1) + ip[2];
+    double a = d2 - d0;
+    double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+    int integerdelay = (int)b;
+    b -= integerdelay;
+    g.a = a; g.b = b; g.integerdelay = integerdelay;
+  } else {                                   // quadratic (mode.cpp:814-838)
+    double a = ip[0];
+    double b = ip[1] + index * ip[0] * 2.0;
+    double c = ip[2] + index * ip[1] + (index * index) * ip[0];
+    int integerdelay = (int)c;
+    c -= integerdelay;
+    g.a = a; g.b = b; g.c = c; g.integerdelay = integerdelay;
+  }
   out[(long long)s * sub.nblocks + index] = g;
 }
 
@@ -519,6 +533,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   float2* rstep = tabs + S.s;
   float2* fsub = rstep + S.nfrsteps;
   float2* fstep = fsub + S.s;
+  float2* rquad = fstep + S.nfracsteps;         // order 2 only
   for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
   for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
 
@@ -535,8 +550,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       if (fraclooffset > kTiny) isfraclooffset = true;
       if (looffset < 0) fraclooffset = -fraclooffset;
     }
-    for (int k = tid; k < s; k += NT) {
-      double ph = (((double)k / (double)L) * G.a + G.b) * lofreq;
+    // pre-F rotator points: order 1 x = a k/L + b (mode.cpp:804-812), order 2
+    // x = (k/L)(b + a s/L) + c with quad points (k/L) 2a s/L and steps
+    // (js/L) b + (js/L)^2 a (mode.cpp:821-837)
+    const double stepxoff1 = (double)(1 * s) / (double)L;
+    for (int k = tid; k < s && S.order != 0; k += NT) {
+      const double subxoff = (double)k / (double)L;
+      double ph = (S.order == 1 ? subxoff * G.a + G.b : subxoff * (G.b + G.a * stepxoff1) + G.c) * lofreq;
       if (S.fractionalLo) ph += (lofreq - (int)lofreq) * (double)G.integerdelay;
       if (looff) {
         ph += ((k * S.sampletime) / 1e6) * (-looffset);
@@ -546,9 +566,16 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       float arg = (float)(-kTwoPi * (ph - (int)ph));
       float sn, cs; sincosf(arg, &sn, &cs);
       rsub[k] = make_float2(cs, sn);
+      if (S.order == 2) {
+        const double qph = (subxoff * (2 * G.a * stepxoff1)) * lofreq;
+        arg = (float)(-kTwoPi * (qph - (int)qph));
+        sincosf(arg, &sn, &cs);
+        rquad[k] = make_float2(cs, sn);
+      }
     }
-    for (int j = tid; j < S.nfrsteps; j += NT) {
-      double ph = (((double)(j * s)) / (double)L) * G.a * lofreq;
+    for (int j = tid; j < S.nfrsteps && S.order != 0; j += NT) {
+      const double stepxoff = ((double)(j * s)) / (double)L;
+      double ph = (S.order == 1 ? stepxoff * G.a : stepxoff * G.b + (stepxoff * stepxoff) * G.a) * lofreq;
       if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
       float arg = (float)(-kTwoPi * (ph - (int)ph));
       float sn, cs; sincosf(arg, &sn, &cs);
@@ -556,9 +583,20 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     }
     const double sign = B.useB ? -1.0 : 1.0;
     const float val = (float)((double)G.fse - B.clock + sign * B.clockdelta / 2);
+    // order 0: the fringe phase of the block at its centre delay, added to
+    // the sub arguments (mode.cpp:1064-1073, 1111-1119)
+    float postf = 0.0f;
+    if (S.order == 0) {
+      double phaserotation = (G.c - G.integerdelay) * lofreq;
+      if (S.fractionalLo) phaserotation += G.integerdelay * (lofreq - (int)lofreq);
+      phaserotation -= G.wall * looffset;
+      postf = (float)(-kTwoPi * (phaserotation - (int)phaserotation));
+    }
     for (int k = tid; k < s; k += NT) {
       float cf = (float)((kTwoPi * k * S.bw) / C);
-      float sn, cs; sincosf(cf * val, &sn, &cs);
+      float arg = cf * val;
+      if (S.order == 0) arg = arg + postf;
+      float sn, cs; sincosf(arg, &sn, &cs);
       fsub[k] = make_float2(cs, sn);
     }
     for (int j = tid; j < S.nfracsteps; j += NT) {
@@ -590,7 +628,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   if (tid < NTF) {
     const int j = tid;
     float2 v[16];
-    if (kWordStep && fast && rotfast) {
+    if (kWordStep && fast && rotfast && S.order == 1) {
       const int bp = bitoff + NBITS * j;
       const unsigned* wp = raw + (bp >> 5);
       const int sh = bp & 31;
@@ -601,9 +639,41 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       for (int r = 0; r < 16; r++) {
         const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
         const float x = lut8s[code];
-        const float2 rot = cmulf(r0, *stp);     // sub[k] * step[j], mode.cpp:1160-1170
+        const float2 rot = cmulf(r0, *stp);     // sub[k] * step[j], mode.cpp:975-977
         stp += sstep;
         v[r] = make_float2(x * rot.x, x * rot.y);
+      }
+    } else if (kWordStep && fast && S.order == 0) {
+      // post-F fringe rotation: the samples are transformed unrotated
+      const int bp = bitoff + NBITS * j;
+      const unsigned* wp = raw + (bp >> 5);
+      const int sh = bp & 31;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
+        v[r] = make_float2(lut8s[code], 0.0f);
+      }
+    } else if (kWordStep && fast && rotfast && S.order == 2) {
+      // piecewise quadratic rotator: rot[j s + k] = piece_j[k] step[j] with
+      // piece_0 = sub[k], piece_{j+1} = piece_j quad[k] (mode.cpp:1043-1049);
+      // this thread's points all have the same k and step index q0 + r sstep
+      const int bp = bitoff + NBITS * j;
+      const unsigned* wp = raw + (bp >> 5);
+      const int sh = bp & 31;
+      const int k = j & (s - 1), q0 = j >> slog, sstep = NTF >> slog;
+      const float2 qd = rquad[k];
+      float2 piece = rsub[k];
+      for (int i = 0; i < q0; i++) piece = cmulf(piece, qd);
+      const float2* stp = rstep + q0;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
+        const float x = lut8s[code];
+        const float2 rot = cmulf(piece, *stp);
+        stp += sstep;
+        v[r] = make_float2(x * rot.x, x * rot.y);
+        if (r < 15)
+          for (int i = 0; i < sstep; i++) piece = cmulf(piece, qd);
       }
     } else {
 #pragma unroll
@@ -618,9 +688,20 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
           x = sample_value(S, Q, A.ftab, NBITS == 8 ? lut8s : A.lut8, lut2, t0 + n, G.validend, B.band);
         }
         float2 rot;
-        if (slog >= 0) rot = n < s ? rsub[n] : cmulf(rsub[n & (s - 1)], rstep[n >> slog]);
-        else rot = n < s ? rsub[n] : cmulf(rsub[n % s], rstep[n / s]);
-        v[r] = make_float2(x * rot.x, x * rot.y);
+        if (S.order == 0) {
+          rot = make_float2(1.0f, 0.0f);
+        } else if (S.order == 1) {
+          if (slog >= 0) rot = n < s ? rsub[n] : cmulf(rsub[n & (s - 1)], rstep[n >> slog]);
+          else rot = n < s ? rsub[n] : cmulf(rsub[n % s], rstep[n / s]);
+        } else {
+          const int kk = slog >= 0 ? (n & (s - 1)) : n % s;
+          const int jj = slog >= 0 ? (n >> slog) : n / s;
+          const float2 qd = rquad[kk];
+          float2 piece = rsub[kk];
+          for (int i = 0; i < jj; i++) piece = cmulf(piece, qd);
+          rot = cmulf(piece, rstep[jj]);
+        }
+        v[r] = S.order == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
       }
     }
     DFT<16>::run(v);

@@ -101,7 +101,7 @@ class SubintDesc(C.Structure):
 class BlockInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("nearestsample", C.c_int32), ("integerdelay", C.c_int32),
                 ("weight", C.c_float), ("fracsampleerror", C.c_float), ("intwalltime", C.c_int32),
-                ("a", C.c_double), ("b", C.c_double), ("fracwalltime", C.c_double)]
+                ("a", C.c_double), ("b", C.c_double), ("fracwalltime", C.c_double), ("c", C.c_double)]
 
 
 class PlanInfo(C.Structure):
@@ -145,6 +145,7 @@ class JobSpec:
     xmacstridelen: int = 0
     max_chunk_blocks: int = 0
     pols: str = "RL"
+    fringe_order: int = 1
 
     @property
     def nrecbands(self) -> int:
@@ -240,7 +241,7 @@ class JobSpec:
         job.ndatastreams = self.nstations
         job.nbaselines = len(pairs)
         job.blockspersend = self.blockspersend
-        job.fringe_order = 1
+        job.fringe_order = self.fringe_order
         job.arraystridelen = self.arraystridelen
         job.xmacstridelen = self.xmacstridelen
         job.crosspol_autocorrs = 1 if (self.crosspol and npol == 2) else 0

@@ -170,9 +170,16 @@ typedef struct {
   float   weight;          /* dataweight                                        */
   float   fracsampleerror; /* µs                                                */
   int32_t intwalltime;
-  double  a;               /* order-1 delay slope over the block (µs)           */
-  double  b;               /* delay at block start minus integerdelay (µs)      */
+  /* delay model of the block by fringe rotation order (mode.cpp:789-838):
+   *   order 1: a = slope over the block, b = delay at block start minus
+   *            integerdelay, c = 0
+   *   order 2: a, b, c = quadratic coefficients in block units, c minus
+   *            integerdelay
+   *   order 0: a = b = 0, c = delay at the block centre (averagedelay)       */
+  double  a;
+  double  b;
   double  fracwalltime;
+  double  c;
 } fxg_block_info;
 
 /* ---- result layout (Configuration::populateResultLengths) --------------- */

@@ -17,7 +17,7 @@
  *            mark5_stream.c:153-202 (next_frame); blanker_mark5.c:53-165,282-312;
  *            mpifxcorr/src/mk5mode.cpp:104-177 (Mk5Mode::unpack)
  *   station  mpifxcorr/src/mode.cpp:43-462 (constructor tables), :649-1397
- *            (Mode::process, fringe order 1, real sampling)
+ *            (Mode::process, fringe orders 0, 1, 2, real sampling)
  *   model    mpifxcorr/src/model.cpp:229-304 (calculateDelayInterpolator)
  *   xmac     mpifxcorr/src/core.cpp:667-1122 (processdata), :1159-1369
  *            (averageAndSendAutocorrs), :1586-1867 (uvshiftAndAverage, single
@@ -303,6 +303,7 @@ struct DsGeom {
   StreamFmt fmt;
   int C, L, s, numfrstrides, nfracsteps;
   double bw, sampletime;          /* MHz, µs */
+  int order;                      /* fringe rotation order 0, 1, 2 */
   bool fractionalLoFreq, deltapoloffsets;
   int bytesperblocknum, bytesperblockden, samplesperblock;
 };
@@ -330,7 +331,7 @@ static int validate(const fxg_job_desc* j) {
   if (j->nfreqs <= 0 || j->nfreqs > FXG_MAX_FREQS) return fail(FXG_EINVAL, "nfreqs");
   if (j->ndatastreams <= 0) return fail(FXG_EINVAL, "ndatastreams");
   if (j->blockspersend <= 0) return fail(FXG_EINVAL, "blockspersend");
-  if (j->fringe_order != 1) return fail(FXG_EUNSUPPORTED, "only fringe rotation order 1");
+  if (j->fringe_order < 0 || j->fringe_order > 2) return fail(FXG_EUNSUPPORTED, "fringe rotation order must be 0, 1 or 2");
   for (int f = 0; f < j->nfreqs; f++) {
     if (j->freqs[f].chans_to_avg != 1) return fail(FXG_EUNSUPPORTED, "chans_to_avg must be 1");
     if (j->freqs[f].nchan <= 0) return fail(FXG_EINVAL, "nchan");
@@ -487,6 +488,7 @@ static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g)
   if (g->C % g->s) return fail(FXG_EINVAL, "array stride must divide nchan");
   g->numfrstrides = g->L / g->s; g->nfracsteps = g->numfrstrides / 2;
   g->sampletime = 1.0 / (2.0 * g->bw);
+  g->order = j->fringe_order;
   g->fractionalLoFreq = false; g->deltapoloffsets = false;
   for (int i = 0; i < ds.nrecfreqs; i++) {
     double loval = j->freqs[ds.recfreq_index[i]].freq_mhz;
@@ -508,11 +510,11 @@ static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g)
 }
 
 /* ------------------------------------------------------------------------ */
-/* Mode::process restated (fringe order 1, real sampling)                   */
+/* Mode::process restated (fringe orders 0, 1, 2, real sampling)           */
 /* ------------------------------------------------------------------------ */
 struct BlockGeom {
   int status; int nearestsample; int integerdelay; float weight; float fracsampleerror;
-  int intwalltime; double a, b, fracwalltime; double starttime, averagedelay;
+  int intwalltime; double a, b, c, fracwalltime, walltimesecs; double starttime, averagedelay;
 };
 
 static bool validbit(const fxg_station_input& in, int index) {
@@ -529,7 +531,8 @@ static bool station_usable(const fxg_subint_desc& sub, const fxg_station_input& 
 }
 
 /* mode.cpp:686-772 (geometry, range check, unpack, zero-weight check) and
- * :795-812 (order-1 interpolator).  The unpacked samples go to `work`. */
+ * :789-838 (integer delay and interpolator for fringe orders 0, 1, 2).  The
+ * unpacked samples go to `work`. */
 static BlockGeom block_geom(const DsGeom& g, const fxg_subint_desc& sub,
                             const fxg_station_input& in, int index,
                             std::vector<float>* work, Unpacked* up) {
@@ -546,7 +549,7 @@ static BlockGeom block_geom(const DsGeom& g, const fxg_subint_desc& sub,
   int nearestsample = int(starttime / g.sampletime + 0.5);
   double walltimesecs = sub.scan_start_sec + sub.sec + sub.ns / 1.0e9 + fftstartmicrosec / 1.0e6;
   int intwalltime = static_cast<int>(walltimesecs);
-  bg.averagedelay = averagedelay; bg.starttime = starttime;
+  bg.averagedelay = averagedelay; bg.starttime = starttime; bg.walltimesecs = walltimesecs;
   bg.intwalltime = intwalltime; bg.fracwalltime = walltimesecs - intwalltime;
   bg.nearestsample = nearestsample;
   if (nearestsample < -1 ||
@@ -560,14 +563,26 @@ static BlockGeom block_geom(const DsGeom& g, const fxg_subint_desc& sub,
   if (!(bg.weight > 0.0f)) { bg.status = 3; return bg; }
   double nearestsampletime = nearestsample * g.sampletime;
   bg.fracsampleerror = float(starttime - nearestsampletime);
-  double d0 = interp[0] * index * index + interp[1] * index + interp[2];
-  double d1 = interp[0] * (index + 0.5) * (index + 0.5) + interp[1] * (index + 0.5) + interp[2];
-  double d2 = interp[0] * (index + 1) * (index + 1) + interp[1] * (index + 1) + interp[2];
-  double a = d2 - d0;
-  double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
-  int integerdelay = static_cast<int>(b);
-  b -= integerdelay;
-  bg.a = a; bg.b = b; bg.integerdelay = integerdelay;
+  if (g.order == 0) {                                   /* post-F (mode.cpp:791-793) */
+    bg.integerdelay = static_cast<int>(averagedelay);
+    bg.a = 0.0; bg.b = 0.0; bg.c = averagedelay;
+  } else if (g.order == 1) {                            /* linear (mode.cpp:794-813) */
+    double d0 = interp[0] * index * index + interp[1] * index + interp[2];
+    double d1 = interp[0] * (index + 0.5) * (index + 0.5) + interp[1] * (index + 0.5) + interp[2];
+    double d2 = interp[0] * (index + 1) * (index + 1) + interp[1] * (index + 1) + interp[2];
+    double a = d2 - d0;
+    double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+    int integerdelay = static_cast<int>(b);
+    b -= integerdelay;
+    bg.a = a; bg.b = b; bg.c = 0.0; bg.integerdelay = integerdelay;
+  } else {                                              /* quadratic (mode.cpp:814-838) */
+    double a = interp[0];
+    double b = interp[1] + index * interp[0] * 2.0;
+    double c = interp[2] + index * interp[1] + (index * index) * interp[0];
+    int integerdelay = int(c);
+    c -= integerdelay;
+    bg.a = a; bg.b = b; bg.c = c; bg.integerdelay = integerdelay;
+  }
   bg.status = 0;
   return bg;
 }
@@ -592,9 +607,22 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
   if (bg.status != 0) return bg;
   const bool fp64 = sizeof(T) == 8;
   w.rot.resize(L); w.fracA.resize(C); w.fracB.resize(C); w.fftbuf.resize(L);
-  std::vector<double> subxval(s), stepxval(g.numfrstrides);
-  for (int k = 0; k < s; k++) subxval[k] = (double(k) / double(L)) * bg.a + bg.b;          /* mode.cpp:804-812 */
-  for (int jj = 0; jj < g.numfrstrides; jj++) stepxval[jj] = (double(jj * s) / double(L)) * bg.a;
+  /* x values of the sub and step rotator points (mode.cpp:804-812, 821-837) */
+  std::vector<double> subxval(s), subquadxval(s), stepxval(g.numfrstrides);
+  const double stepxoff1 = double(1 * s) / double(L);
+  for (int k = 0; k < s; k++) {
+    const double subxoff = double(k) / double(L);
+    if (g.order == 1) subxval[k] = subxoff * bg.a + bg.b;
+    else if (g.order == 2) {
+      subxval[k] = subxoff * (bg.b + bg.a * stepxoff1) + bg.c;
+      subquadxval[k] = subxoff * (2 * bg.a * stepxoff1);
+    }
+  }
+  for (int jj = 0; jj < g.numfrstrides; jj++) {
+    const double stepxoff = double(jj * s) / double(L);
+    if (g.order == 1) stepxval[jj] = stepxoff * bg.a;
+    else if (g.order == 2) stepxval[jj] = stepxoff * bg.b + (stepxoff * stepxoff) * bg.a;
+  }
 
   for (int i = 0; i < ds.nrecfreqs; i++) {
     const fxg_freq& fq = j->freqs[ds.recfreq_index[i]];
@@ -607,28 +635,47 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
       if (fraclooffset > TINY) isfraclooffset = true;
       if (looffset < 0) fraclooffset = -fraclooffset;
     }
-    /* fringe rotator, order 1 (mode.cpp:909-978) */
-    std::vector<cpx<T>> sub_c(s), step_c(g.numfrstrides);
-    for (int k = 0; k < s; k++) {
-      double ph = subxval[k] * lofreq;
-      if (g.fractionalLoFreq) ph += (lofreq - int(lofreq)) * double(bg.integerdelay);
-      if (looff) {
-        ph += (k * g.sampletime / 1e6) * (-looffset);
-        ph += -looffset * bg.fracwalltime;
-        if (isfraclooffset) ph += -fraclooffset * bg.intwalltime;
+    /* pre-F fringe rotator: order 1 (mode.cpp:909-978), order 2 (:980-1050);
+     * order 0 rotates after the FFT (phase folded into the frac rotator) */
+    if (g.order == 0) {
+      for (int n = 0; n < L; n++) { w.rot[n].re = 1; w.rot[n].im = 0; }
+    } else {
+      std::vector<cpx<T>> sub_c(s), quad_c(s), step_c(g.numfrstrides);
+      auto rot_of = [&](double ph, cpx<T>* out) {
+        T arg = fp64 ? (T)(-TWO_PI * (ph - int(ph))) : (T)(float)(-TWO_PI * (ph - int(ph)));
+        T sn, cs; sincos_t(arg, &sn, &cs); out->re = cs; out->im = sn;
+      };
+      for (int k = 0; k < s; k++) {
+        double ph = subxval[k] * lofreq;
+        if (g.fractionalLoFreq) ph += (lofreq - int(lofreq)) * double(bg.integerdelay);
+        if (looff) {
+          ph += (k * g.sampletime / 1e6) * (-looffset);
+          ph += -looffset * bg.fracwalltime;
+          if (isfraclooffset) ph += -fraclooffset * bg.intwalltime;
+        }
+        rot_of(ph, &sub_c[k]);
+        if (g.order == 2) rot_of(subquadxval[k] * lofreq, &quad_c[k]);
       }
-      T arg = fp64 ? (T)(-TWO_PI * (ph - int(ph))) : (T)(float)(-TWO_PI * (ph - int(ph)));
-      T sn, cs; sincos_t(arg, &sn, &cs); sub_c[k].re = cs; sub_c[k].im = sn;
+      for (int jj = 0; jj < g.numfrstrides; jj++) {
+        double ph = stepxval[jj] * lofreq;
+        if (looff) ph += (jj * s * g.sampletime / 1e6) * (-looffset);
+        rot_of(ph, &step_c[jj]);
+      }
+      if (g.order == 1) {
+        for (int k = 0; k < s; k++) w.rot[k] = sub_c[k];
+        for (int jj = 1; jj < g.numfrstrides; jj++)
+          for (int k = 0; k < s; k++) w.rot[(size_t)jj * s + k] = cmul(sub_c[k], step_c[jj]);
+      } else {
+        /* piecewise rotator: rot[j s + k] = piece_j[k] step[j], piece_0 = sub,
+         * piece_{j+1} = piece_j * quad, in that operation order (mode.cpp:1043-1049) */
+        std::vector<cpx<T>> piece(sub_c);
+        for (int jj = 0; jj < g.numfrstrides; jj++)
+          for (int k = 0; k < s; k++) {
+            w.rot[(size_t)jj * s + k] = cmul(piece[k], step_c[jj]);
+            piece[k] = cmul(piece[k], quad_c[k]);
+          }
+      }
     }
-    for (int jj = 0; jj < g.numfrstrides; jj++) {
-      double ph = stepxval[jj] * lofreq;
-      if (looff) ph += (jj * s * g.sampletime / 1e6) * (-looffset);
-      T arg = fp64 ? (T)(-TWO_PI * (ph - int(ph))) : (T)(float)(-TWO_PI * (ph - int(ph)));
-      T sn, cs; sincos_t(arg, &sn, &cs); step_c[jj].re = cs; step_c[jj].im = sn;
-    }
-    for (int k = 0; k < s; k++) w.rot[k] = sub_c[k];
-    for (int jj = 1; jj < g.numfrstrides; jj++)
-      for (int k = 0; k < s; k++) w.rot[(size_t)jj * s + k] = cmul(sub_c[k], step_c[jj]);
 
     /* fractional-sample rotator (mode.cpp:1054-1097, 1100-1146) */
     for (int pass = 0; pass < (g.deltapoloffsets ? 2 : 1); pass++) {
@@ -638,9 +685,21 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
                     + (pass == 0 ? 1.0 : -1.0) * ds.recfreq_clockoffsetdelta_us[i] / 2;
       T val = fp64 ? (T)dval : (T)(float)dval;
       std::vector<cpx<T>> stepf(g.nfracsteps);
+      /* order 0: the fringe phase of the block, added to the sub arguments
+       * (mode.cpp:1064-1073, 1111-1119) */
+      T postf = 0;
+      if (g.order == 0) {
+        double phaserotation = (bg.averagedelay - bg.integerdelay) * lofreq;
+        if (g.fractionalLoFreq) phaserotation += bg.integerdelay * (lofreq - int(lofreq));
+        phaserotation -= bg.walltimesecs * looffset;
+        postf = fp64 ? (T)(-TWO_PI * (phaserotation - int(phaserotation)))
+                     : (T)(float)(-TWO_PI * (phaserotation - int(phaserotation)));
+      }
       for (int k = 0; k < s; k++) {
         T cf = fp64 ? (T)((TWO_PI * (k) * g.bw) / C) : (T)(float)((TWO_PI * (k) * g.bw) / C);
-        T arg = cf * val; T sn, cs; sincos_t(arg, &sn, &cs); fr[k].re = cs; fr[k].im = sn;
+        T arg = cf * val;
+        if (g.order == 0) arg = arg + postf;
+        T sn, cs; sincos_t(arg, &sn, &cs); fr[k].re = cs; fr[k].im = sn;
       }
       for (int jj = 0; jj < g.nfracsteps; jj++) {
         double dcf = fq.lsb ? (-TWO_PI * ((g.nfracsteps - jj) * s) * g.bw) / C
@@ -869,6 +928,7 @@ int orc_block_info(const fxg_job_desc* j, const fxg_subint_desc* sub, const fxg_
   out->status = bg.status; out->nearestsample = bg.nearestsample; out->integerdelay = bg.integerdelay;
   out->weight = bg.status == 0 ? bg.weight : 0.0f; out->fracsampleerror = bg.fracsampleerror;
   out->intwalltime = bg.intwalltime; out->a = bg.a; out->b = bg.b; out->fracwalltime = bg.fracwalltime;
+  out->c = bg.c;
   return 0;
 }
 

@@ -165,9 +165,10 @@ def make_data(spec, seed=5, rho=0.05, **kw):
     return make_subint(spec, seed=seed, rho=rho, delays_us=delays, rates_us_per_s=rates, **kw)
 
 
-def test_block_geometry_exact(gpu, oracle):
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_block_geometry_exact(gpu, oracle, order):
     lib, ctx = gpu
-    spec = c1_spec()
+    spec = c1_spec(fringe_order=order)
     job, keep = spec.build()
     sd = make_data(spec, invalid_frames={1: [2, 3]}, invalid_blocks={2: [5, 6, 40]})
     g = GpuJob(lib, ctx, job)
@@ -179,7 +180,7 @@ def test_block_geometry_exact(gpu, oracle):
             for f in ("status", "nearestsample", "integerdelay", "intwalltime"):
                 assert getattr(a, f) == getattr(b, f), (ds, blk, f)
             if b.status == 0:
-                for f in ("weight", "fracsampleerror", "a", "b", "fracwalltime"):
+                for f in ("weight", "fracsampleerror", "a", "b", "c", "fracwalltime"):
                     assert getattr(a, f) == getattr(b, f), (ds, blk, f, getattr(a, f), getattr(b, f))
     g.close()
 
@@ -228,6 +229,33 @@ def wide_spec(**kw):
     base = dict(nstations=20, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=40)
     base.update(kw)
     return JobSpec(**base)
+
+
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("mk", ["c1", "c2"])
+def test_visibilities_fp32_fringe_orders(gpu, oracle, mk, order):
+    """Post-F (order 0) and quadratic (order 2) fringe rotation, FP32 path,
+    with a curved delay model so the quadratic term matters."""
+    lib, ctx = gpu
+    spec = SPECS[mk](fringe_order=order, looffset_hz=[12.5] * SPECS[mk]().nsubbands)
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [1]}, invalid_blocks={1: [2]})
+    for i, st in enumerate(sd.stations):
+        st.interpolator = st.interpolator + np.array([3e-5 * (i - 1.5), 0.0, 0.0])
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    ref = oracle_results(oracle, job, sd)
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    assert rel_err(got[:xend], ref[:xend]) < FP32_TOL
+    assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL
+    assert np.array_equal(got[xend:a0].view(np.float32), ref[xend:a0].view(np.float32))
+    for ds in range(spec.nstations):
+        blk = spec.blockspersend - 1
+        refs = oracle_spectra(oracle, job, sd, ds, blk, spec.nrecbands, spec.nchan)
+        for band in range(spec.nrecbands):
+            assert rel_err(g.spectrum(ds, band, blk, spec.nchan), refs[band]) < FP32_TOL
+    g.close()
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])

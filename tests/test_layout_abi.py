@@ -198,7 +198,7 @@ def test_plan_without_gpu_and_error_paths():
     assert lib.fxg_plan(C.byref(job), C.byref(info)) == 0
     assert info.result_length > 0 and info.device_bytes > 0
     bad, keep2 = JobSpec(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256).build()
-    bad.fringe_order = 2
+    bad.fringe_order = 3
     assert lib.fxg_plan(C.byref(bad), C.byref(info)) == -4
     assert b"order" in lib.fxg_last_error()
     bad.fringe_order = 1

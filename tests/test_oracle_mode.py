@@ -12,6 +12,11 @@ closed forms and physics, independently of the sub/step rotator structure:
     cross spectrum's phase is 2 pi (sky frequency) delta, channel by channel
     (sign and scale of fringe and fractional-sample corrections and the
     conj(second station) order of the product);
+  - fringe order 0 (post-F): FFT of the unrotated samples times the block's
+    fringe phase at its centre delay and the fractional-sample rotation;
+  - fringe order 2 (quadratic): the piecewise phase of the sub/step/quad
+    rotators written out directly, and its distance from the exact quadratic
+    phase bounded by the chord error lofreq a (s/L)^2 / 4;
   - FP32 flavour against the FP64 flavour.
 """
 import ctypes as C
@@ -94,6 +99,81 @@ def test_station_spectrum_closed_form(oracle, lsb, looff):
                 assert rel_err(got[band], exp) < 1e-9, (ds, blk, band, rel_err(got[band], exp))
 
 
+@pytest.mark.parametrize("lsb", [0, 1])
+@pytest.mark.parametrize("looff", [0.0, 1234.567])
+def test_station_spectrum_order0_closed_form(oracle, lsb, looff):
+    spec = JobSpec(nstations=2, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=16,
+                   base_freq_mhz=8400.123, lsb=[lsb, lsb], clockoffset_us=[0.013, -0.021],
+                   looffset_hz=[looff, -looff], fringe_order=0)
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=7, delays_us=[3.2111, -7.777], rates_us_per_s=[0.8, -0.3])
+    L, Cn, ts, bw = spec.fftchannels, spec.nchan, spec.sampletime_us, spec.bw_mhz
+    for ds in range(2):
+        ip = sd.stations[ds].interpolator
+        for blk in (0, 9):
+            bi = oracle_block_info(oracle, job, sd, ds, blk)
+            assert bi.status == 0
+            fc = blk + 0.5
+            avgdelay = ip[0] * fc * fc + ip[1] * fc + ip[2]
+            assert bi.integerdelay == int(np.trunc(avgdelay)) and abs(bi.c - avgdelay) < 1e-12
+            x = unpacked(oracle, spec, job, sd, ds, bi.nearestsample)
+            got = oracle_spectra(oracle, job, sd, ds, blk, spec.nrecbands, Cn, fp64=True)
+            fse = model_starttime(spec, sd, ds, blk) - bi.nearestsample * ts
+            walltime = bi.intwalltime + bi.fracwalltime
+            c = np.arange(Cn)
+            lo_off = looff if ds == 0 else -looff
+            clk = 0.013 if ds == 0 else -0.021
+            for band in range(spec.nrecbands):
+                lofreq = spec.base_freq_mhz + (band // spec.npol) * bw
+                X = np.fft.fft(x[band])[c + Cn * lsb]
+                fringe = np.exp(-2j * np.pi * (avgdelay * lofreq - walltime * lo_off))
+                theta = 2 * np.pi * (c - Cn * lsb) * bw / Cn * (fse - clk)
+                exp = X * fringe * np.exp(1j * theta)
+                assert rel_err(got[band], exp) < 1e-9, (ds, blk, band, rel_err(got[band], exp))
+
+
+@pytest.mark.parametrize("lsb", [0, 1])
+def test_station_spectrum_order2_closed_form(oracle, lsb):
+    spec = JobSpec(nstations=2, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=16,
+                   base_freq_mhz=8400.123, lsb=[lsb, lsb], clockoffset_us=[0.013, -0.021],
+                   looffset_hz=[777.25, -31.5], fringe_order=2)
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=7, delays_us=[3.2111, -7.777], rates_us_per_s=[0.8, -0.3])
+    # a strongly curved model so the quadratic term matters
+    for st, q in zip(sd.stations, (2.5e-5, -4.0e-5)):
+        st.interpolator = st.interpolator + np.array([q, 0.0, 0.0])
+    L, Cn, ts, bw = spec.fftchannels, spec.nchan, spec.sampletime_us, spec.bw_mhz
+    s = oracle.orc_calcstridelength(Cn)
+    n = np.arange(L)
+    k, j = n % s, n // s
+    for ds in range(2):
+        ip = sd.stations[ds].interpolator
+        for blk in (0, 9):
+            bi = oracle_block_info(oracle, job, sd, ds, blk)
+            assert bi.status == 0
+            cc = ip[2] + blk * ip[1] + blk * blk * ip[0]
+            assert bi.integerdelay == int(np.trunc(cc)) and abs(bi.a - ip[0]) < 1e-15
+            x = unpacked(oracle, spec, job, sd, ds, bi.nearestsample)
+            got = oracle_spectra(oracle, job, sd, ds, blk, spec.nrecbands, Cn, fp64=True)
+            fse = model_starttime(spec, sd, ds, blk) - bi.nearestsample * ts
+            walltime = bi.intwalltime + bi.fracwalltime
+            c = np.arange(Cn)
+            lo_off = 777.25 if ds == 0 else -31.5
+            clk = 0.013 if ds == 0 else -0.021
+            # piecewise quadratic delay in block units: exact at the stride
+            # points j s, linear in between (the sub/quad/step rotators)
+            u = (bi.b * n + bi.a * (k * s + 2 * j * k * s + (j * s) ** 2) / L) / L + bi.c
+            exact = (bi.b * n + bi.a * n * n / L) / L + bi.c
+            assert np.max(np.abs(u - exact)) <= abs(bi.a) * (s / L) ** 2 / 4 * 1.000001
+            for band in range(spec.nrecbands):
+                lofreq = spec.base_freq_mhz + (band // spec.npol) * bw
+                phase = lofreq * (u + bi.integerdelay) - lo_off * (n * ts * 1e-6 + walltime)
+                X = np.fft.fft(x[band] * np.exp(-2j * np.pi * phase))[c + Cn * lsb]
+                theta = 2 * np.pi * (c - Cn * lsb) * bw / Cn * (fse - clk)
+                exp = X * np.exp(1j * theta)
+                assert rel_err(got[band], exp) < 1e-9, (ds, blk, band, rel_err(got[band], exp))
+
+
 def test_autocorrelation_parseval(oracle):
     spec = JobSpec(nstations=2, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=8)
     job, keep = spec.build()
@@ -114,10 +194,11 @@ def test_autocorrelation_parseval(oracle):
         assert np.max(np.abs(ac.imag)) == 0.0
 
 
+@pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("lsb", [0, 1])
-def test_cross_phase_equals_sky_frequency_times_delay(oracle, lsb):
+def test_cross_phase_equals_sky_frequency_times_delay(oracle, lsb, order):
     spec = JobSpec(nstations=2, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=8,
-                   base_freq_mhz=8400.0, lsb=[lsb])
+                   base_freq_mhz=8400.0, lsb=[lsb], fringe_order=order)
     job, keep = spec.build()
     ts = spec.sampletime_us
     delta = 0.3 * ts
