@@ -75,7 +75,7 @@ struct DevSeg {
   long long voff;       // element offset into V (SEG_VIS) or into the weight table
   int kind, nchan;
   int conj;             // take conj(V[b][a]) instead of V[a][b]
-  int vstride;          // unused for floats
+  int avg;              // channels averaged into one output channel (1: copy)
 };
 
 // ---- the XMAC description of one frequency --------------------------------

@@ -77,7 +77,8 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
   for (int f = 0; f < d->nfreqs; f++) {
     REQUIRE(d->freqs[f].nchan > 0, FXG_EINVAL, "freq nchan must be positive");
     REQUIRE(d->freqs[f].bw_mhz > 0, FXG_EINVAL, "freq bandwidth must be positive");
-    REQUIRE(d->freqs[f].chans_to_avg == 1, FXG_EUNSUPPORTED, "channel averaging is not built");
+    REQUIRE(d->freqs[f].chans_to_avg >= 1 && d->freqs[f].nchan % d->freqs[f].chans_to_avg == 0, FXG_EINVAL,
+            "chans_to_avg must divide nchan");
   }
   for (int s = 0; s < d->ndatastreams; s++) {
     const fxg_datastream& ds = d->datastreams[s];
@@ -191,7 +192,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       int lf = P.bllocal[b][f];
       if (lf < 0) continue;
       P.bloff[f][b] = at;
-      at += (long long)d->baselines[b].npolproducts[lf] * d->freqs[f].nchan;
+      at += (long long)d->baselines[b].npolproducts[lf] * d->freqs[f].nchan / d->freqs[f].chans_to_avg;
     }
   }
   for (int f = 0; f < d->nfreqs; f++) {
@@ -209,7 +210,8 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     const fxg_datastream& ds = d->datastreams[s];
     P.acoff[s] = at;
     for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++)
-      if (ac_wanted(freq_of_band(ds, k))) at += (long long)P.bandsperac * d->freqs[freq_of_band(ds, k)].nchan;
+      if (ac_wanted(freq_of_band(ds, k)))
+        at += (long long)P.bandsperac * d->freqs[freq_of_band(ds, k)].nchan / d->freqs[freq_of_band(ds, k)].chans_to_avg;
   }
   P.acwoff.resize(d->ndatastreams);
   for (int s = 0; s < d->ndatastreams; s++) {
@@ -328,12 +330,13 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     return -1;
   };
   bool seg_ok = true;
-  auto vis_seg = [&](long long resoff, int xfi, int a, int b, int nchan, int choff) {
+  // nchan output channels, each the mean of avg consecutive V channels from choff
+  auto vis_seg = [&](long long resoff, int xfi, int a, int b, int nchan, int choff, int avg) {
     if (xfi < 0 || a < 0 || b < 0) { seg_ok = false; return; }
     const XmacFreq& x = P.xf[xfi];
     int nin = (int)x.in_station.size();
     DevSeg g{};
-    g.kind = SEG_VIS; g.resoff = resoff; g.nchan = nchan;
+    g.kind = SEG_VIS; g.resoff = resoff; g.nchan = nchan; g.avg = avg;
     // V is accumulated for a <= b tiles and in full inside diagonal tiles;
     // read the upper triangle and conjugate when the order is swapped
     if (a <= b) { g.voff = x.voff + ((long long)a * nin + b) * x.nchan + choff; g.conj = 0; }
@@ -351,7 +354,9 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       for (int pp = 0; pp < bl.npolproducts[lf]; pp++) {
         int ia = input_index(xfi, bl.ds1, bl.band1[lf][pp]);
         int ib = input_index(xfi, bl.ds2, bl.band2[lf][pp]);
-        vis_seg(P.bloff[f][b] + (long long)pp * d->freqs[f].nchan, xfi, ia, ib, d->freqs[f].nchan, 0);
+        // channel averaging of the products (core.cpp:1788-1802)
+        const int avg = d->freqs[f].chans_to_avg, nout = d->freqs[f].nchan / avg;
+        vis_seg(P.bloff[f][b] + (long long)pp * nout, xfi, ia, ib, nout, 0, avg);
       }
     }
   }
@@ -377,14 +382,18 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
         int fk = freq_of_band(ds, k);
         if (!ac_wanted(fk)) continue;
-        int nch = d->freqs[fk].nchan;
+        // Mode::averageFrequency averages with the datastream's first recorded
+        // frequency's factor (configuration.cpp:962, mode.cpp:1399-1424); a zoom
+        // band then starts at choffset/avg of its averaged parent (mode.cpp:396)
+        const int savg = d->freqs[ds.recfreq_index[0]].chans_to_avg;
+        int nch = d->freqs[fk].nchan / d->freqs[fk].chans_to_avg;
         int parent = parent_of_band(ds, k);
         int pf = ds.recfreq_index[ds.recband_freq[parent]];
         int xfi = P.xf_of_freq[pf];
-        int choff = choff_of_band(ds, k);
+        int choff = (choff_of_band(ds, k) / savg) * savg;
         if (pass == 0) {
           int a = input_index(xfi, s, parent);
-          vis_seg(at2, xfi, a, a, nch, choff);
+          vis_seg(at2, xfi, a, a, nch, choff, savg);
         } else if (d->crosspol_autocorrs) {
           // the first two recorded bands of the parent's local frequency
           int lf = ds.recband_freq[parent];
@@ -393,7 +402,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
             if (ds.recband_freq[q] == lf) { if (i0 < 0) i0 = q; else if (i1 < 0) i1 = q; }
           if (i1 >= 0 && (parent == i0 || parent == i1)) {
             int other = parent == i0 ? i1 : i0;
-            vis_seg(at2, xfi, input_index(xfi, s, parent), input_index(xfi, s, other), nch, choff);
+            vis_seg(at2, xfi, input_index(xfi, s, parent), input_index(xfi, s, other), nch, choff, savg);
           }
         }
         at2 += nch;

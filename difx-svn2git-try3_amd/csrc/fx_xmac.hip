@@ -153,8 +153,8 @@ __global__ __launch_bounds__(256) void k_weights(const DevBlock* __restrict__ bl
 }
 
 // ---------------------------------------------------------------------------
-// scatter: result[seg.resoff + c] = V[seg.voff + c] (conjugated if flagged),
-// scaled by vscale; float segments copy one weight
+// scatter: result[seg.resoff + c] = mean of V[seg.voff + c avg + i], i < avg
+// (conjugated if flagged), scaled by vscale; float segments copy one weight
 // ---------------------------------------------------------------------------
 __global__ void k_scatter(const DevSeg* __restrict__ segs, int nseg, const float2* __restrict__ V,
                           const float* __restrict__ w, float vscale, float* __restrict__ res) {
@@ -167,7 +167,17 @@ __global__ void k_scatter(const DevSeg* __restrict__ segs, int nseg, const float
     return;
   }
   if (c >= g.nchan) return;
-  float2 v = V[g.voff + c];
+  float2 v;
+  if (g.avg == 1) {
+    v = V[g.voff + c];
+  } else {
+    // vectorMean_cf32 of the group: sum in channel order, then divide
+    v = make_float2(0.0f, 0.0f);
+    const float2* p = V + g.voff + (long long)c * g.avg;
+    for (int i = 0; i < g.avg; i++) { v.x += p[i].x; v.y += p[i].y; }
+    v.x /= (float)g.avg;
+    v.y /= (float)g.avg;
+  }
   if (g.conj) v.y = -v.y;
   res[2 * (g.resoff + c)] = v.x * vscale;
   res[2 * (g.resoff + c) + 1] = v.y * vscale;

@@ -146,6 +146,7 @@ class JobSpec:
     max_chunk_blocks: int = 0
     pols: str = "RL"
     fringe_order: int = 1
+    chans_to_avg: int = 1               # channel averaging of every frequency
 
     @property
     def nrecbands(self) -> int:
@@ -175,7 +176,7 @@ class JobSpec:
             f.bw_mhz = self.bw_mhz
             f.lsb = int(self.lsb[sb]) if sb < len(self.lsb) else 0
             f.nchan = self.nchan
-            f.chans_to_avg = 1
+            f.chans_to_avg = self.chans_to_avg
         for z, (sb, choff, znchan) in enumerate(self.zoom):
             f = freqs[nsb + z]
             parent = freqs[sb]
@@ -184,7 +185,7 @@ class JobSpec:
             f.bw_mhz = zbw
             f.lsb = parent.lsb
             f.nchan = znchan
-            f.chans_to_avg = 1
+            f.chans_to_avg = self.chans_to_avg
         dss = (Datastream * self.nstations)()
         for s in range(self.nstations):
             d = dss[s]

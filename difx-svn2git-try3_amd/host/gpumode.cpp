@@ -56,6 +56,7 @@ GpuMode::GpuMode(GpuCore* core, int dsindex, int numBufferedFFTs)
   nrec_ = d.nrecbands;
   nzoom_ = d.nzoombands;
   nchan_ = core->freqs_[d.recfreq_index[0]].nchan;
+  chanavg_ = std::max(1, core->freqs_[d.recfreq_index[0]].chans_to_avg);
   freqs_.assign(nbuf_, std::vector<cf32>((size_t)nrec_ * nchan_, cf32{0, 0}));
   conj_ = freqs_;
   dataweight_.assign(nbuf_, 0.0f);
@@ -117,6 +118,22 @@ void GpuMode::zeroAutocorrelations() {
     std::fill(ac_[p].begin(), ac_[p].end(), cf32{0, 0});
     std::fill(acw_[p].begin(), acw_[p].end(), 0.0f);
   }
+}
+
+void GpuMode::averageFrequency() {
+  if (chanavg_ == 1) return;
+  const int outputchans = nchan_ / chanavg_;
+  for (int p = 0; p < 2; p++)
+    for (int j = 0; j < nrec_; j++) {
+      cf32* a = &ac_[p][(size_t)j * nchan_];
+      for (int k = 0; k < outputchans; k++) {
+        cf32 m{0, 0};                                // genericMean_32fc
+        for (int i = 0; i < chanavg_; i++) { m.re += a[k * chanavg_ + i].re; m.im += a[k * chanavg_ + i].im; }
+        m.re /= chanavg_;
+        m.im /= chanavg_;
+        a[k] = m;
+      }
+    }
 }
 
 // acc += a * conj(b) in the operation order of vectorAddProduct_cf32 applied
@@ -188,8 +205,8 @@ const cf32* GpuMode::getAutocorrelation(bool crosspol, int band) const {
   const fxg_datastream& d = core_->dss_[ds_];
   const std::vector<cf32>& a = ac_[crosspol ? 1 : 0];
   if (band < nrec_) return &a[(size_t)band * nchan_];
-  int z = band - nrec_;
-  return &a[(size_t)d.zoomband_parent[z] * nchan_ + d.zoomband_choffset[z]];
+  int z = band - nrec_;   // zoom bands alias the (averaged) parent, mode.cpp:396
+  return &a[(size_t)d.zoomband_parent[z] * nchan_ + d.zoomband_choffset[z] / chanavg_];
 }
 
 f32 GpuMode::getWeight(bool crosspol, int band) const {

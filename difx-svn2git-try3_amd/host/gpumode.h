@@ -70,6 +70,9 @@ class GpuMode {
   void setOffsets(int scan, int seconds, int ns, const f64 interpolator[3], bool foundok = true);
   // mode.cpp:1480-1493
   void zeroAutocorrelations();
+  // mode.cpp:1399-1424: autocorrelations averaged in place over the
+  // datastream's channelstoaverage (its first recorded frequency's)
+  void averageFrequency();
   // mode.cpp:649-1397 for one FFT block
   void process(int index, int subloopindex);
   // mode.h:199, 207 (band index counts zoom bands after the recorded bands)
@@ -92,7 +95,7 @@ class GpuMode {
   int ds_;
   bool initok_ = true;
   int nbuf_;
-  int nrec_, nzoom_, nchan_;
+  int nrec_, nzoom_, nchan_, chanavg_;
   // inputs of the current subintegration (what Core hands Mode, core.cpp:705-730)
   fxg_station_input in_{};
   std::vector<s32> flags_;

@@ -333,7 +333,8 @@ static int validate(const fxg_job_desc* j) {
   if (j->blockspersend <= 0) return fail(FXG_EINVAL, "blockspersend");
   if (j->fringe_order < 0 || j->fringe_order > 2) return fail(FXG_EUNSUPPORTED, "fringe rotation order must be 0, 1 or 2");
   for (int f = 0; f < j->nfreqs; f++) {
-    if (j->freqs[f].chans_to_avg != 1) return fail(FXG_EUNSUPPORTED, "chans_to_avg must be 1");
+    if (j->freqs[f].chans_to_avg < 1 || j->freqs[f].nchan % j->freqs[f].chans_to_avg)
+      return fail(FXG_EINVAL, "chans_to_avg must divide nchan");
     if (j->freqs[f].nchan <= 0) return fail(FXG_EINVAL, "nchan");
   }
   for (int d = 0; d < j->ndatastreams; d++) {
@@ -772,6 +773,18 @@ struct CoreRun {
       w0[d].assign(ds.nrecbands, 0); w1[d].assign(ds.nrecbands, 0);
     }
     std::vector<float> dweight(nds);
+    /* this thread's cross products before averaging (threadcrosscorrs):
+     * [freq][baseline][product][nchan] at xoff[f][b] */
+    std::vector<std::vector<int64_t>> xoff(j->nfreqs, std::vector<int64_t>(j->nbaselines, -1));
+    int64_t xlen = 0;
+    for (int f = 0; f < j->nfreqs; f++) {
+      if (!lay.freqused[f]) continue;
+      for (int b = 0; b < j->nbaselines; b++) {
+        int lf = lay.blfreq_local[b][f]; if (lf < 0) continue;
+        xoff[f][b] = xlen; xlen += (int64_t)j->baselines[b].npolproducts[lf] * j->freqs[f].nchan;
+      }
+    }
+    std::vector<T> tx(2 * (size_t)xlen, (T)0);
     for (int blk = b0; blk < b1; blk++) {
       for (int d = 0; d < nds; d++) {
         const fxg_datastream& ds = j->datastreams[d];
@@ -809,7 +822,7 @@ struct CoreRun {
           for (int p = 0; p < bl.npolproducts[lf]; p++) {
             const cpx<T>* v1 = band_spectrum(bl.ds1, bl.band1[lf][p], spec);
             const cpx<T>* v2 = band_spectrum(bl.ds2, bl.band2[lf][p], spec);
-            T* out = acc.data() + 2 * (size_t)(lay.bloff[f][b] + (int64_t)p * nchan);
+            T* out = tx.data() + 2 * (size_t)(xoff[f][b] + (int64_t)p * nchan);
             for (int c = 0; c < nchan; c++) addprod_conj<T, T>(out + 2 * c, v1[c], v2[c]);
             T* bw = acc.data() + 2 * (size_t)lay.bwoff[f][b] + p;
             *bw += (T)(dweight[bl.ds1] * dweight[bl.ds2]);
@@ -817,18 +830,64 @@ struct CoreRun {
         }
       }
     }
-    /* averageAndSendAutocorrs (core.cpp:1299-1369) */
+    /* uvshiftAndAverageBaselineFreq, single phase centre (core.cpp:1586-1802):
+     * per xmac stride, copy, or average groups of channelinc channels with
+     * vectorMean and add mean / stridestoaverage */
+    const int xmacstridelen = lay.xmacstridelen;
+    for (int f = 0; f < j->nfreqs; f++) {
+      if (!lay.freqused[f]) continue;
+      const int nchan = j->freqs[f].nchan, channelinc = j->freqs[f].chans_to_avg;
+      int stridestoaverage = channelinc / xmacstridelen; if (stridestoaverage == 0) stridestoaverage = 1;
+      int averagesperstride = xmacstridelen / channelinc; if (averagesperstride == 0) averagesperstride = 1;
+      const int averagelength = xmacstridelen / averagesperstride;
+      for (int b = 0; b < j->nbaselines; b++) {
+        int lf = lay.blfreq_local[b][f]; if (lf < 0) continue;
+        for (int p = 0; p < j->baselines[b].npolproducts[lf]; p++) {
+          for (int x = 0; x < nchan / xmacstridelen; x++) {
+            const T* src = tx.data() + 2 * (size_t)(xoff[f][b] + (int64_t)p * nchan + (int64_t)x * xmacstridelen);
+            int64_t dest = lay.bloff[f][b] + ((int64_t)p * nchan + (int64_t)x * xmacstridelen) / channelinc;
+            if (channelinc == 1) {
+              for (int c = 0; c < 2 * xmacstridelen; c++) acc[2 * (size_t)dest + c] += src[c];
+            } else {
+              for (int l = 0; l < averagesperstride; l++) {
+                T mre = 0, mim = 0;                          /* genericMean_32fc */
+                for (int i = 0; i < averagelength; i++) { mre += src[2 * (l * averagelength + i)]; mim += src[2 * (l * averagelength + i) + 1]; }
+                mre /= averagelength; mim /= averagelength;
+                acc[2 * (size_t)dest] += mre / stridestoaverage;
+                acc[2 * (size_t)dest + 1] += mim / stridestoaverage;
+                dest++;
+              }
+            }
+          }
+        }
+      }
+    }
+    /* Mode::averageFrequency with the datastream's channelstoaverage (its
+     * first recorded frequency's, configuration.cpp:962), then
+     * averageAndSendAutocorrs (core.cpp:1253-1369) */
     for (int d = 0; d < nds; d++) {
       const fxg_datastream& ds = j->datastreams[d];
+      const int savg = j->freqs[ds.recfreq_index[0]].chans_to_avg;
+      if (savg > 1) {
+        for (std::vector<T>* ac : {&ac0[d], &ac1[d]})
+          for (int jb = 0; jb < ds.nrecbands; jb++) {
+            T* a = ac->data() + (size_t)jb * geom[d].C * 2;
+            for (int k = 0; k < geom[d].C / savg; k++) {
+              T mre = 0, mim = 0;
+              for (int i = 0; i < savg; i++) { mre += a[2 * (k * savg + i)]; mim += a[2 * (k * savg + i) + 1]; }
+              a[2 * k] = mre / savg; a[2 * k + 1] = mim / savg;
+            }
+          }
+      }
       int64_t ri = lay.acoff[d];
       for (int pass = 0; pass < lay.bandsperautocorr; pass++) {
         if (pass == 1 && !j->crosspol_autocorrs) break;
         for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
           int fi = ds_band_freqindex(ds, k);
           if (!(lay.freqused[fi] || lay.equivused[fi])) continue;
-          int nch = j->freqs[fi].nchan;
+          int nch = j->freqs[fi].nchan / j->freqs[fi].chans_to_avg;
           int parent = k < ds.nrecbands ? k : ds.zoomband_parent[k - ds.nrecbands];
-          int choff = k < ds.nrecbands ? 0 : ds.zoomband_choffset[k - ds.nrecbands];
+          int choff = k < ds.nrecbands ? 0 : ds.zoomband_choffset[k - ds.nrecbands] / savg;   /* mode.cpp:396 */
           const T* src = (pass == 0 ? ac0[d] : ac1[d]).data() + ((size_t)parent * geom[d].C + choff) * 2;
           for (int c = 0; c < 2 * nch; c++) acc[2 * (size_t)ri + c] += src[c];
           ri += nch;

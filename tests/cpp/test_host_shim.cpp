@@ -108,9 +108,10 @@ int main(int argc, char** argv) {
       }
     }
   }
-  // autocorrelations and their weights (core.cpp:1299-1369), one pass per slot
+  // autocorrelations and their weights (core.cpp:1253-1369), one pass per slot
   for (int d = 0; d < nds; d++) {
     GpuMode* m = core.getMode(d);
+    m->averageFrequency();
     const fxg_datastream& ds = dss[d];
     long long at = core.getCoreResultAutocorrOffset(d);
     float* wat = (float*)&rb[core.getCoreResultACWeightOffset(d)];
@@ -122,7 +123,7 @@ int main(int argc, char** argv) {
         bool used = false;
         for (int b = 0; b < nbl; b++) for (int q = 0; q < bls[b].nfreqs; q++) used |= bls[b].freq_index[q] == fk;
         if (!used) continue;
-        int nchan = freqs[fk].nchan;
+        int nchan = freqs[fk].nchan / freqs[fk].chans_to_avg;
         if (pass == 0 || m->writeCrossAutoCorrs()) {
           const cf32* a = m->getAutocorrelation(pass == 1, k);
           for (int c = 0; c < nchan; c++) { rb[at + c].re += a[c].re; rb[at + c].im += a[c].im; }

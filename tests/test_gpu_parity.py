@@ -258,6 +258,28 @@ def test_visibilities_fp32_fringe_orders(gpu, oracle, mk, order):
     g.close()
 
 
+@pytest.mark.parametrize("avg,zoom", [(4, [(0, 37, 64)]), (256, [])])
+def test_visibilities_fp32_channel_averaging(gpu, oracle, avg, zoom):
+    """Channel averaging of cross products (core.cpp:1788-1802) and of
+    autocorrelations (mode.cpp:1399-1424), with a zoom band whose offset is
+    not a multiple of the averaging."""
+    lib, ctx = gpu
+    spec = JobSpec(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=12, zoom=zoom,
+                   chans_to_avg=avg)
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [1]})
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd)
+    info = plan_info(oracle, "orc_", job)
+    assert g.info.result_length == info.result_length
+    xend, a0, a1 = region_slices(oracle, job, info)
+    assert rel_err(got[:xend], ref[:xend]) < FP32_TOL
+    assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL
+    assert np.array_equal(got[xend:a0].view(np.float32), ref[xend:a0].view(np.float32))
+    g.close()
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("mk", ["c1", "c2", "wide"])
 def test_visibilities_mixed_vs_fp64(gpu, oracle, prec, mk):
