@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <deque>
 #include <vector>
 
 using namespace fxg;
@@ -85,21 +86,29 @@ struct fxg_job {
   // the subintegration whose frames / corner turn / geometry are resident
   std::vector<unsigned char> prepared_key;
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
+  // asynchronous submissions: (ticket, completion event) in submission order
+  std::deque<std::pair<int64_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> free_events;
+  int64_t next_ticket = 1;
   // stage profiling
   int profiling = 0;
   std::vector<hipEvent_t> evpool;
   std::vector<std::pair<int, int>> evused;   // (stage, pool index of the start event)
   double stage_ms[FXG_NSTAGES] = {0};
   long long stage_n[FXG_NSTAGES] = {0};
+  std::vector<int> evfree;                     // free pairs of evpool (index of the start event)
   int ev_begin(int stage) {
     if (!profiling) return 0;
-    size_t need = 2 * (evused.size() + 1);
-    while (evpool.size() < need) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return set_err(FXG_EHIP, "hipEventCreate");
-      evpool.push_back(e);
+    if (evfree.empty()) {
+      for (int k = 0; k < 2; k++) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return set_err(FXG_EHIP, "hipEventCreate");
+        evpool.push_back(e);
+      }
+      evfree.push_back((int)evpool.size() - 2);
     }
-    int idx = (int)(2 * evused.size());
+    int idx = evfree.back();
+    evfree.pop_back();
     evused.push_back({stage, idx});
     if (hipEventRecord(evpool[idx], stream) != hipSuccess) return set_err(FXG_EHIP, "hipEventRecord");
     return 0;
@@ -109,15 +118,20 @@ struct fxg_job {
     if (hipEventRecord(evpool[evused.back().second + 1], stream) != hipSuccess) return set_err(FXG_EHIP, "hipEventRecord");
     return 0;
   }
+  // accumulate the stages whose events have completed; keep the rest
   void ev_collect() {
+    std::vector<std::pair<int, int>> keep;
     for (auto& u : evused) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, evpool[u.second], evpool[u.second + 1]) == hipSuccess) {
+      hipError_t e = hipEventElapsedTime(&ms, evpool[u.second], evpool[u.second + 1]);
+      if (e == hipErrorNotReady) { keep.push_back(u); continue; }
+      if (e == hipSuccess) {
         stage_ms[u.first] += ms;
         stage_n[u.first] += 1;
       }
+      evfree.push_back(u.second);
     }
-    evused.clear();
+    evused.swap(keep);
   }
 };
 
@@ -193,6 +207,16 @@ int fxg_ctx_create(int device, fxg_ctx** out) {
   if (c->lut8.ensure(sizeof(lut))) { hipStreamDestroy(c->stream); delete c; return FXG_ENOMEM; }
   HIPCHK(hipMemcpy(c->lut8.p, lut, sizeof(lut), hipMemcpyHostToDevice));
   *out = c;
+  return 0;
+}
+
+int fxg_ctx_memory(fxg_ctx* c, int64_t* free_bytes, int64_t* total_bytes) {
+  if (!c || !free_bytes || !total_bytes) return set_err(FXG_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  *free_bytes = (int64_t)fr;
+  *total_bytes = (int64_t)tot;
   return 0;
 }
 
@@ -305,6 +329,10 @@ static void job_free(fxg_job* j) {
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
   j->evpool.clear();
+  for (auto& p : j->pending) hipEventDestroy(p.second);
+  j->pending.clear();
+  for (hipEvent_t e : j->free_events) hipEventDestroy(e);
+  j->free_events.clear();
   if (j->stream) hipStreamDestroy(j->stream);
 }
 
@@ -545,8 +573,9 @@ static std::vector<unsigned char> subint_key(const fxg_job* j, const fxg_subint_
   return k;
 }
 
-int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
-                       int data_on_device, float* results, int results_on_device) {
+// Everything of one subintegration, enqueued on the job's stream.
+static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                          int data_on_device, float* results, int results_on_device) {
   if (!j || !sub || !st || !results) return set_err(FXG_EINVAL, "null argument");
   Plan& P = j->plan;
   const fxg_job_desc& D = j->desc;
@@ -611,9 +640,62 @@ int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station
     HIPCHK(hipMemcpyAsync(results, dres, resbytes, hipMemcpyDeviceToHost, s));
     if ((rc = j->ev_end())) return rc;
   }
-  HIPCHK(hipStreamSynchronize(s));
-  j->ev_collect();
   j->have_subint = 1;
+  return 0;
+}
+
+int fxg_subint_process(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                       int data_on_device, float* results, int results_on_device) {
+  int rc = enqueue_subint(j, sub, st, data_on_device, results, results_on_device);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(j->stream));
+  j->ev_collect();
+  return 0;
+}
+
+int fxg_subint_submit(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                      int data_on_device, float* results, int results_on_device, int64_t* ticket) {
+  if (!ticket) return set_err(FXG_EINVAL, "null ticket");
+  int rc = enqueue_subint(j, sub, st, data_on_device, results, results_on_device);
+  if (rc) return rc;
+  hipEvent_t e;
+  if (!j->free_events.empty()) {
+    e = j->free_events.back();
+    j->free_events.pop_back();
+  } else {
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  HIPCHK(hipEventRecord(e, j->stream));
+  *ticket = j->next_ticket++;
+  j->pending.push_back({*ticket, e});
+  return 0;
+}
+
+int fxg_subint_query(fxg_job* j, int64_t ticket, int* done) {
+  if (!j || !done) return set_err(FXG_EINVAL, "null argument");
+  if (ticket <= 0 || ticket >= j->next_ticket) return set_err(FXG_EINVAL, "unknown ticket");
+  *done = 1;
+  for (auto& p : j->pending)
+    if (p.first == ticket) {
+      hipError_t q = hipEventQuery(p.second);
+      if (q == hipErrorNotReady) *done = 0;
+      else if (q != hipSuccess) return set_err(FXG_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+      break;
+    }
+  return 0;
+}
+
+int fxg_subint_wait(fxg_job* j, int64_t ticket) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  if (ticket <= 0 || ticket >= j->next_ticket) return set_err(FXG_EINVAL, "unknown ticket");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  // tickets complete in submission order on the job's stream
+  while (!j->pending.empty() && j->pending.front().first <= ticket) {
+    HIPCHK(hipEventSynchronize(j->pending.front().second));
+    j->free_events.push_back(j->pending.front().second);
+    j->pending.pop_front();
+  }
+  j->ev_collect();
   return 0;
 }
 

@@ -31,6 +31,7 @@ FXG_BLANKER_NONE = 1
 FXG_XMAC_FP32 = 0
 FXG_XMAC_FP16 = 1
 FXG_XMAC_BF16 = 2
+FXG_OK, FXG_EINVAL, FXG_ENOMEM, FXG_EHIP, FXG_EUNSUPPORTED, FXG_ESTATE, FXG_ENODEV = 0, -1, -2, -3, -4, -5, -6
 FXG_INVALID_SUBINT = -99999999
 FXG_FLAGS_PER_INT = 30
 STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h"]
@@ -297,12 +298,49 @@ def load_gpu_lib():
     lib.fxg_job_plan_info.argtypes = [C.c_void_p, P(PlanInfo)]
     lib.fxg_subint_process.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int,
                                        C.c_void_p, C.c_int]
+    lib.fxg_subint_submit.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int,
+                                      C.c_void_p, C.c_int, P(C.c_int64)]
+    lib.fxg_subint_query.argtypes = [C.c_void_p, C.c_int64, P(C.c_int)]
+    lib.fxg_subint_wait.argtypes = [C.c_void_p, C.c_int64]
+    lib.fxg_station_process_block.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int,
+                                              C.c_int, C.c_int, C.c_void_p, P(BlockInfo)]
     lib.fxg_job_set_keep_taps.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]
     lib.fxg_tap_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     _gpu_lib = lib
+    return lib
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("budget_bytes", C.c_int64), ("used_bytes", C.c_int64), ("nconfigs", C.c_int32),
+                ("nslots", C.c_int32), ("submitted", C.c_int64), ("completed", C.c_int64),
+                ("queued", C.c_int64), ("refused", C.c_int64)]
+
+
+_HOST_LIB_PATH = os.path.join(os.path.dirname(_GPU_LIB_PATH), "libfxcorr_host.so")
+_host_lib = None
+
+
+def load_host_lib():
+    """Load libfxcorr_host.so (the C++ Mode/Core/Visibility layer and the
+    request loop, include/fxcorr_serve.h); raises if it has not been built."""
+    global _host_lib
+    if _host_lib is not None:
+        return _host_lib
+    load_gpu_lib()
+    if not os.path.exists(_HOST_LIB_PATH):
+        raise RuntimeError(f"host library not built: {_HOST_LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(_HOST_LIB_PATH)
+    P = C.POINTER
+    lib.fxs_create.argtypes = [C.c_int, C.c_int64, P(C.c_void_p)]
+    lib.fxs_destroy.argtypes = [C.c_void_p]
+    lib.fxs_add_config.argtypes = [C.c_void_p, P(JobDesc), C.c_int, P(C.c_int)]
+    lib.fxs_submit.argtypes = [C.c_void_p, C.c_int, P(SubintDesc), P(StationInput), C.c_void_p, P(C.c_int64)]
+    lib.fxs_wait.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64)]
+    lib.fxs_stats.argtypes = [C.c_void_p, P(ServeStats)]
+    _host_lib = lib
     return lib
 
 

@@ -215,6 +215,8 @@ typedef struct fxg_job fxg_job;
 
 int fxg_ctx_create(int device, fxg_ctx** ctx);
 int fxg_ctx_destroy(fxg_ctx* ctx);
+/* Free and total HBM of the context's device (admission by footprint). */
+int fxg_ctx_memory(fxg_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Device allocations for callers that keep their raw data resident. */
 int fxg_device_alloc(fxg_ctx* ctx, int64_t bytes, void** dptr);
 int fxg_device_free(fxg_ctx* ctx, void* dptr);
@@ -244,6 +246,20 @@ int fxg_job_plan_info(const fxg_job* job, fxg_plan_info* info);
 int fxg_subint_process(fxg_job* job, const fxg_subint_desc* sub,
                        const fxg_station_input* stations, int data_on_device,
                        float* results, int results_on_device);
+
+/* The same, asynchronous: enqueues the subintegration on the job's stream and
+ * returns a ticket (core.cpp:539's per-slot call, split so that a request loop
+ * can keep several slots of several jobs in flight).  Inputs and results must
+ * stay valid until fxg_subint_wait returns for the ticket; with host results
+ * (results_on_device == 0) the call may block on the copy.  Tickets of one job
+ * complete in submission order. */
+int fxg_subint_submit(fxg_job* job, const fxg_subint_desc* sub,
+                      const fxg_station_input* stations, int data_on_device,
+                      float* results, int results_on_device, int64_t* ticket);
+/* *done = 1 when the ticket's work has finished, else 0. */
+int fxg_subint_query(fxg_job* job, int64_t ticket, int* done);
+/* Blocks until the ticket (and every earlier one of the job) has finished. */
+int fxg_subint_wait(fxg_job* job, int64_t ticket);
 
 /* Mode::process for one block of one datastream (mode.cpp:649-1397): the
  * spectra of all its recorded bands (nrecbands x nchan cf32 as float pairs,

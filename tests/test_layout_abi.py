@@ -191,6 +191,15 @@ def test_library_exports_every_declared_symbol():
     assert lib.fxg_abi_version() == 1
 
 
+def test_host_library_exports_serving_abi():
+    host = fxcorr.load_host_lib()
+    hdr = open(os.path.join(REPO, "include", "fxcorr_serve.h")).read()
+    syms = sorted(set(re.findall(r"^\s*int\s+(fxs_\w+)\s*\(", hdr, re.M)))
+    assert len(syms) == 6
+    for s in syms:
+        assert hasattr(host, s), s
+
+
 def test_plan_without_gpu_and_error_paths():
     lib = fxcorr.load_gpu_lib()
     job, keep = JobSpec(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256).build()
