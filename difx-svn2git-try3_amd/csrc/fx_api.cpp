@@ -86,6 +86,8 @@ struct fxg_job {
   // the subintegration whose frames / corner turn / geometry are resident
   std::vector<unsigned char> prepared_key;
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
+  DBuf d_zero;                    // 64 zero bytes: the rows past the data for streaming loads
+  int xmac_streaming = 0;         // direct-to-LDS MFMA XMAC where the layout allows
   // asynchronous submissions: (ticket, completion event) in submission order
   std::deque<std::pair<int64_t, hipEvent_t>> pending;
   std::vector<hipEvent_t> free_events;
@@ -324,7 +326,7 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
-                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota};
+                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -348,10 +350,10 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     // device-side limits of this build
     std::string why;
     if (!P.uniform) why = "all datastreams of a job must share nchan";
-    else if (!(P.L >= 128 && P.L <= 8192 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 8192]";
+    else if (!(P.L >= 128 && P.L <= 16384 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 16384]";
     for (const DevStation& st : P.st)
-      if (why.empty() && (st.order == 2 ? 3 : 2) * st.s + st.L / st.s + st.C / st.s > 512)
-        why = "array stride gives more than 512 rotator sub/step points";
+      if (why.empty() && (st.order == 2 ? 3 : 2) * st.s + st.L / st.s + st.C / st.s > fxg_rotator_table_entries(st.L))
+        why = "array stride gives more rotator sub/step points than the F stage holds";
     if (!why.empty()) { delete j; return set_err(FXG_EUNSUPPORTED, why); }
   }
   // own copies of the descriptor tables
@@ -427,6 +429,8 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   if ((rc = j->d_iota.ensure(sizeof(int) * std::max<size_t>(1, iota.size())))) return fail(rc);
   if (hipMemcpy(j->d_iota.p, iota.data(), sizeof(int) * iota.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_err(FXG_EHIP, "band index upload failed"));
+  if ((rc = j->d_zero.ensure(64))) return fail(rc);
+  if (hipMemset(j->d_zero.p, 0, 64) != hipSuccess) return fail(set_err(FXG_EHIP, "zero row"));
   j->d_raw.resize(P.nds);
   *out = j;
   return 0;
@@ -444,6 +448,12 @@ int fxg_job_destroy(fxg_job* j) {
 int fxg_job_plan_info(const fxg_job* j, fxg_plan_info* info) {
   if (!j || !info) return set_err(FXG_EINVAL, "null argument");
   *info = j->plan.info;
+  return 0;
+}
+
+int fxg_job_set_xmac_streaming(fxg_job* j, int on) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  j->xmac_streaming = on ? 1 : 0;
   return 0;
 }
 
@@ -616,8 +626,14 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
       xa.V = j->d_V.as<float2>() + xf.voff;
       xa.accumulate = c0 > 0;
       if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
-      if (prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan)) HIPCHK(launch_xmac_mfma(xa, prec, s));
-      else HIPCHK(launch_xmac_f32(xa, prec, s));
+      if (prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan)) {
+        if (j->xmac_streaming && xmac_mfma_streaming_ok(xa, xf.in_off.data()))
+          HIPCHK(launch_xmac_mfma_streaming(xa, prec, j->d_zero.p, s));
+        else
+          HIPCHK(launch_xmac_mfma(xa, prec, s));
+      } else {
+        HIPCHK(launch_xmac_f32(xa, prec, s));
+      }
       if ((rc = j->ev_end())) return rc;
     }
     j->tap_chunk0 = c0;

@@ -6,6 +6,10 @@
 
 namespace fxg {
 
+// LDS entries for an F-stage workgroup's rotator tables (sub, step,
+// fractional sub and step, quadratic pieces) at FFT length L
+constexpr int fxg_rotator_table_entries(int L) { return L >= 16384 ? 768 : 512; }
+
 struct FStageArgs {
   const DevStation* st;
   const DevBand* bands;
@@ -47,6 +51,9 @@ hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const in
 hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream);
 bool xmac_mfma_supported(int nin, int nchan);
 hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream);
+// the direct-to-LDS double-buffered variant; needs 16-byte aligned rows
+bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host);
+hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream);
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream);
 hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w,

@@ -310,7 +310,7 @@ template <> struct DFT<16> {
 // Every FFT thread holds 16 points in every pass.
 template <int L> struct FftShape {
   static constexpr int LOG = (L == 128) ? 7 : (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10
-                           : (L == 2048) ? 11 : (L == 4096) ? 12 : 13;
+                           : (L == 2048) ? 11 : (L == 4096) ? 12 : (L == 8192) ? 13 : 14;
   static constexpr int Q = LOG % 4;
   static constexpr int RL = Q ? (1 << Q) : 16;                    // last-pass radix
   static constexpr int NMID = Q ? LOG / 4 - 1 : LOG / 4 - 2;      // radix-16 passes between
@@ -318,6 +318,7 @@ template <int L> struct FftShape {
   static constexpr int NT = NTF < 64 ? 64 : NTF;                  // block size
   static constexpr int PAD = L + L / 16;                          // padded LDS length
   static constexpr int NSL = L / RL;                              // Ns of the last pass
+  static constexpr int TABS = fxg_rotator_table_entries(L);        // rotator sub/step/quad points
 };
 
 // twiddle e^{-2 pi i m/L} from split tables (m < L): one complex product
@@ -491,9 +492,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
   constexpr int RAWW = (L * NBITS) / 32 + 2;    // staged stream words of one block
   __shared__ float2 buf[Sh::PAD];
-  __shared__ float2 tabs[512];                  // rotator sub/step tables, packed
+  __shared__ float2 tabs[Sh::TABS];             // rotator sub/step tables, packed
   __shared__ float2 twlo[64], twhi[L / 64];     // e^{-2 pi i m/L}, e^{-2 pi i 64h/L}
-  __shared__ unsigned raw[RAWW];
+  // 8-bit samples at the largest L: the staged stream words share the tail of
+  // the FFT buffer (pass 1 reads them all before its first write, below)
+  constexpr bool kAliasRaw = NBITS == 8 && L >= 16384;
+  __shared__ unsigned raw_own[kAliasRaw ? 1 : RAWW];
+  unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : raw_own;
   __shared__ float lut8s[NBITS == 8 ? 256 : 4];  // level table indexed by the sample code
 
   const int tid = threadIdx.x;
@@ -625,9 +630,9 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   const bool rotfast = slog >= 0 && (NTF & (s - 1)) == 0;
 
   // ---- pass 1 (Ns = 1): decode, fringe-rotate, radix-16 DFT, to LDS
+  float2 v[16];
   if (tid < NTF) {
     const int j = tid;
-    float2 v[16];
     if (kWordStep && fast && rotfast && S.order == 1) {
       const int bp = bitoff + NBITS * j;
       const unsigned* wp = raw + (bp >> 5);
@@ -705,7 +710,10 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       }
     }
     DFT<16>::run(v);
-    float2* o = buf + 17 * j;                // pidx(16 j + r) = 17 j + r
+  }
+  if (kAliasRaw) __syncthreads();            // every stream word read before buf is written
+  if (tid < NTF) {
+    float2* o = buf + 17 * tid;              // pidx(16 j + r) = 17 j + r
 #pragma unroll
     for (int r = 0; r < 16; r++) o[r] = v[r];
   }
@@ -791,7 +799,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
 #define INSTANTIATE(Lv) template __global__ void k_fstage<Lv, 2>(FStageArgs); \
                         template __global__ void k_fstage<Lv, 8>(FStageArgs);
 INSTANTIATE(128) INSTANTIATE(256) INSTANTIATE(512) INSTANTIATE(1024)
-INSTANTIATE(2048) INSTANTIATE(4096) INSTANTIATE(8192)
+INSTANTIATE(2048) INSTANTIATE(4096) INSTANTIATE(8192) INSTANTIATE(16384)
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -819,7 +827,7 @@ hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int 
       if (nbits == 2) hipLaunchKernelGGL((k_fstage<Lv, 2>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
       else hipLaunchKernelGGL((k_fstage<Lv, 8>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
       break;
-    CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192)
+    CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192) CASE(16384)
 #undef CASE
     default: return hipErrorInvalidValue;
   }

@@ -131,7 +131,139 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming variant: the spectra go straight from HBM into LDS with gfx950's
+// global_load_lds_dwordx4 (no registers), double-buffered, so the loads of
+// the next 16 blocks are in flight while the matrix cores work on the
+// current ones.  LDS per stage: [input][block][16 channels], one 64-byte row
+// per (input, block), inputs padded to 260 dwords.  One load instruction
+// fills 16 rows (16 blocks) of one input; rows past the last block or past
+// the last input read a 16-byte zero buffer.
+// ---------------------------------------------------------------------------
+template <int T, bool BF>
+__global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* __restrict__ zero16) {
+  constexpr int CG = 16;                 // channels (wavefronts) per workgroup
+  constexpr int TT = 16;                 // blocks per stage
+  constexpr int NR = 32 * T;             // inputs, padded to the tile size
+  constexpr int NTILE = T * (T + 1) / 2;
+  constexpr int SDW = TT * CG + 4;       // dwords per input per stage
+  constexpr int STAGE = NR * SDW;        // dwords per stage
+  constexpr int LPW = NR / CG;           // load instructions per wave per stage
+  __shared__ __align__(16) unsigned lds[2 * STAGE];
+
+  const char* spec = (const char*)A.spec;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int nwg = gridDim.x;
+  const int bid = blockIdx.x;
+  const int grp = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;   // XCD-aware groups
+  const int c0 = grp * CG;
+  const int half = l >> 5, row = l & 31;
+  const int lrow = l >> 2, lq = l & 3;   // load lane: block row and 4-channel quad
+  // the inputs this wavefront loads, with their offsets read once into
+  // scalar registers (a vector load here would make every later wait for
+  // it also wait for the loads in flight)
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  long long ioff[LPW];
+#pragma unroll
+  for (int u = 0; u < LPW; u++) {
+    const int a = wu * LPW + u;
+    ioff[u] = a < A.nin ? A.in_off[a] + c0 : -1;
+  }
+
+  auto issue = [&](int stage, int buf) {
+    const int t = stage * TT + lrow;
+#pragma unroll
+    for (int u = 0; u < LPW; u++) {
+      const void* src = zero16;
+      if (ioff[u] >= 0 && t < A.nblocks)
+        src = spec + 4 * (ioff[u] + (long long)t * A.block_stride + 4 * lq);
+      __builtin_amdgcn_global_load_lds(src,
+                                       (__attribute__((address_space(3))) void*)(lds + buf * STAGE + (wu * LPW + u) * SDW),
+                                       16, 0, 0);
+    }
+  };
+
+  f16_t accR[NTILE], accI[NTILE];
+#pragma unroll
+  for (int q = 0; q < NTILE; q++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) { accR[q][e] = 0.0f; accI[q][e] = 0.0f; }
+
+  const int nstage = (A.nblocks + TT - 1) / TT;
+  issue(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int sg = 0; sg < nstage; sg++) {
+    const int buf = sg & 1;
+    if (sg + 1 < nstage) issue(sg + 1, buf ^ 1);
+    const unsigned* base = lds + buf * STAGE + w;                 // this wavefront's channel
+#pragma unroll
+    for (int ks = 0; ks < TT / 8; ks++) {
+      uint4 f[T];
+#pragma unroll
+      for (int I = 0; I < T; I++) {
+        const unsigned* p = base + (32 * I + row) * SDW + (8 * ks + 4 * half) * CG;
+        f[I] = make_uint4(p[0], p[CG], p[2 * CG], p[3 * CG]);
+      }
+      int q = 0;
+#pragma unroll
+      for (int I = 0; I < T; I++) {
+        const uint4 fh = make_uint4(hat(f[I].x), hat(f[I].y), hat(f[I].z), hat(f[I].w));
+#pragma unroll
+        for (int J = I; J < T; J++, q++) {
+          accR[q] = mfma<BF>(f[I], f[J], accR[q]);
+          accI[q] = mfma<BF>(fh, f[J], accI[q]);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);       // this wavefront's loads of the next stage have landed
+    __syncthreads();                     // everyone's have, and nobody still reads this stage
+  }
+
+  const int c = c0 + w;
+  if (c >= A.nchan) return;
+  int q = 0;
+#pragma unroll
+  for (int I = 0; I < T; I++)
+#pragma unroll
+    for (int J = I; J < T; J++, q++) {
+      const int b = 32 * J + row;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int a = 32 * I + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (a >= A.nin || b >= A.nin) continue;
+        float2* p = A.V + ((long long)a * A.nin + b) * A.nchan + c;
+        float2 v = make_float2(accR[q][r], accI[q][r]);
+        if (A.accumulate) { float2 o = *p; v.x += o.x; v.y += o.y; }
+        *p = v;
+      }
+    }
+}
+
 bool xmac_mfma_supported(int nin, int nchan) { return nin <= 64 && nchan % 16 == 0; }
+
+bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host) {
+  if (a.block_stride % 4) return false;
+  for (int i = 0; i < a.nin; i++)
+    if (in_off_host[i] % 4) return false;      // 16-byte aligned rows
+  return true;
+}
+
+hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream) {
+  if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(a.nchan / 16));
+  const bool bf = prec == FXG_XMAC_BF16;
+  const uint4* z = (const uint4*)zero16;
+  if (a.nin <= 32) {
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma_dl<1, true>), grid, dim3(1024), 0, stream, a, z);
+    else hipLaunchKernelGGL((k_xmac_mfma_dl<1, false>), grid, dim3(1024), 0, stream, a, z);
+  } else {
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma_dl<2, true>), grid, dim3(1024), 0, stream, a, z);
+    else hipLaunchKernelGGL((k_xmac_mfma_dl<2, false>), grid, dim3(1024), 0, stream, a, z);
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream) {
   if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;

@@ -305,6 +305,7 @@ def load_gpu_lib():
     lib.fxg_station_process_block.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int,
                                               C.c_int, C.c_int, C.c_void_p, P(BlockInfo)]
     lib.fxg_job_set_keep_taps.argtypes = [C.c_void_p, C.c_int]
+    lib.fxg_job_set_xmac_streaming.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]

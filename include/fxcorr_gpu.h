@@ -288,6 +288,11 @@ int fxg_job_set_profiling(fxg_job* job, int on);
  * reset != 0 clears the accumulators after reading. */
 int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
 
+/* Reduced-precision XMAC staging: 1 (default) streams the spectra from HBM
+ * into LDS with direct loads, double-buffered, where the layout allows;
+ * 0 (default) stages them through registers. */
+int fxg_job_set_xmac_streaming(fxg_job* job, int on);
+
 /* Debug taps valid after fxg_subint_process with keep_taps set. */
 int fxg_job_set_keep_taps(fxg_job* job, int keep);
 int fxg_tap_block_info(fxg_job* job, int datastream, int block, fxg_block_info* out);

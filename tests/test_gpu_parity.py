@@ -258,6 +258,32 @@ def test_visibilities_fp32_fringe_orders(gpu, oracle, mk, order):
     g.close()
 
 
+@pytest.mark.parametrize("nbits", [2, 8])
+@pytest.mark.parametrize("xmac", ["fp32", "fp16"])
+def test_visibilities_largest_fft(gpu, oracle, nbits, xmac):
+    """8192 channels: the 16384-point F stage (1024-thread workgroups; 8-bit
+    stream words sharing the FFT buffer)."""
+    lib, ctx = gpu
+    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=64.0, nchan=8192, blockspersend=5, nbits=nbits,
+                   xmac=xmac)
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [3]})
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    ref = oracle_results(oracle, job, sd, fp64=(xmac != "fp32"))
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    tol = FP32_TOL if xmac == "fp32" else MIXED_TOL[xmac]
+    assert rel_err(got[:xend], ref[:xend]) < tol
+    assert rel_err(got[a0:a1], ref[a0:a1]) < tol
+    if xmac == "fp32":
+        for ds in range(spec.nstations):
+            refs = oracle_spectra(oracle, job, sd, ds, 4, spec.nrecbands, spec.nchan)
+            for band in range(spec.nrecbands):
+                assert rel_err(g.spectrum(ds, band, 4, spec.nchan), refs[band]) < FP32_TOL
+    g.close()
+
+
 @pytest.mark.parametrize("avg,zoom", [(4, [(0, 37, 64)]), (256, [])])
 def test_visibilities_fp32_channel_averaging(gpu, oracle, avg, zoom):
     """Channel averaging of cross products (core.cpp:1788-1802) and of
@@ -278,6 +304,25 @@ def test_visibilities_fp32_channel_averaging(gpu, oracle, avg, zoom):
     assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL
     assert np.array_equal(got[xend:a0].view(np.float32), ref[xend:a0].view(np.float32))
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("mk", ["c1", "c2", "wide"])
+def test_xmac_streaming_equals_register_staging(gpu, prec, mk):
+    """The direct-to-LDS XMAC accumulates the same products in the same order
+    as the register-staged one: identical visibilities (block counts that
+    leave a partial stage, input counts that leave zero rows)."""
+    lib, ctx = gpu
+    spec = {"c1": c1_spec, "c2": c2_spec, "wide": wide_spec}[mk](xmac=prec, blockspersend=37)
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [1]})
+    outs = []
+    for on in (0, 1):
+        g = GpuJob(lib, ctx, job)
+        check(lib, lib.fxg_job_set_xmac_streaming(g.h, on), "streaming")
+        outs.append(g.process(sd))
+        g.close()
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
