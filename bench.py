@@ -196,6 +196,138 @@ def cpu_baseline(orc, sd_host, spec, threads, nblocks):
     return samples / el / 1e9, el, samples
 
 
+# The serving workload: three concurrent correlation jobs (stations,
+# channels) of 8 subbands x 64 MHz, 2-bit, streaming 0.125 s subints.
+SERVING_JOBS = [(10, 1024), (20, 4096), (32, 8192)]
+SERVING_SUBINT_S = 0.125
+
+
+def serving_bench(args):
+    """Poisson arrivals of 0.125 s subintegrations of three jobs at a multiple
+    of the real-time aggregate rate, through the request loop
+    (include/fxcorr_serve.h); p50/p99 latency (completion - submission) and
+    sustained input Gsamples/s."""
+    import threading
+    import torch
+    import fxcorr
+    from fxcorr import JobSpec, ServeStats, check
+    from fxsynth import make_subint_device, station_inputs_device
+    lib = fxcorr.load_gpu_lib()
+    host = fxcorr.load_host_lib()
+    srv = C.c_void_p()
+    check(host, host.fxs_create(0, 0, C.byref(srv)), "fxs_create")
+    cfgs = []
+    for i, (nst, nchan) in enumerate(SERVING_JOBS):
+        L = 2 * nchan
+        blocks = int(round(SERVING_SUBINT_S * 128e6 / L))
+        spec = JobSpec(nstations=nst, nsubbands=8, npol=2, bw_mhz=64.0, nchan=nchan, blockspersend=blocks,
+                       xmac=args.precision)
+        job, keep = spec.build()
+        cid = C.c_int()
+        check(host, host.fxs_add_config(srv, C.byref(job), args.slots, C.byref(cid)), "fxs_add_config")
+        sd, raws = make_subint_device(spec, seed=args.seed + i, rho=rhos_for(8, args.seed + i))
+        st, kst = station_inputs_device(sd, raws)
+        info = fxcorr.PlanInfo()
+        check(lib, lib.fxg_plan(C.byref(job), C.byref(info)), "plan")
+        ring = [torch.empty(2 * info.result_length, dtype=torch.float32, device="cuda:0") for _ in range(args.slots + 2)]
+        samples = nst * 16 * blocks * L
+        cfgs.append(dict(cid=cid.value, job=job, keep=keep, sd=sd, raws=raws, st=st, kst=kst, sub=sd.subint_desc(),
+                         ring=ring, samples=samples, nst=nst, nchan=nchan, blocks=blocks))
+        log(f"job {i}: {nst} stations, {nchan} ch, {blocks} blocks/subint, {samples / 1e9:.2f} Gsamples/subint")
+    torch.cuda.synchronize()
+    rate = args.load / SERVING_SUBINT_S            # arrivals per second per job
+    rng = np.random.default_rng(args.seed)
+    events = []
+    for i in range(len(cfgs)):
+        t = 0.0
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > args.duration:
+                break
+            events.append((t, i))
+    events.sort()
+    log(f"{len(events)} subints over {args.duration} s at {args.load}x real time")
+    # warm every configuration's first slot before the clock starts
+    for c in cfgs:
+        req = C.c_int64()
+        check(host, host.fxs_submit(srv, c["cid"], C.byref(c["sub"]), c["st"], C.c_void_p(c["ring"][0].data_ptr()),
+                                    C.byref(req)), "warm")
+        check(host, host.fxs_wait(srv, req, None), "warm wait")
+    records = []                                   # (job, request, ring slot)
+    lock = threading.Lock()
+    inflight = [dict() for _ in cfgs]              # ring index -> request of the job
+    done = []
+    cv = threading.Condition(lock)
+
+    def waiter():
+        while True:
+            with cv:
+                while not records:
+                    cv.wait()
+                item = records.pop(0)
+            if item is None:
+                return
+            i, req, k, t_sub = item
+            lat = C.c_int64()
+            check(host, host.fxs_wait(srv, req, C.byref(lat)), "fxs_wait")
+            with cv:
+                done.append((i, lat.value / 1e6, time.perf_counter()))
+                inflight[i].pop(k, None)
+                cv.notify_all()
+
+    th = threading.Thread(target=waiter, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    nxt = [0] * len(cfgs)
+    for t_arr, i in events:
+        now = time.perf_counter() - t0
+        if t_arr > now:
+            time.sleep(t_arr - now)
+        c = cfgs[i]
+        with cv:                                   # a result buffer of this job that is free
+            k = nxt[i] % len(c["ring"])
+            while k in inflight[i]:
+                cv.wait()
+            nxt[i] += 1
+        req = C.c_int64()
+        check(host, host.fxs_submit(srv, c["cid"], C.byref(c["sub"]), c["st"], C.c_void_p(c["ring"][k].data_ptr()),
+                                    C.byref(req)), "fxs_submit")
+        with cv:
+            inflight[i][k] = req.value
+            records.append((i, req.value, k, time.perf_counter()))
+            cv.notify_all()
+    with cv:
+        records.append(None)
+        cv.notify_all()
+    th.join()
+    t_end = max(d[2] for d in done)
+    el = t_end - t0
+    stats = ServeStats()
+    check(host, host.fxs_stats(srv, C.byref(stats)), "stats")
+    lat = np.array([d[1] for d in done])
+    per = {}
+    for i, c in enumerate(cfgs):
+        li = np.array([d[1] for d in done if d[0] == i])
+        per[f"{c['nst']}st_{c['nchan']}ch"] = {"subints": int(li.size), "p50_ms": float(np.percentile(li, 50)),
+                                             "p99_ms": float(np.percentile(li, 99))}
+    samples = sum(cfgs[d[0]]["samples"] for d in done)
+    offered = sum(c["samples"] for c in cfgs) / SERVING_SUBINT_S * args.load / 1e9
+    out = {
+        "metric": "serving: sustained input Gsamples/s and subint latency, 3 concurrent jobs",
+        "value": samples / el / 1e9, "unit": "Gsamples/s", "n_gpus": 1, "higher_is_better": True,
+        "offered_gsamples_per_s": offered, "load_vs_realtime": args.load, "duration_s": el,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "per_job": per, "dtype": args.precision, "data": "synthetic 2-bit VDIF generated on the GPU",
+        "config": {"workload": "3 jobs (10/20/32 stations; 1024/4096/8192 ch) x 8 subbands x 64 MHz, 2-bit, "
+                               "0.125 s subints, Poisson arrivals", "slots_per_job": args.slots},
+        "server": {"budget_gb": stats.budget_bytes / 1e9, "used_gb": stats.used_bytes / 1e9,
+                   "slots": stats.nslots, "refused_slots": stats.refused},
+    }
+    print(json.dumps(out), flush=True)
+    host.fxs_destroy(srv)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,9 +342,21 @@ def main():
     ap.add_argument("--chunk", type=int, default=2000,
                     help="FFT blocks of spectra resident in HBM at once (default: the whole subint)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--serving", action="store_true",
+                    help="the request-loop workload instead: 3 concurrent jobs, Poisson subint arrivals")
+    ap.add_argument("--load", type=float, default=1.5, help="serving: offered load as a multiple of real time")
+    ap.add_argument("--duration", type=float, default=8.0, help="serving: seconds of arrivals")
+    ap.add_argument("--slots", type=int, default=2, help="serving: concurrent slots per job")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.serving:
+        if args.gpus != 1:
+            print("bench.py runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
+            sys.exit(2)
+        serving_bench(args)
+        return
 
     if args.copy_calibration:
         import torch

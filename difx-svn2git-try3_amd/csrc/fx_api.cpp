@@ -80,6 +80,7 @@ struct fxg_job {
   std::vector<DBuf> d_raw;        // staged host data per datastream
   // big buffers
   DBuf d_spec, d_V;
+  DBuf d_z;                       // FFTs above 16384 points: the sub-FFT scratch
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
@@ -326,7 +327,7 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
-                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero};
+                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -350,10 +351,15 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     // device-side limits of this build
     std::string why;
     if (!P.uniform) why = "all datastreams of a job must share nchan";
-    else if (!(P.L >= 128 && P.L <= 16384 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 16384]";
-    for (const DevStation& st : P.st)
-      if (why.empty() && (st.order == 2 ? 3 : 2) * st.s + st.L / st.s + st.C / st.s > fxg_rotator_table_entries(st.L))
-        why = "array stride gives more rotator sub/step points than the F stage holds";
+    else if (!(P.L >= 128 && P.L <= 65536 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 65536]";
+    for (const DevStation& st : P.st) {
+      const int tabs = fxg_rotator_table_entries(st.L);
+      const int rotpts = (st.order == 2 ? 2 : 1) * st.s + st.L / st.s, fracpts = st.s + st.C / st.s;
+      // one workgroup holds both sets up to kFStageMaxL; above, the two kernels hold one each
+      const bool fits = st.L <= kFStageMaxL ? rotpts + fracpts <= tabs : (rotpts <= tabs && fracpts <= tabs);
+      if (why.empty() && !fits) why = "array stride gives more rotator sub/step points than the F stage holds";
+      if (why.empty() && st.L > kFStageMaxL && st.slog < 0) why = "FFTs above 16384 points need a power-of-two array stride";
+    }
     if (!why.empty()) { delete j; return set_err(FXG_EUNSUPPORTED, why); }
   }
   // own copies of the descriptor tables
@@ -397,6 +403,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     tw[64 + h] = make_float2((float)std::cos(ang), (float)std::sin(ang));
   }
   size_t specbytes = (size_t)P.nbands_total * chunk * P.C * (prec == FXG_XMAC_FP32 ? 8 : 4);
+  const size_t zbytes = P.L > kFStageMaxL ? (size_t)P.nbands_total * chunk * P.L * sizeof(float2) : 0;
   if ((rc = j->d_st.ensure(sizeof(DevStation) * P.st.size())) ||
       (rc = j->d_bands.ensure(sizeof(DevBand) * P.bands.size())) ||
       (rc = j->d_segs.ensure(sizeof(DevSeg) * std::max<size_t>(1, P.segs.size()))) ||
@@ -407,6 +414,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
       (rc = j->d_weights.ensure(sizeof(float) * (size_t)(P.nbl + P.nds))) ||
       (rc = j->d_results.ensure(sizeof(float) * 2 * (size_t)std::max<long long>(1, P.info.result_length))) ||
       (rc = j->d_spec.ensure(specbytes)) ||
+      (zbytes && (rc = j->d_z.ensure(zbytes))) ||
       (rc = j->d_V.ensure(sizeof(float2) * (size_t)std::max<long long>(1, P.vsize))))
     return fail(rc);
   if (hipMemcpy(j->d_st.p, P.st.data(), sizeof(DevStation) * P.st.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -565,6 +573,8 @@ static void fill_fstage_args(fxg_job* j, FStageArgs* fa) {
   fa->prec = j->desc.xmac_precision;
   fa->scale = j->scale;
   fa->nblocks_subint = j->desc.blockspersend;
+  fa->zbuf = j->d_z.p ? j->d_z.as<float2>() : nullptr;
+  fa->chunkn = j->plan.chunk_blocks;
 }
 
 // identity of a subintegration's inputs: the prepared state is reused while it matches

@@ -9,6 +9,9 @@ namespace fxg {
 // LDS entries for an F-stage workgroup's rotator tables (sub, step,
 // fractional sub and step, quadratic pieces) at FFT length L
 constexpr int fxg_rotator_table_entries(int L) { return L >= 16384 ? 768 : 512; }
+// the largest FFT length of the single-workgroup F stage; longer ones (to
+// 65536) take the two-kernel path
+constexpr int kFStageMaxL = 16384;
 
 struct FStageArgs {
   const DevStation* st;
@@ -25,6 +28,8 @@ struct FStageArgs {
   float scale;                // applied before reduced-precision stores
   int nblocks_subint;
   int chunk0;                 // first block of the chunk
+  float2* zbuf;               // FFTs above 16384 points: [band][chunk block][16][L/16] scratch
+  int chunkn;                 // blocks per chunk (the scratch's block dimension)
 };
 
 struct XmacArgs {

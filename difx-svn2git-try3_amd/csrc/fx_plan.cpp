@@ -423,6 +423,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
 
   // ---- spectra chunking: keep the spectra of at most ~16 GiB resident
   long long per_block = (long long)P.nbands_total * P.C * (d->xmac_precision == FXG_XMAC_FP32 ? 8 : 4);
+  if (P.L > 16384) per_block += (long long)P.nbands_total * P.L * 8;     // the sub-FFT scratch
   long long budget = 16LL << 30;
   int chunk = d->max_chunk_blocks > 0 ? d->max_chunk_blocks : (int)std::max(1LL, budget / per_block);
   chunk = std::min(chunk, d->blockspersend);
@@ -438,7 +439,12 @@ int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d) {
   long long blocks = (long long)p.nds * d->blockspersend * (long long)sizeof(DevBlock);
   long long res = p.info.result_length * 8;
   long long tables = (long long)p.segs.size() * sizeof(DevSeg) + p.bands.size() * sizeof(DevBand) + (1 << 20);
-  return spec + v + blocks + res + tables;
+  long long z = p.L > 16384 ? (long long)p.nbands_total * p.chunk_blocks * p.L * 8 : 0;
+  // corner-turned samples and the frame table (about the raw data of the subint)
+  long long ct = 0;
+  for (const DevStation& st : p.st)
+    ct += (long long)(d->blockspersend + 2) * st.L * st.nbits * st.nchan / 8 + st.framebytes * 4;
+  return spec + v + blocks + res + tables + z + ct + ct / 8;
 }
 
 }  // namespace fxg

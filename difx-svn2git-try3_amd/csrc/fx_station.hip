@@ -447,6 +447,83 @@ __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Rotator tables of one (band, block) at FFT length L, by the threads tid of
+// NT: FP64 phases, FP32 residual sin/cos (mode.cpp:909-1097).  rot: the pre-F
+// sub/step (and quadratic) rotators; frac: the fractional-sample sub/step.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStation& S, const DevBlock& G, int L,
+                                               int tid, int NT, bool rot, bool frac, float2* rsub,
+                                               float2* rstep, float2* fsub, float2* fstep, float2* rquad) {
+#pragma clang fp contract(off)
+  const int C = L / 2;
+  const int s = S.s;
+  const double lofreq = B.lofreq, looffset = B.looffset;
+  const bool looff = looffset > 0.0 || looffset < 0.0;
+  double fraclooffset = 0.0;
+  bool isfraclooffset = false;
+  if (looff) {
+    fraclooffset = fabs(looffset) - (int)fabs(looffset);
+    if (fraclooffset > kTiny) isfraclooffset = true;
+    if (looffset < 0) fraclooffset = -fraclooffset;
+  }
+  // pre-F rotator points: order 1 x = a k/L + b (mode.cpp:804-812), order 2
+  // x = (k/L)(b + a s/L) + c with quad points (k/L) 2a s/L and steps
+  // (js/L) b + (js/L)^2 a (mode.cpp:821-837)
+  const double stepxoff1 = (double)(1 * s) / (double)L;
+  for (int k = tid; rot && k < s && S.order != 0; k += NT) {
+    const double subxoff = (double)k / (double)L;
+    double ph = (S.order == 1 ? subxoff * G.a + G.b : subxoff * (G.b + G.a * stepxoff1) + G.c) * lofreq;
+    if (S.fractionalLo) ph += (lofreq - (int)lofreq) * (double)G.integerdelay;
+    if (looff) {
+      ph += ((k * S.sampletime) / 1e6) * (-looffset);
+      ph += -looffset * G.fracwall;
+      if (isfraclooffset) ph += -fraclooffset * G.intwall;
+    }
+    float arg = (float)(-kTwoPi * (ph - (int)ph));
+    float sn, cs; sincosf(arg, &sn, &cs);
+    rsub[k] = make_float2(cs, sn);
+    if (S.order == 2) {
+      const double qph = (subxoff * (2 * G.a * stepxoff1)) * lofreq;
+      arg = (float)(-kTwoPi * (qph - (int)qph));
+      sincosf(arg, &sn, &cs);
+      rquad[k] = make_float2(cs, sn);
+    }
+  }
+  for (int j = tid; rot && j < S.nfrsteps && S.order != 0; j += NT) {
+    const double stepxoff = ((double)(j * s)) / (double)L;
+    double ph = (S.order == 1 ? stepxoff * G.a : stepxoff * G.b + (stepxoff * stepxoff) * G.a) * lofreq;
+    if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
+    float arg = (float)(-kTwoPi * (ph - (int)ph));
+    float sn, cs; sincosf(arg, &sn, &cs);
+    rstep[j] = make_float2(cs, sn);
+  }
+  const double sign = B.useB ? -1.0 : 1.0;
+  const float val = (float)((double)G.fse - B.clock + sign * B.clockdelta / 2);
+  // order 0: the fringe phase of the block at its centre delay, added to
+  // the sub arguments (mode.cpp:1064-1073, 1111-1119)
+  float postf = 0.0f;
+  if (S.order == 0) {
+    double phaserotation = (G.c - G.integerdelay) * lofreq;
+    if (S.fractionalLo) phaserotation += G.integerdelay * (lofreq - (int)lofreq);
+    phaserotation -= G.wall * looffset;
+    postf = (float)(-kTwoPi * (phaserotation - (int)phaserotation));
+  }
+  for (int k = tid; frac && k < s; k += NT) {
+    float cf = (float)((kTwoPi * k * S.bw) / C);
+    float arg = cf * val;
+    if (S.order == 0) arg = arg + postf;
+    float sn, cs; sincosf(arg, &sn, &cs);
+    fsub[k] = make_float2(cs, sn);
+  }
+  for (int j = tid; frac && j < S.nfracsteps; j += NT) {
+    double dcf = B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
+    float cf = (float)dcf;
+    float sn, cs; sincosf(cf * val, &sn, &cs);
+    fstep[j] = make_float2(cs, sn);
+  }
+}
+
 __device__ __forceinline__ void put_spec(float2* o, int c, float2 y) { o[c] = y; }
 __device__ __forceinline__ void put_spec(__half2* o, int c, float2 y) { o[c] = __floats2half2_rn(y.x, y.y); }
 __device__ __forceinline__ void put_spec(__hip_bfloat162* o, int c, float2 y) { o[c] = __float22bfloat162_rn(y); }
@@ -543,73 +620,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
 
   // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
+  rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
   {
-#pragma clang fp contract(off)
-    const int s = S.s;
-    const double lofreq = B.lofreq, looffset = B.looffset;
-    const bool looff = looffset > 0.0 || looffset < 0.0;
-    double fraclooffset = 0.0;
-    bool isfraclooffset = false;
-    if (looff) {
-      fraclooffset = fabs(looffset) - (int)fabs(looffset);
-      if (fraclooffset > kTiny) isfraclooffset = true;
-      if (looffset < 0) fraclooffset = -fraclooffset;
-    }
-    // pre-F rotator points: order 1 x = a k/L + b (mode.cpp:804-812), order 2
-    // x = (k/L)(b + a s/L) + c with quad points (k/L) 2a s/L and steps
-    // (js/L) b + (js/L)^2 a (mode.cpp:821-837)
-    const double stepxoff1 = (double)(1 * s) / (double)L;
-    for (int k = tid; k < s && S.order != 0; k += NT) {
-      const double subxoff = (double)k / (double)L;
-      double ph = (S.order == 1 ? subxoff * G.a + G.b : subxoff * (G.b + G.a * stepxoff1) + G.c) * lofreq;
-      if (S.fractionalLo) ph += (lofreq - (int)lofreq) * (double)G.integerdelay;
-      if (looff) {
-        ph += ((k * S.sampletime) / 1e6) * (-looffset);
-        ph += -looffset * G.fracwall;
-        if (isfraclooffset) ph += -fraclooffset * G.intwall;
-      }
-      float arg = (float)(-kTwoPi * (ph - (int)ph));
-      float sn, cs; sincosf(arg, &sn, &cs);
-      rsub[k] = make_float2(cs, sn);
-      if (S.order == 2) {
-        const double qph = (subxoff * (2 * G.a * stepxoff1)) * lofreq;
-        arg = (float)(-kTwoPi * (qph - (int)qph));
-        sincosf(arg, &sn, &cs);
-        rquad[k] = make_float2(cs, sn);
-      }
-    }
-    for (int j = tid; j < S.nfrsteps && S.order != 0; j += NT) {
-      const double stepxoff = ((double)(j * s)) / (double)L;
-      double ph = (S.order == 1 ? stepxoff * G.a : stepxoff * G.b + (stepxoff * stepxoff) * G.a) * lofreq;
-      if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
-      float arg = (float)(-kTwoPi * (ph - (int)ph));
-      float sn, cs; sincosf(arg, &sn, &cs);
-      rstep[j] = make_float2(cs, sn);
-    }
-    const double sign = B.useB ? -1.0 : 1.0;
-    const float val = (float)((double)G.fse - B.clock + sign * B.clockdelta / 2);
-    // order 0: the fringe phase of the block at its centre delay, added to
-    // the sub arguments (mode.cpp:1064-1073, 1111-1119)
-    float postf = 0.0f;
-    if (S.order == 0) {
-      double phaserotation = (G.c - G.integerdelay) * lofreq;
-      if (S.fractionalLo) phaserotation += G.integerdelay * (lofreq - (int)lofreq);
-      phaserotation -= G.wall * looffset;
-      postf = (float)(-kTwoPi * (phaserotation - (int)phaserotation));
-    }
-    for (int k = tid; k < s; k += NT) {
-      float cf = (float)((kTwoPi * k * S.bw) / C);
-      float arg = cf * val;
-      if (S.order == 0) arg = arg + postf;
-      float sn, cs; sincosf(arg, &sn, &cs);
-      fsub[k] = make_float2(cs, sn);
-    }
-    for (int j = tid; j < S.nfracsteps; j += NT) {
-      double dcf = B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
-      float cf = (float)dcf;
-      float sn, cs; sincosf(cf * val, &sn, &cs);
-      fstep[j] = make_float2(cs, sn);
-    }
     if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
     else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];   // (S.lut[tid] would put S in scratch)
   }
@@ -796,10 +808,198 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// FFT lengths above 16384: L = 16 M and
+//     X[k2 + M k1] = sum_n1 W_16^(n1 k1) W_L^(n1 k2) Y_n1[k2],
+// Y_n1 the M-point FFT of x[n1 + 16 n2], n2 < M.  k_bigfft_a: one workgroup
+// per (block, band, n1) decodes and fringe-rotates its decimated samples,
+// transforms them in LDS and writes W_L^(n1 k2) Y_n1[k2] to the scratch
+// Z[band][block][n1][M]; k_bigfft_b: one thread per k2 does the 16-point DFT
+// over n1, keeps the sideband's eight outputs and applies the
+// fractional-sample rotation.
+// ---------------------------------------------------------------------------
+template <int M, int NBITS>
+__global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
+  using Sh = FftShape<M>;
+  constexpr int L = 16 * M;
+  constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
+  __shared__ float2 buf[Sh::PAD];
+  __shared__ float2 tabs[fxg_rotator_table_entries(L)];
+  __shared__ float2 twlo[64], twhi[L / 64];
+  __shared__ float lut8s[NBITS == 8 ? 256 : 4];
+
+  const int tid = threadIdx.x;
+  const int cb = blockIdx.x, n1 = blockIdx.z;
+  const int blk = A.chunk0 + cb;
+  const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
+  const DevBand B = A.bands[gband];
+  const DevStation S = A.st[B.station];
+  const DevStationSub Q = A.ss[B.station];
+  const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
+  if (G.status != 0) return;                     // k_bigfft_b writes the zero spectrum
+
+  float2* rsub = tabs;
+  float2* rstep = tabs + S.s;
+  float2* rquad = rstep + S.nfrsteps;
+  for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
+  for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
+  rotator_tables(B, S, G, L, tid, NT, true, false, rsub, rstep, nullptr, nullptr, rquad);
+  if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
+  else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
+  __syncthreads();
+
+  const float4 lut2 = make_float4(S.lut[0], S.lut[1], S.lut[2], S.lut[3]);
+  const int s = S.s, slog = S.slog;
+  const long long t0 = G.nearestsample;
+  const bool fast = G.allvalid != 0;
+  const unsigned* stream = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
+  const long long nwords = Q.ctstride >> 2;
+  const long long bit0 = t0 * NBITS;
+
+  // ---- pass 1 (Ns = 1) over n2 = j + r M/16: decode, rotate, radix-16 DFT
+  if (tid < NTF) {
+    const int j = tid;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int n = n1 + 16 * (j + r * NTF);       // sample of the block
+      float x;
+      if (fast) {
+        const long long bp = bit0 + (long long)NBITS * n;
+        const unsigned wv = (bp >> 5) < nwords ? stream[bp >> 5] : 0u;
+        x = lut8s[(wv >> (bp & 31)) & ((1u << NBITS) - 1u)];
+      } else {
+        x = sample_value(S, Q, A.ftab, NBITS == 8 ? lut8s : A.lut8, lut2, t0 + n, G.validend, B.band);
+      }
+      float2 rot = make_float2(1.0f, 0.0f);
+      if (S.order != 0) {
+        const int kk = slog >= 0 ? (n & (s - 1)) : n % s;
+        const int jj = slog >= 0 ? (n >> slog) : n / s;
+        if (S.order == 1) {
+          rot = n < s ? rsub[n] : cmulf(rsub[kk], rstep[jj]);
+        } else {
+          const float2 qd = rquad[kk];
+          float2 piece = rsub[kk];
+          for (int i = 0; i < jj; i++) piece = cmulf(piece, qd);
+          rot = cmulf(piece, rstep[jj]);
+        }
+      }
+      v[r] = S.order == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
+    }
+    DFT<16>::run(v);
+    float2* o = buf + 17 * j;
+#pragma unroll
+    for (int r = 0; r < 16; r++) o[r] = v[r];
+  }
+  __syncthreads();
+
+  // ---- middle radix-16 passes (W_M^m = W_L^(16 m))
+#pragma unroll
+  for (int p = 0; p < Sh::NMID; p++) {
+    const int Ns = 16 << (4 * p);
+    float2 v[16];
+    const int j = tid;
+    if (tid < NTF) {
+      const int k = j & (Ns - 1);
+      float2 w[16];
+      twiddles16(twlo, twhi, 16 * k * (M / (Ns * 16)), w);
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float2 x = buf[pidx(j + r * NTF)];
+        v[r] = r == 0 ? x : cmulf(x, w[r]);
+      }
+      DFT<16>::run(v);
+    }
+    __syncthreads();
+    if (tid < NTF) {
+      const int k = j & (Ns - 1);
+      const int idxD = (j / Ns) * Ns * 16 + k;
+#pragma unroll
+      for (int r = 0; r < 16; r++) buf[pidx(idxD + r * Ns)] = v[r];
+    }
+    __syncthreads();
+  }
+
+  // ---- last pass, every output, times W_L^(n1 k2), to the scratch
+  if (tid < NTF) {
+    const int j = tid;
+    float2* z = A.zbuf + (((long long)gband * A.chunkn + cb) * 16 + n1) * M;
+#pragma unroll
+    for (int u = 0; u < 16 / RL; u++) {
+      const int k = j + u * NTF;
+      float2 v[RL], w[RL];
+      if (RL == 16) twiddles16(twlo, twhi, 16 * k, w);
+      else twiddlesR<RL>(twlo, twhi, 16 * k, w);
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+        const float2 x = buf[pidx(k + NSL * r)];
+        v[r] = r == 0 ? x : cmulf(x, w[r]);
+      }
+      if (RL == 2) dft2(v[0], v[1]);
+      else DFT<RL>::run(v);
+#pragma unroll
+      for (int m = 0; m < RL; m++) {
+        const int k2 = k + NSL * m;
+        z[k2] = n1 == 0 ? v[m] : cmulf(v[m], tw_of(twlo, twhi, n1 * k2));
+      }
+    }
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
+  constexpr int L = 16 * M, C = L / 2;
+  __shared__ float2 tabs[fxg_rotator_table_entries(L)];
+  const int tid = threadIdx.x;
+  const int cb = blockIdx.x;
+  const int blk = A.chunk0 + cb;
+  const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
+  const DevBand B = A.bands[gband];
+  const DevStation S = A.st[B.station];
+  const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
+  const int k2 = blockIdx.z * 256 + tid;
+  const long long outbase = B.specoff + (long long)cb * C;
+  if (G.status != 0) {
+    for (int q = 0; q < 8; q++) {
+      if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + k2 + M * q] = make_float2(0.f, 0.f);
+      else ((unsigned*)A.spectra)[outbase + k2 + M * q] = 0u;
+    }
+    return;
+  }
+  float2* fsub = tabs;
+  float2* fstep = tabs + S.s;
+  rotator_tables(B, S, G, L, tid, 256, false, true, nullptr, nullptr, fsub, fstep, nullptr);
+  __syncthreads();
+  const int s = S.s, slog = S.slog;
+  const float2* z = A.zbuf + ((long long)gband * A.chunkn + cb) * 16 * M + k2;
+  float2 v[16];
+#pragma unroll
+  for (int n1 = 0; n1 < 16; n1++) v[n1] = z[(long long)n1 * M];
+  DFT<16>::run(v);
+  const int mh = B.lsb ? 8 : 0;          // USB keeps k1 < 8, real LSB k1 >= 8
+  const float scale = A.prec == FXG_XMAC_FP32 ? 1.0f : A.scale;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int c = k2 + M * q;
+    const float2 x = v[mh + q];
+    const float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
+    const float2 y = make_float2((x.x * fr.x - x.y * fr.y) * scale, (x.x * fr.y + x.y * fr.x) * scale);
+    if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + c] = y;
+    else if (A.prec == FXG_XMAC_FP16) ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x, y.y);
+    else ((__hip_bfloat162*)A.spectra)[outbase + c] = __float22bfloat162_rn(y);
+  }
+}
+
 #define INSTANTIATE(Lv) template __global__ void k_fstage<Lv, 2>(FStageArgs); \
                         template __global__ void k_fstage<Lv, 8>(FStageArgs);
 INSTANTIATE(128) INSTANTIATE(256) INSTANTIATE(512) INSTANTIATE(1024)
 INSTANTIATE(2048) INSTANTIATE(4096) INSTANTIATE(8192) INSTANTIATE(16384)
+template __global__ void k_bigfft_a<2048, 2>(FStageArgs);
+template __global__ void k_bigfft_a<2048, 8>(FStageArgs);
+template __global__ void k_bigfft_a<4096, 2>(FStageArgs);
+template __global__ void k_bigfft_a<4096, 8>(FStageArgs);
+template __global__ void k_bigfft_b<2048>(FStageArgs);
+template __global__ void k_bigfft_b<4096>(FStageArgs);
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -829,6 +1029,15 @@ hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int 
       break;
     CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192) CASE(16384)
 #undef CASE
+#define BIG(Mv) case 16 * Mv: { \
+      if (!a.zbuf) return hipErrorInvalidValue; \
+      dim3 ga(nblocks, nbands, 16), gb(nblocks, nbands, Mv / 256); \
+      if (nbits == 2) hipLaunchKernelGGL((k_bigfft_a<Mv, 2>), ga, dim3(FftShape<Mv>::NT), 0, stream, a); \
+      else hipLaunchKernelGGL((k_bigfft_a<Mv, 8>), ga, dim3(FftShape<Mv>::NT), 0, stream, a); \
+      hipLaunchKernelGGL((k_bigfft_b<Mv>), gb, dim3(256), 0, stream, a); \
+      break; }
+    BIG(2048) BIG(4096)
+#undef BIG
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -258,13 +258,15 @@ def test_visibilities_fp32_fringe_orders(gpu, oracle, mk, order):
     g.close()
 
 
+@pytest.mark.parametrize("nchan", [8192, 16384, 32768])
 @pytest.mark.parametrize("nbits", [2, 8])
 @pytest.mark.parametrize("xmac", ["fp32", "fp16"])
-def test_visibilities_largest_fft(gpu, oracle, nbits, xmac):
+def test_visibilities_largest_fft(gpu, oracle, nchan, nbits, xmac):
     """8192 channels: the 16384-point F stage (1024-thread workgroups; 8-bit
-    stream words sharing the FFT buffer)."""
+    stream words sharing the FFT buffer); 16384 and 32768 channels: the
+    two-kernel 32768- and 65536-point transforms through the scratch."""
     lib, ctx = gpu
-    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=64.0, nchan=8192, blockspersend=5, nbits=nbits,
+    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=64.0, nchan=nchan, blockspersend=5, nbits=nbits,
                    xmac=xmac)
     job, keep = spec.build()
     sd = make_data(spec, invalid_frames={0: [3]})
