@@ -315,3 +315,29 @@ def station_inputs_device(sd: SubintData, raws):
         for k in range(3):
             a.interpolator[k] = float(st.interpolator[k])
     return arr, keep
+
+
+def uneven_workload_specs(subint_s: float = 0.128, xmac: str = "fp16", max_blocks: Optional[int] = None):
+    """Six jobs of uneven size (stations 4..30, channels 64..32768, 4..16
+    subbands of 16..128 MHz, 2- and 8-bit, two zoom bands of 1/16 of a
+    subband each): (spec, label) pairs, blockspersend covering subint_s."""
+    rows = [  # stations, channels, subbands, MHz, bits
+        (4, 64, 4, 16.0, 2),
+        (7, 256, 8, 32.0, 8),
+        (12, 1024, 16, 64.0, 2),
+        (18, 4096, 8, 128.0, 2),
+        (25, 16384, 4, 64.0, 8),
+        (30, 32768, 4, 128.0, 2),
+    ]
+    out = []
+    for nst, nchan, nsb, bw, bits in rows:
+        L = 2 * nchan
+        blocks = max(1, int(round(subint_s * 2e6 * bw / L)))
+        if max_blocks:
+            blocks = min(blocks, max_blocks)
+        zw = nchan // 16
+        zoom = [(0, nchan // 4, zw), (nsb - 1, nchan // 2 + zw // 2, zw)]
+        spec = JobSpec(nstations=nst, nsubbands=nsb, npol=2, bw_mhz=bw, nchan=nchan, nbits=bits,
+                       blockspersend=blocks, xmac=xmac, zoom=zoom)
+        out.append((spec, f"{nst}st_{nchan}ch_{nsb}x{int(bw)}MHz_{bits}bit"))
+    return out

@@ -343,3 +343,36 @@ def test_visibilities_mixed_vs_fp64(gpu, oracle, prec, mk):
     assert e < MIXED_TOL[prec], e
     assert rel_err(got[a0:a1], ref[a0:a1]) < MIXED_TOL[prec]
     g.close()
+
+
+def _uneven_cases():
+    from fxsynth import uneven_workload_specs
+    out = []
+    for spec, label in uneven_workload_specs(xmac="fp32", max_blocks=3):
+        if spec.nchan >= 4096:          # the oracle's cost grows with baselines x channels
+            spec.nstations = min(spec.nstations, 6)
+        out.append(pytest.param(spec, id=label))
+    return out
+
+
+@pytest.mark.parametrize("spec", _uneven_cases())
+def test_uneven_workload_jobs(gpu, oracle, spec):
+    """The six uneven jobs of the batched workload (64..32768 channels, 2-
+    and 8-bit, 4..16 subbands, two zoom bands each), FP32 path vs the oracle
+    and fp16 vs the FP64 oracle."""
+    import dataclasses
+    lib, ctx = gpu
+    sd = make_data(spec, invalid_frames={1: [2]})
+    for xmac in ("fp32", "fp16"):
+        sp = dataclasses.replace(spec, xmac=xmac)
+        job, keep = sp.build()
+        g = GpuJob(lib, ctx, job)
+        got = g.process(sd)
+        ref = oracle_results(oracle, job, sd, fp64=(xmac != "fp32"), nthreads=8)
+        info = plan_info(oracle, "orc_", job)
+        assert g.info.result_length == info.result_length
+        xend, a0, a1 = region_slices(oracle, job, info)
+        tol = FP32_TOL if xmac == "fp32" else MIXED_TOL[xmac]
+        assert rel_err(got[:xend], ref[:xend]) < tol, (xmac, rel_err(got[:xend], ref[:xend]))
+        assert rel_err(got[a0:a1], ref[a0:a1]) < tol
+        g.close()
