@@ -81,6 +81,7 @@ struct fxg_job {
   // big buffers
   DBuf d_spec, d_V;
   DBuf d_z;                       // FFTs above 16384 points: the sub-FFT scratch
+  DBuf d_pcal, d_pcalaux, d_pcaltone, d_pcalpart;   // phase-cal tables and partial sums
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
@@ -179,6 +180,21 @@ int fxg_plan_acweight_offset(const fxg_job_desc* desc, int ds, int64_t* off) {
   Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
   if (!off || ds < 0 || ds >= p.nds) return set_err(FXG_EINVAL, "index");
   *off = p.acwoff[ds];
+  return 0;
+}
+int fxg_plan_pcal_offset(const fxg_job_desc* desc, int ds, int64_t* off) {
+  Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
+  if (!off || ds < 0 || ds >= p.nds) return set_err(FXG_EINVAL, "index");
+  *off = p.pcaloff[ds];
+  return 0;
+}
+int fxg_plan_pcal_tones(const fxg_job_desc* desc, int ds, int recfreq, int32_t* ntones, int32_t* offset_hz) {
+  Plan p; int rc = plan_lookup(desc, &p); if (rc) return rc;
+  if (!ntones || !offset_hz || ds < 0 || ds >= p.nds) return set_err(FXG_EINVAL, "index");
+  if (p.pcal_ntones[ds].empty()) { *ntones = 0; *offset_hz = -1; return 0; }
+  if (recfreq < 0 || recfreq >= (int)p.pcal_ntones[ds].size()) return set_err(FXG_EINVAL, "index");
+  *ntones = p.pcal_ntones[ds][recfreq];
+  *offset_hz = p.pcal_offhz[ds][recfreq];
   return 0;
 }
 
@@ -327,7 +343,8 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
-                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z};
+                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z,
+                  &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -439,6 +456,18 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     return fail(set_err(FXG_EHIP, "band index upload failed"));
   if ((rc = j->d_zero.ensure(64))) return fail(rc);
   if (hipMemset(j->d_zero.p, 0, 64) != hipSuccess) return fail(set_err(FXG_EHIP, "zero row"));
+  if (!P.pcal.empty()) {
+    if ((rc = j->d_pcal.ensure(sizeof(DevPcal) * P.pcal.size())) ||
+        (rc = j->d_pcalaux.ensure(sizeof(float) * P.pcal_aux.size())) ||
+        (rc = j->d_pcaltone.ensure(sizeof(int) * std::max<size_t>(1, P.pcal_tonebin.size()))) ||
+        (rc = j->d_pcalpart.ensure(sizeof(float) * (size_t)P.pcal_partfloats)))
+      return fail(rc);
+    if (hipMemcpy(j->d_pcal.p, P.pcal.data(), sizeof(DevPcal) * P.pcal.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(j->d_pcalaux.p, P.pcal_aux.data(), sizeof(float) * P.pcal_aux.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        (P.pcal_tonebin.size() && hipMemcpy(j->d_pcaltone.p, P.pcal_tonebin.data(), sizeof(int) * P.pcal_tonebin.size(),
+                                            hipMemcpyHostToDevice) != hipSuccess))
+      return fail(set_err(FXG_EHIP, "phase-cal table upload failed"));
+  }
   j->d_raw.resize(P.nds);
   *out = j;
   return 0;
@@ -531,6 +560,7 @@ static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
       if (W) maxsamples = std::max(maxsamples, q.nframes * (S.databytes / (4 * W)));
     }
     q.datasec = in.sec; q.datans = in.ns;
+    q.datasamples = static_cast<int>(in.ns / (P.st[d].sampletime * 1e3) + 0.5);   // mode.cpp:1572
     // Mode::setOffsets / setData guards (mode.cpp:1525-1573, 671)
     q.usable = (sub->model_ok && in.sec > FXG_INVALID_SUBINT && in.scan == sub->scan && in.nbytes > 1) ? 1 : 0;
     q.nflags = in.validflags ? in.nvalidflags : 0;
@@ -661,6 +691,17 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
   HIPCHK(launch_scatter(j->d_segs.as<DevSeg>(), (int)P.segs.size(), P.maxseglen, j->d_V.as<float2>(),
                         j->d_weights.as<float>(), vscale, dres, s));
   if ((rc = j->ev_end())) return rc;
+  if (!P.pcal.empty()) {
+    PcalArgs pa{};
+    pa.st = j->d_st.as<DevStation>(); pa.bands = j->d_bands.as<DevBand>(); pa.ss = j->d_ss.as<DevStationSub>();
+    pa.blocks = j->d_blocks.as<DevBlock>(); pa.ftab = j->d_ftab.as<int2>(); pa.lut8 = j->ctx->lut8.as<float>();
+    pa.ct = j->d_ct.as<unsigned char>(); pa.pc = j->d_pcal.as<DevPcal>(); pa.aux = j->d_pcalaux.as<float2>();
+    pa.tonebin = j->d_pcaltone.as<int>(); pa.part = j->d_pcalpart.as<float>(); pa.res = dres;
+    pa.nblocks = nblocks; pa.ngroups = P.pcal_groups;
+    if ((rc = j->ev_begin(FXG_STAGE_PCAL))) return rc;
+    HIPCHK(launch_pcal(pa, (int)P.pcal.size(), s));
+    if ((rc = j->ev_end())) return rc;
+  }
   if (!results_on_device) {
     if ((rc = j->ev_begin(FXG_STAGE_D2H))) return rc;
     HIPCHK(hipMemcpyAsync(results, dres, resbytes, hipMemcpyDeviceToHost, s));

@@ -45,7 +45,7 @@ struct DevStationSub {
   long long fbase;      // offset of this station in the frame table
   int datasec, datans, usable, nflags;
   int flagoff;          // offset of this station's valid flags
-  int pad;
+  int datasamples;      // samples from the second boundary to the data start (mode.cpp:1572)
   long long ctbase;     // byte offset of band 0's stream in the corner-turn buffer
   long long ctstride;   // bytes between consecutive bands' streams (multiple of 4)
   double interp[3];
@@ -76,6 +76,24 @@ struct DevSeg {
   int kind, nchan;
   int conj;             // take conj(V[b][a]) instead of V[a][b]
   int avg;              // channels averaged into one output channel (1: copy)
+};
+
+// ---- phase-cal extraction of one recorded band ----------------------------
+// Samples of the block's L-sample window are summed into nbins bins by their
+// absolute sample number (since the second boundary) modulo nbins, either as
+// they are or (cplx) counter-rotated by the first tone's offset; the tones
+// are DFT bins of the sums (pcal.cpp:388-410, 590-650, 1037-1062).
+struct DevPcal {
+  int gband;            // global recorded band
+  int nbins;            // accumulation bins N
+  int cplx;             // counter-rotated complex bins (the shifting extractor)
+  int period;           // rotator period in samples (cplx)
+  int ntones;
+  int toneoff;          // first entry of the band's tone-bin table (-1 entries give 0)
+  long long rotoff;     // float2 offset of the rotator table in the aux buffer (cplx)
+  long long twoff;      // float2 offset of e^{-2 pi i m / N}, m < N, in the aux buffer
+  long long partoff;    // float offset of the band's partial sums [group][N (x2 if cplx)]
+  long long resoff;     // cf32 offset of the band's tones in the result
 };
 
 // ---- the XMAC description of one frequency --------------------------------
@@ -110,7 +128,20 @@ struct Plan {
   int maxseglen = 0;
   // weight table: [0, nbl) baseline weights, [nbl, nbl+nds) AC weights
   int chunk_blocks = 0;
+  // phase cal: result offsets per datastream, tones and first-tone offsets
+  // per (datastream, recorded freq) (empty: no extraction), the device tables
+  std::vector<long long> pcaloff;
+  std::vector<std::vector<int>> pcal_ntones, pcal_offhz;
+  std::vector<DevPcal> pcal;
+  std::vector<int> pcal_tonebin;
+  std::vector<float> pcal_aux;                         // float2 pairs
+  int pcal_groups = 0;                                 // block groups per band
+  long long pcal_partfloats = 0;
+  int pcal_maxtones = 0;
 };
+
+// the largest accumulation of one band's phase-cal bins (LDS floats)
+constexpr int kPcalMaxFloats = 32768;
 
 int build_plan(const fxg_job_desc* d, Plan* p, std::string* err);
 int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d);

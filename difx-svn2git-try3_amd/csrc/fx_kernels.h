@@ -1,5 +1,6 @@
 // fx_kernels.h — kernel argument blocks and launchers of libfxcorr_gpu
-// (implemented in fx_station.hip and fx_xmac.hip).
+// (implemented in fx_station.hip, fx_xmac.hip, fx_xmac_mfma.hip and
+// fx_pcal.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fx_internal.h"
@@ -42,6 +43,23 @@ struct XmacArgs {
   int accumulate;
 };
 
+struct PcalArgs {
+  const DevStation* st;
+  const DevBand* bands;
+  const DevStationSub* ss;
+  const DevBlock* blocks;
+  const int2* ftab;
+  const float* lut8;
+  const unsigned char* ct;    // corner-turned per-band sample streams
+  const DevPcal* pc;          // [bands with phase cal]
+  const float2* aux;          // mixer and DFT tables
+  const int* tonebin;         // DFT bin per tone, -1 = zero
+  float* part;                // partial sums [band][group][bins]
+  float* res;                 // the result buffer as floats
+  int nblocks;                // blocks per subintegration
+  int ngroups;                // block groups per band
+};
+
 hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
                          long long maxframes, hipStream_t stream);
 hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,
@@ -61,6 +79,7 @@ bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host);
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream);
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream);
+hipError_t launch_pcal(const PcalArgs& a, int npcal, hipStream_t stream);
 hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w,
                           float vscale, float* res, hipStream_t stream);
 

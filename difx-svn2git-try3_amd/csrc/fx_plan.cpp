@@ -8,6 +8,7 @@
 //   baseline weights    [freq][baseline] ceil(npol/2) cf32 slots of floats
 //   autocorrelations    [datastream][pass][total band][chan]  (cf32)
 //   autocorr weights    [datastream] ceil(nbands*pass/2) cf32 slots of floats
+//   phase cal           [datastream][recorded band][tone]            (cf32)
 // Used frequencies are those named by any baseline; a frequency that is not
 // used but matches a used one (configuration.cpp:345-367) still gets its
 // autocorrelations.
@@ -61,6 +62,109 @@ int choff_of_band(const fxg_datastream& ds, int k) {
 }
 
 bool power_of_two(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// ---- phase cal ------------------------------------------------------------
+// gcd of two values rounded to integers (mathutil.cpp:8-30)
+long long gcd_ll(long long a, long long b) {
+  if (a == 0 || b == 0) return std::max(a, b);
+  for (;;) {
+    a %= b;
+    if (a == 0) return b;
+    b %= a;
+    if (b == 0) return a;
+  }
+}
+long long gcd_d(double a, double b) { return gcd_ll((long long)std::floor(a + 0.5), (long long)std::floor(b + 0.5)); }
+
+// tones from `offset` every `step` up to and including `bw`, a tone on both
+// band edges counted once (PCal::calcNumTones, pcal.cpp:212-229)
+int count_tones(double bw, double offset, double step) {
+  int n = 0;
+  if (bw <= 0.0 || step <= 0.0) return 0;
+  double f = offset;
+  while (f <= bw) { n++; f += step; }
+  if (offset == 0.0 && f - step == bw) n--;
+  return n;
+}
+
+// tones of one recorded frequency and the first tone's distance from the band
+// edge in Hz, -1 when no tone falls inside (configuration.cpp:1771-1845)
+void config_tones(const fxg_freq& fq, double interval, double base, int* ntones, int* offhz) {
+  const double lo = fq.freq_mhz, bw = fq.bw_mhz;
+  int n = 0;
+  double tone = double(int(lo / interval)) * interval;
+  double first;
+  if (fq.lsb) {
+    if (tone == lo) tone -= interval;
+    if (tone >= lo - bw)
+      while (tone - n * interval >= lo - bw) n++;
+    first = 1e6 * tone - 1e6 * base;
+    if (n > 0) *offhz = (int)(long)(1e6 * lo - first + 0.5);
+  } else {
+    if (tone <= lo) tone += interval;
+    if (tone <= lo + bw)
+      while (tone + n * interval <= lo + bw) n++;
+    first = 1e6 * tone + 1e6 * base;
+    if (n > 0) *offhz = (int)(long)(first - 1e6 * lo + 0.5);
+  }
+  if (n == 0) { n = 1; *offhz = -1; }
+  *ntones = n;
+}
+
+// The extractor PCal::getNew picks for real sampling with the 1 MHz minimum
+// resolution Mode sets (mode.cpp:412-428; pcal.cpp:139-203), its bins and the
+// DFT bin of each tone (-1: output zero).  kind 0: no tone (zeros), 1: first
+// tone at DC (pcal.cpp:333-350, 448-470), 2: implicit shift (:927-960,
+// 1064-1104), 3: shifting (:498-540, 652-690).
+struct PcalShape {
+  int kind = 0;
+  long long nbins = 0, period = 0;
+  int ntones = 0;
+  double dphi = 0.0;
+  std::vector<int> tonebin;
+};
+
+PcalShape pcal_shape(double bw_hz, double spacing_hz, int offset_hz) {
+  PcalShape p;
+  const double fs = 2.0 * bw_hz, minres = 1e6;
+  if (offset_hz == -1) return p;
+  long long No = (long long)(fs / gcd_d(fs, (double)offset_hz));
+  const long long Np = (long long)(fs / gcd_d(fs, spacing_hz));
+  const int Nt = count_tones(bw_hz, (double)offset_hz, spacing_hz);
+  while (fs / (2 * No) > minres) No *= 2;
+  if (Nt == 0) return p;
+  if (offset_hz == 0) {
+    p.kind = 1;
+    p.nbins = (long long)(fs / gcd_d(fs, spacing_hz));
+    p.ntones = count_tones(bw_hz, 0.0, spacing_hz) - 1;
+    const long long step = (long long)(spacing_hz / gcd_d(fs, spacing_hz));
+    for (int i = 0; i < p.ntones; i++) p.tonebin.push_back((int)((i + 1) * step));
+  } else if (No % Np == 0) {
+    p.kind = 2;
+    p.nbins = (long long)(fs / gcd_d(fs, offset_hz));
+    while (fs / (2 * p.nbins) > minres) p.nbins *= 2;
+    p.ntones = Nt;
+    const long long step = (long long)std::floor(double(p.nbins) * (spacing_hz / fs));
+    const long long off = (long long)std::floor(double(p.nbins) * (offset_hz / fs));
+    for (int i = 0; i < p.ntones; i++) {
+      long long idx = off + i * step;
+      p.tonebin.push_back(idx >= p.nbins ? -1 : (int)idx);
+    }
+  } else {
+    p.kind = 3;
+    p.nbins = (long long)(fs / gcd_d(fs, spacing_hz));
+    p.ntones = Nt;
+    p.period = (long long)(fs / gcd_d(fs, offset_hz));
+    p.dphi = kTwoPi * (-offset_hz / fs);
+    const long long step = (long long)std::floor(p.nbins * (spacing_hz / fs));
+    for (int i = 0; i < p.ntones; i++) {
+      long long idx = i * step;
+      // the reference stops copying here and leaves the rest untouched; zero
+      p.tonebin.push_back(idx >= p.nbins ? -1 : (int)idx);
+    }
+  }
+  return p;
+}
 
 #define REQUIRE(cond, code, msg) do { if (!(cond)) { *err = (msg); return (code); } } while (0)
 
@@ -222,7 +326,21 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       if (ac_wanted(freq_of_band(ds, k))) n += P.bandsperac;
     at += std::max(1, (n + 1) / 2);
   }
-  // phase-cal results: none (no phase-cal extraction in this build)
+  // phase-cal tones per (datastream, recorded freq) and their result region
+  P.pcaloff.assign(d->ndatastreams, 0);
+  P.pcal_ntones.assign(d->ndatastreams, {});
+  P.pcal_offhz.assign(d->ndatastreams, {});
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    P.pcaloff[s] = at;
+    if (!(ds.pcal_interval_mhz > 0)) continue;
+    P.pcal_ntones[s].resize(ds.nrecfreqs);
+    P.pcal_offhz[s].resize(ds.nrecfreqs);
+    for (int i = 0; i < ds.nrecfreqs; i++)
+      config_tones(d->freqs[ds.recfreq_index[i]], ds.pcal_interval_mhz, ds.pcal_base_mhz,
+                   &P.pcal_ntones[s][i], &P.pcal_offhz[s][i]);
+    for (int k = 0; k < ds.nrecbands; k++) at += P.pcal_ntones[s][ds.recband_freq[k]];
+  }
 
   // ---- per-datastream device constants; one FFT length per job
   P.C = d->freqs[d->datastreams[0].recfreq_index[0]].nchan;
@@ -285,6 +403,68 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     }
   }
   P.nbands_total = band;
+
+  // ---- phase-cal device tables
+  for (int s = 0; s < d->ndatastreams; s++) {
+    const fxg_datastream& ds = d->datastreams[s];
+    if (P.pcal_ntones[s].empty()) continue;
+    const double bwhz = 1e6 * d->freqs[ds.recfreq_index[0]].bw_mhz;
+    long long at3 = P.pcaloff[s];
+    for (int k = 0; k < ds.nrecbands; k++) {
+      const int lf = ds.recband_freq[k];
+      const int nt = P.pcal_ntones[s][lf];
+      REQUIRE(P.pcal_offhz[s][lf] >= -1, FXG_EUNSUPPORTED,
+              "phase cal: the tone base puts the first tone outside the band");
+      PcalShape sh = pcal_shape(bwhz, 1e6 * ds.pcal_interval_mhz, P.pcal_offhz[s][lf]);
+      const long long resoff = at3;
+      at3 += nt;
+      if (sh.kind == 0) continue;                       // no tone in the band: zeros
+      REQUIRE(sh.ntones == nt, FXG_EUNSUPPORTED,
+              "phase cal: the extractor's tone count differs from the configuration's");
+      REQUIRE(std::fabs(bwhz * 2 - std::floor(bwhz * 2 + 0.5)) < 1e-6, FXG_EUNSUPPORTED,
+              "phase cal needs an integral sample rate in Hz");
+      REQUIRE(sh.nbins * (sh.kind == 3 ? 2 : 1) <= kPcalMaxFloats, FXG_EUNSUPPORTED,
+              "phase cal: too many accumulation bins for one workgroup");
+      REQUIRE(sh.period <= (1 << 22), FXG_EUNSUPPORTED, "phase cal: rotator period too long");
+      DevPcal pc{};
+      pc.gband = P.st[s].band0 + k;
+      pc.nbins = (int)sh.nbins;
+      pc.cplx = sh.kind == 3;
+      pc.period = (int)sh.period;
+      pc.ntones = nt;
+      pc.toneoff = (int)P.pcal_tonebin.size();
+      P.pcal_tonebin.insert(P.pcal_tonebin.end(), sh.tonebin.begin(), sh.tonebin.end());
+      pc.resoff = resoff;
+      if (pc.cplx) {
+        // the shifting extractor's mixer, (f32) of the FP64 value (pcal.cpp:530-535)
+        pc.rotoff = (long long)P.pcal_aux.size() / 2;
+        for (long long m = 0; m < sh.period; m++) {
+          const double arg = sh.dphi * double(m);
+          P.pcal_aux.push_back((float)std::cos(arg));
+          P.pcal_aux.push_back((float)std::sin(arg));
+        }
+      }
+      pc.twoff = (long long)P.pcal_aux.size() / 2;
+      for (long long m = 0; m < sh.nbins; m++) {
+        const double arg = -kTwoPi * double(m) / double(sh.nbins);
+        P.pcal_aux.push_back((float)std::cos(arg));
+        P.pcal_aux.push_back((float)std::sin(arg));
+      }
+      P.pcal_maxtones = std::max(P.pcal_maxtones, nt);
+      P.pcal.push_back(pc);
+    }
+  }
+  if (!P.pcal.empty()) {
+    // enough (band, block group) workgroups to cover the chip
+    const int nb = (int)P.pcal.size();
+    P.pcal_groups = std::max(1, std::min(d->blockspersend, (1024 + nb - 1) / nb));
+    long long part = 0;
+    for (DevPcal& pc : P.pcal) {
+      pc.partoff = part;
+      part += (long long)P.pcal_groups * pc.nbins * (pc.cplx ? 2 : 1);
+    }
+    P.pcal_partfloats = part;
+  }
 
   // ---- XMAC frequencies: every frequency whose products or autocorrelations
   //      are output (zoom autocorrelations read their parent's accumulation)
@@ -440,11 +620,13 @@ int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d) {
   long long res = p.info.result_length * 8;
   long long tables = (long long)p.segs.size() * sizeof(DevSeg) + p.bands.size() * sizeof(DevBand) + (1 << 20);
   long long z = p.L > 16384 ? (long long)p.nbands_total * p.chunk_blocks * p.L * 8 : 0;
+  long long pcal = p.pcal_partfloats * 4 + (long long)p.pcal_aux.size() * 4 +
+                   (long long)p.pcal.size() * sizeof(DevPcal) + (long long)p.pcal_tonebin.size() * 4;
   // corner-turned samples and the frame table (about the raw data of the subint)
   long long ct = 0;
   for (const DevStation& st : p.st)
     ct += (long long)(d->blockspersend + 2) * st.L * st.nbits * st.nchan / 8 + st.framebytes * 4;
-  return spec + v + blocks + res + tables + z + ct + ct / 8;
+  return spec + v + blocks + res + tables + z + pcal + ct + ct / 8;
 }
 
 }  // namespace fxg

@@ -12,6 +12,7 @@
 #include <hip/hip_bf16.h>
 #include "fx_kernels.h"
 #include "fx_bits.h"
+#include "fx_device.h"
 #include <type_traits>
 
 namespace fxg {
@@ -31,10 +32,6 @@ __device__ __forceinline__ unsigned ld32(const unsigned char* p) {
 
 constexpr unsigned long long kFill64 = 0x1122334411223344ULL;  // fill pattern, little endian
 constexpr unsigned kFill32 = 0x11223344U;
-
-__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 // ---------------------------------------------------------------------------
 // k_frames: one thread per frame -> (startvalid, endvalid) payload byte range
@@ -201,31 +198,6 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
 // ---------------------------------------------------------------------------
 // sample decoding
 // ---------------------------------------------------------------------------
-// 2-bit level lookup by selects (a dynamically indexed local array would live
-// in scratch memory)
-__device__ __forceinline__ float pick4(const float4& l, unsigned c) {
-  const float lo = (c & 1u) ? l.y : l.x;
-  const float hi = (c & 1u) ? l.w : l.z;
-  return (c & 2u) ? hi : lo;
-}
-
-__device__ __forceinline__ float sample_value(const DevStation& S, const DevStationSub& Q,
-                                              const int2* __restrict__ ftab, const float* lut8,
-                                              const float4& lut2, long long t, long long validend, int band) {
-  if (t >= validend || t < 0) return 0.0f;
-  long long f = t / S.framesamples;
-  if (f >= Q.nframes) return 0.0f;
-  int tin = (int)(t - f * S.framesamples);
-  int grp = tin / S.groupsamples;
-  int ib = grp * S.groupbytes;
-  int2 z = ftab[Q.fbase + f];
-  if (ib < z.x || ib >= z.y) return 0.0f;
-  int bit = ((tin - grp * S.groupsamples) * S.nchan + band) * S.nbits;
-  unsigned char byte = Q.data[f * S.framebytes + S.headerbytes + ib + (bit >> 3)];
-  if (S.nbits == 8) return lut8[byte];
-  return pick4(lut2, (byte >> (bit & 7)) & 3u);
-}
-
 // standalone decode of nsamp sample-times from `first` into out[nchan][nsamp]
 __global__ void k_unpack(const DevStation* __restrict__ st, const DevStationSub* __restrict__ ss,
                          const int2* __restrict__ ftab, const float* __restrict__ lut8,

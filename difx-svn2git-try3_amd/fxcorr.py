@@ -34,7 +34,7 @@ FXG_XMAC_BF16 = 2
 FXG_OK, FXG_EINVAL, FXG_ENOMEM, FXG_EHIP, FXG_EUNSUPPORTED, FXG_ESTATE, FXG_ENODEV = 0, -1, -2, -3, -4, -5, -6
 FXG_INVALID_SUBINT = -99999999
 FXG_FLAGS_PER_INT = 30
-STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h"]
+STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h", "pcal"]
 
 XMAC_NAMES = {"fp32": FXG_XMAC_FP32, "fp16": FXG_XMAC_FP16, "bf16": FXG_XMAC_BF16}
 
@@ -60,6 +60,8 @@ class Datastream(C.Structure):
         ("zoomband_pol", C.c_char * FXG_MAX_ZOOMBANDS),
         ("zoomband_parent", C.c_int32 * FXG_MAX_ZOOMBANDS),
         ("zoomband_choffset", C.c_int32 * FXG_MAX_ZOOMBANDS),
+        ("pcal_interval_mhz", C.c_double),
+        ("pcal_base_mhz", C.c_double),
     ]
 
 
@@ -148,6 +150,8 @@ class JobSpec:
     pols: str = "RL"
     fringe_order: int = 1
     chans_to_avg: int = 1               # channel averaging of every frequency
+    pcal_interval_mhz: float = 0.0      # phase-cal tone interval (0 = no extraction)
+    pcal_base_mhz: float = 0.0
 
     @property
     def nrecbands(self) -> int:
@@ -198,6 +202,8 @@ class JobSpec:
             d.nrecfreqs = nsb
             d.nrecbands = nsb * npol
             d.nzoombands = nz * npol
+            d.pcal_interval_mhz = self.pcal_interval_mhz
+            d.pcal_base_mhz = self.pcal_base_mhz
             recpol = bytearray(FXG_MAX_BANDS)
             zpol = bytearray(FXG_MAX_ZOOMBANDS)
             for sb in range(nsb):
@@ -270,8 +276,10 @@ def _declare_common(lib, prefix):
     lib.__getattr__(prefix + "plan").argtypes = [P(JobDesc), P(PlanInfo)]
     for name in ("plan_baseline_offset", "plan_bweight_offset"):
         lib.__getattr__(prefix + name).argtypes = [P(JobDesc), C.c_int, C.c_int, P(C.c_int64)]
-    for name in ("plan_autocorr_offset", "plan_acweight_offset"):
+    for name in ("plan_autocorr_offset", "plan_acweight_offset", "plan_pcal_offset"):
         lib.__getattr__(prefix + name).argtypes = [P(JobDesc), C.c_int, P(C.c_int64)]
+    lib.__getattr__(prefix + "plan_pcal_tones").argtypes = [P(JobDesc), C.c_int, C.c_int, P(C.c_int32),
+                                                            P(C.c_int32)]
 
 
 def load_gpu_lib():

@@ -142,12 +142,17 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
                 sub_sec: int = 10, guard_frames: Optional[int] = None,
                 invalid_frames: Optional[dict] = None,
                 invalid_blocks: Optional[dict] = None,
-                extra_frames: int = 1) -> SubintData:
+                extra_frames: int = 1,
+                pcal: Optional[dict] = None) -> SubintData:
     """Generate one subintegration of raw frames for every station.
 
     delays_us: model delay per station (the data carry the whole-sample part
     of it; any fractional part is then a model-vs-data offset).  rates add a
     linear delay rate to the model only (exercises the fringe rotator).
+    pcal: phase-cal comb added to every band before quantisation, locked to
+    the second boundary: {"amp": a, "spacing_hz": s, "offsets_hz": [per
+    subband], "ntones": [per subband], "slope_deg": d}; tone k of a band is
+    a * sin(2 pi (offset + k s) n / fs + k d) at absolute sample n.
     """
     rng = np.random.default_rng(seed)
     ts = spec.sampletime_us
@@ -182,13 +187,23 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
     hi = max(dsamp)
     stations = []
     codes_all = [np.empty((T, spec.nrecbands), dtype=np.uint8) for _ in range(nst)]
+    datasamples = int(datans / (ts * 1e3) + 0.5)
     for sb in range(nsb):
         g = rng.standard_normal(T + hi - lo)
         r = float(rhos[sb])
+        comb = 0.0
+        if pcal:
+            fs_hz = 2e6 * spec.bw_mhz
+            nabs = datasamples + np.arange(T, dtype=np.float64)
+            comb = np.zeros(T)
+            for k in range(pcal["ntones"][sb]):
+                f = pcal["offsets_hz"][sb] + k * pcal["spacing_hz"]
+                ph = 2 * np.pi * np.mod(f * nabs, fs_hz) / fs_hz
+                comb += pcal["amp"] * np.sin(ph + np.radians(k * pcal.get("slope_deg", 0.0)))
         for s in range(nst):
             gs = g[dsamp[s] - lo: dsamp[s] - lo + T]
             for p in range(npol):
-                x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * rng.standard_normal(T)
+                x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * rng.standard_normal(T) + comb
                 q = quantise2(x) if spec.nbits == 2 else quantise8(x)
                 codes_all[s][:, sb * npol + p] = q
     nflags = (spec.blockspersend + FXG_FLAGS_PER_INT - 1) // FXG_FLAGS_PER_INT

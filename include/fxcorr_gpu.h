@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FXG_ABI_VERSION 1
+#define FXG_ABI_VERSION 2
 
 /* ---- limits of the fixed-size descriptors ------------------------------ */
 #define FXG_MAX_FREQS        64   /* frequency table entries per job          */
@@ -79,7 +79,7 @@ typedef struct {
   double  bw_mhz;          /* bandwidth                                         */
   int32_t lsb;             /* 1 = lower sideband                                */
   int32_t nchan;           /* output channels (recordedbandchannels)            */
-  int32_t chans_to_avg;    /* must be 1 in this build                           */
+  int32_t chans_to_avg;    /* output channels averaged into one (divides nchan) */
   int32_t pad;
 } fxg_freq;
 
@@ -108,6 +108,12 @@ typedef struct {
   char    zoomband_pol[FXG_MAX_ZOOMBANDS];
   int32_t zoomband_parent[FXG_MAX_ZOOMBANDS];
   int32_t zoomband_choffset[FXG_MAX_ZOOMBANDS];
+  /* multi-tone phase calibration (datastreamdata phasecalintervalmhz /
+   * phasecalbasemhz, configuration.cpp:1615-1623): tones every interval MHz
+   * of sky frequency, extracted per recorded band (Mode::process,
+   * mode.cpp:774-788; PCal::getNew, pcal.cpp:139-203).  0 = no extraction. */
+  double  pcal_interval_mhz;
+  double  pcal_base_mhz;
 } fxg_datastream;
 
 /* ---- baseline table entry (configuration.h baselinedata) ---------------- */
@@ -129,7 +135,7 @@ typedef struct {
   int32_t ndatastreams;
   int32_t nbaselines;
   int32_t blockspersend;   /* FFT blocks per subintegration                     */
-  int32_t fringe_order;    /* 1 (linear, the vex2difx default) — 0/2 not built  */
+  int32_t fringe_order;    /* 0 (post-F), 1 (linear, the vex2difx default), 2  */
   int32_t arraystridelen;  /* 0 = automatic (configuration.cpp:46-59)           */
   int32_t xmacstridelen;   /* 0 = automatic (configuration.cpp:2464-2511)       */
   int32_t crosspol_autocorrs; /* compute and output cross-pol autocorrelations  */
@@ -208,6 +214,16 @@ int fxg_plan_baseline_offset(const fxg_job_desc* desc, int freq, int baseline, i
 int fxg_plan_bweight_offset(const fxg_job_desc* desc, int freq, int baseline, int64_t* off);
 int fxg_plan_autocorr_offset(const fxg_job_desc* desc, int datastream, int64_t* off);
 int fxg_plan_acweight_offset(const fxg_job_desc* desc, int datastream, int64_t* off);
+/* Phase-cal results of a datastream (Configuration::getCoreResultPCalOffset,
+ * configuration.cpp:2422-2431): per recorded band in band order, the tones of
+ * its frequency (getDRecordedFreqNumPCalTones, configuration.cpp:1771-1845).
+ * A datastream without phase cal has no entries there. */
+int fxg_plan_pcal_offset(const fxg_job_desc* desc, int datastream, int64_t* off);
+/* Tones of a recorded frequency (local index) and the offset of the first
+ * tone from the band edge in Hz (-1: no tone inside the band; then one zero
+ * entry is kept, as the reference does).  *ntones = 0 without phase cal. */
+int fxg_plan_pcal_tones(const fxg_job_desc* desc, int datastream, int recfreq, int32_t* ntones,
+                        int32_t* offset_hz);
 
 /* ---- context / device ---------------------------------------------------- */
 typedef struct fxg_ctx fxg_ctx;
@@ -282,7 +298,8 @@ int fxg_station_process_block(fxg_job* job, const fxg_subint_desc* sub,
 #define FXG_STAGE_SCATTER   5   /* copy into the result layout                 */
 #define FXG_STAGE_H2D       6   /* host -> device copies of raw data           */
 #define FXG_STAGE_D2H       7   /* device -> host copy of results              */
-#define FXG_NSTAGES         8
+#define FXG_STAGE_PCAL      8   /* phase-cal accumulation and tone DFT         */
+#define FXG_NSTAGES         9
 int fxg_job_set_profiling(fxg_job* job, int on);
 /* ms[FXG_NSTAGES] accumulated milliseconds, launches[FXG_NSTAGES] counts;
  * reset != 0 clears the accumulators after reading. */

@@ -22,7 +22,12 @@
  *   xmac     mpifxcorr/src/core.cpp:667-1122 (processdata), :1159-1369
  *            (averageAndSendAutocorrs), :1586-1867 (uvshiftAndAverage, single
  *            phase centre, no channel averaging)
- *   layout   mpifxcorr/src/configuration.cpp:46-59, 2212-2441, 2464-2546
+ *   layout   mpifxcorr/src/configuration.cpp:46-59, 1771-1845, 2212-2441, 2464-2546
+ *   pcal     mpifxcorr/src/mode.cpp:412-428, 774-788 (extraction per block);
+ *            pcal.cpp:139-229 (getNew, calcNumTones), :253-331 (the direct
+ *            reference method), :333-470 (first tone at DC), :498-690
+ *            (shifting), :927-1104 (implicit shift), :1376-1450 (dummy);
+ *            mathutil.cpp:8-30 (gcd); core.cpp:1124-1157 (copyPCalTones)
  *
  * Two arithmetic flavours share one code path through a template parameter:
  *   float  — the reference's own FP32 arithmetic (the drop-in parity target);
@@ -313,6 +318,7 @@ struct Layout {
   std::vector<char> freqused, equivused;
   std::vector<std::vector<int64_t>> bloff, bwoff;  /* [freq][baseline], -1 if absent */
   std::vector<int64_t> acoff, acwoff, pcaloff;
+  std::vector<std::vector<int>> pcal_ntones, pcal_offhz;  /* [ds][recfreq]; empty: no phase cal */
   std::vector<std::vector<int>> blfreq_local;      /* [baseline][freq] -> local index or -1 */
   int64_t total = 0;
   int bandsperautocorr = 1;
@@ -378,6 +384,12 @@ static int validate(const fxg_job_desc* j) {
 /* populateResultLengths for one config, single phase centre, no pulsar
  * binning (configuration.cpp:2212-2441), plus the used / equivalent-used
  * frequency flags (configuration.cpp:325-367) and stride selection. */
+/* phase-cal extractor kinds (PCal::getNew, pcal.cpp:139-203) */
+enum { PC_DUMMY = 0, PC_TRIVIAL = 1, PC_SHIFT = 2, PC_IMPLICIT = 3 };
+static void recfreq_pcal_tones(const fxg_freq& fq, double interval, double base, int* ntones, int* offsethz);
+static int pcal_kind(double bw_hz, double spacing_hz, int offset_hz, double minres);
+static int pcal_ntones_of(int kind, double bw_hz, double spacing_hz, int offset_hz);
+
 static int make_layout(const fxg_job_desc* j, Layout* L) {
   int r = validate(j); if (r) return r;
   L->nfreqs = j->nfreqs; L->nds = j->ndatastreams; L->nbl = j->nbaselines;
@@ -474,7 +486,27 @@ static int make_layout(const fxg_job_desc* j, Layout* L) {
     if (toadd == 0) toadd = 1;
     idx += toadd;
   }
-  for (int d = 0; d < j->ndatastreams; d++) L->pcaloff[d] = idx;  /* phase cal off in this build */
+  /* and finally the phase cal (configuration.cpp:2422-2431) */
+  L->pcal_ntones.assign(j->ndatastreams, {}); L->pcal_offhz.assign(j->ndatastreams, {});
+  for (int d = 0; d < j->ndatastreams; d++) {
+    const fxg_datastream& ds = j->datastreams[d];
+    L->pcaloff[d] = idx;
+    if (!(ds.pcal_interval_mhz > 0)) continue;
+    L->pcal_ntones[d].resize(ds.nrecfreqs); L->pcal_offhz[d].resize(ds.nrecfreqs);
+    for (int i = 0; i < ds.nrecfreqs; i++)
+      recfreq_pcal_tones(j->freqs[ds.recfreq_index[i]], ds.pcal_interval_mhz, ds.pcal_base_mhz,
+                         &L->pcal_ntones[d][i], &L->pcal_offhz[d][i]);
+    for (int k = 0; k < ds.nrecbands; k++) idx += L->pcal_ntones[d][ds.recband_freq[k]];
+    /* Mode's check that configuration.cpp and pcal.cpp agree (mode.cpp:426-427) */
+    const double bwhz = 1e6 * j->freqs[ds.recfreq_index[0]].bw_mhz;
+    for (int i = 0; i < ds.nrecfreqs; i++) {
+      if (L->pcal_offhz[d][i] < -1) return fail(FXG_EUNSUPPORTED, "phase cal: first tone outside the band");
+      const int kind = pcal_kind(bwhz, 1e6 * ds.pcal_interval_mhz, L->pcal_offhz[d][i], 1e6);
+      if (kind != PC_DUMMY && pcal_ntones_of(kind, bwhz, 1e6 * ds.pcal_interval_mhz, L->pcal_offhz[d][i]) !=
+                                  L->pcal_ntones[d][i])
+        return fail(FXG_EUNSUPPORTED, "phase cal: extractor and configuration tone counts differ");
+    }
+  }
   L->total = idx;
   return 0;
 }
@@ -590,6 +622,7 @@ static BlockGeom block_geom(const DsGeom& g, const fxg_subint_desc& sub,
 
 template <typename T> struct StationWork {
   std::vector<float> unpacked;
+  Unpacked up{};                    /* where the block's samples start in `unpacked` */
   std::vector<cpx<T>> rot, fracA, fracB, buf;
   std::vector<std::complex<T>> fftbuf;
 };
@@ -604,6 +637,7 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
   const int C = g.C, L = g.L, s = g.s;
   Unpacked up{};
   BlockGeom bg = block_geom(g, sub, in, index, &w.unpacked, &up);
+  w.up = up;
   std::memset((void*)spectra, 0, sizeof(cpx<T>) * (size_t)ds.nrecbands * C);
   if (bg.status != 0) return bg;
   const bool fp64 = sizeof(T) == 8;
@@ -746,6 +780,255 @@ static inline void addprod_conj(A* acc, const cpx<T>& a, const cpx<T>& b) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Phase cal                                                                */
+/* ------------------------------------------------------------------------ */
+/* mathutil.cpp:8-30 */
+static long long gcd_l(long a, long b) {
+  if (a == 0 || b == 0) return std::max(a, b);
+  while (true) {
+    a = a % b;
+    if (a == 0) return b;
+    b = b % a;
+    if (b == 0) return a;
+  }
+}
+static long long gcd_d(double a, double b) { return gcd_l((long)std::floor(a + 0.5), (long)std::floor(b + 0.5)); }
+
+/* PCal::calcNumTones (pcal.cpp:212-229) */
+static int calc_num_tones(double bw, double offset, double step) {
+  int n = 0;
+  if (bw <= 0.0f || step <= 0.0f) return 0;
+  double ftone = offset;
+  while (ftone <= bw) {
+    n++;
+    ftone += step;
+    if (step <= 0.0f) break;
+  }
+  if (offset == 0.0 && ftone - step == bw) n--;
+  return n;
+}
+
+/* Tones of one recorded frequency (configuration.cpp:1771-1845) */
+static void recfreq_pcal_tones(const fxg_freq& fq, double interval, double base, int* ntones, int* offsethz) {
+  int n = 0;
+  const double lofreq = fq.freq_mhz, bw = fq.bw_mhz;
+  double tonefreq;
+  if (fq.lsb) {
+    tonefreq = double(int(lofreq / interval)) * interval;
+    if (tonefreq == lofreq) tonefreq -= interval;
+    if (tonefreq >= lofreq - bw)
+      while (tonefreq - n * interval >= lofreq - bw) n++;
+    if (n > 0) {
+      double t0 = 1e6 * tonefreq - double(0) * 1e6 * interval - 1e6 * base;
+      *offsethz = (int)long(1e6 * lofreq - t0 + 0.5);
+    }
+  } else {
+    tonefreq = double(int(lofreq / interval)) * interval;
+    if (tonefreq <= lofreq) tonefreq += interval;
+    if (tonefreq <= lofreq + bw)
+      while (tonefreq + n * interval <= lofreq + bw) n++;
+    if (n > 0) {
+      double t0 = 1e6 * tonefreq + 0 * 1e6 * interval + 1e6 * base;
+      *offsethz = (int)long(t0 - 1e6 * lofreq + 0.5);
+    }
+  }
+  if (n == 0) { n = 1; *offsethz = -1; }
+  *ntones = n;
+}
+
+/* Forward DFT bin k of x[N] (vectorDFT_CtoC_cf32, unnormalised), in FP64 */
+template <typename T>
+static cpx<double> dft_bin(const cpx<T>* x, long N, long k) {
+  double re = 0, im = 0;
+  for (long n = 0; n < N; n++) {
+    const long m = (long)(((long long)n * k) % N);
+    const double a = -TWO_PI * double(m) / double(N);
+    const double c = std::cos(a), sn = std::sin(a);
+    re += double(x[n].re) * c - double(x[n].im) * sn;
+    im += double(x[n].re) * sn + double(x[n].im) * c;
+  }
+  cpx<double> o; o.re = re; o.im = im; return o;
+}
+
+/* The real-sampling extractors of pcal.cpp, with their state: trivial
+ * (:333-470), shifting (:498-690), implicit shift (:927-1104), dummy
+ * (:1376-1450).  T is the accumulator type (f32 in the reference). */
+template <typename T>
+struct PcalExtractor {
+  int kind = PC_DUMMY;
+  double fs = 0, spacing = 0;
+  long offset = 0;
+  int nbins = 0, ntones = 0, tonestep = 0;
+  size_t rotatorlen = 0, rotator_index = 0, pcal_index = 0;
+  uint64_t samplecount = 0;
+  bool finalized = false;
+  std::vector<T> preal;
+  std::vector<cpx<T>> pcplx, rotated, rotator;
+
+  void init(int k, double bw_hz, double spacing_hz, int offset_hz, double minres) {
+    kind = k; spacing = std::fabs(spacing_hz); offset = offset_hz; fs = 2 * bw_hz;
+    if (kind == PC_TRIVIAL) {
+      offset = 0;
+      nbins = (int)(fs / gcd_d(fs, spacing));
+      ntones = calc_num_tones(bw_hz, 0.0f, spacing) - 1;
+      tonestep = (int)(spacing / gcd_d(fs, spacing));
+    } else if (kind == PC_SHIFT) {
+      nbins = (int)(fs / gcd_d(fs, spacing));
+      ntones = calc_num_tones(bw_hz, offset, spacing);
+      rotatorlen = (size_t)(fs / gcd_d(fs, (double)offset));
+      rotatorlen = nbins * rotatorlen / gcd_l((long)rotatorlen, (long)nbins);
+      rotator.resize(2 * rotatorlen);
+      const double dphi = 2 * M_PI * (-offset / fs);
+      for (size_t n = 0; n < 2 * rotatorlen; n++) {      /* f32(cos(arg)) in the reference */
+        const double arg = dphi * double(n);
+        rotator[n].re = (T)std::cos(arg);
+        rotator[n].im = (T)std::sin(arg);
+      }
+      rotated.assign(2 * rotatorlen, cpx<T>{0, 0});
+    } else if (kind == PC_IMPLICIT) {
+      nbins = (int)(fs / gcd_d(fs, (double)offset));
+      ntones = calc_num_tones(bw_hz, offset, spacing);
+      while (fs / (2 * nbins) > minres) nbins *= 2;
+    } else {
+      ntones = calc_num_tones(bw_hz, (double)offset, spacing);
+      nbins = 2 * ntones;
+    }
+    clear();
+  }
+  void clear() {
+    samplecount = 0; finalized = false;
+    pcplx.assign(2 * (size_t)std::max(nbins, 1), cpx<T>{0, 0});
+    preal.assign(2 * (size_t)std::max(nbins, 1), (T)0);
+    if (kind == PC_SHIFT) rotated.assign(2 * rotatorlen, cpx<T>{0, 0});
+  }
+  void adjust(size_t so) {
+    if (kind == PC_SHIFT) { rotator_index = so % rotatorlen; pcal_index = so % nbins; }
+    else if (kind == PC_TRIVIAL || kind == PC_IMPLICIT) { rotator_index = 0; pcal_index = so % nbins; }
+  }
+  void extract(const float* src, size_t len) {
+    if (finalized) return;
+    if (kind == PC_TRIVIAL || kind == PC_IMPLICIT) {
+      T* dst = &preal[pcal_index];
+      const size_t tail = len % nbins, end = len - tail;
+      for (size_t n = 0; n < end; n += nbins, src += nbins)
+        for (int i = 0; i < nbins; i++) dst[i] += (T)src[i];
+      if (tail != 0) {
+        for (size_t i = 0; i < tail; i++) dst[i] += (T)src[i];
+        pcal_index = (pcal_index + tail) % nbins;
+      }
+    } else if (kind == PC_SHIFT) {
+      cpx<T>* dst = &pcplx[pcal_index];
+      const size_t tail = len % rotatorlen, end = len - tail;
+      auto mix = [&](const float* x, size_t n) {       /* vectorMul_f32cf32 */
+        for (size_t i = 0; i < n; i++) {
+          rotated[rotator_index + i].re = (T)x[i] * rotator[rotator_index + i].re;
+          rotated[rotator_index + i].im = (T)x[i] * rotator[rotator_index + i].im;
+        }
+      };
+      auto add = [&](const cpx<T>* p, size_t n) {
+        for (size_t i = 0; i < n; i++) { dst[i].re += p[i].re; dst[i].im += p[i].im; }
+      };
+      for (size_t n = 0; n < end; n += rotatorlen, src += rotatorlen) {
+        mix(src, rotatorlen);
+        const cpx<T>* pulse = &rotated[rotator_index];
+        for (size_t p = 0; p < rotatorlen / nbins; p++) { add(pulse, nbins); pulse += nbins; }
+      }
+      if (tail != 0) {
+        mix(src, tail);
+        const cpx<T>* pulse = &rotated[rotator_index];
+        const size_t tail2 = tail % nbins, end2 = tail - tail2;
+        for (size_t p = 0; p < end2 / (size_t)nbins; p++) { add(pulse, nbins); pulse += nbins; }
+        add(pulse, tail2);
+        rotator_index = (rotator_index + tail) % rotatorlen;
+        pcal_index = (pcal_index + tail2) % nbins;
+      }
+    }
+    samplecount += len;
+  }
+  /* getFinalPCal: out[ntones] */
+  uint64_t final(cpx<T>* out) {
+    if (kind == PC_DUMMY) {
+      finalized = true;
+      for (int i = 0; i < ntones; i++) out[i] = cpx<T>{0, 0};
+      return samplecount;
+    }
+    if (!finalized) {
+      finalized = true;
+      if (kind == PC_TRIVIAL) {
+        for (int i = 0; i < nbins; i++) preal[i] += preal[nbins + i];
+        for (int i = 0; i < nbins; i++) { pcplx[i].re = preal[i]; pcplx[i].im = 0; }
+      } else if (kind == PC_IMPLICIT) {
+        for (int i = 0; i < 2 * nbins; i++) { pcplx[i].re = preal[i]; pcplx[i].im = 0; }
+        for (int i = 0; i < nbins; i++) { pcplx[i].re += pcplx[nbins + i].re; pcplx[i].im += pcplx[nbins + i].im; }
+      } else {
+        for (int i = 0; i < nbins; i++) { pcplx[i].re += pcplx[nbins + i].re; pcplx[i].im += pcplx[nbins + i].im; }
+      }
+    }
+    auto bin = [&](long k) { cpx<double> d = dft_bin(pcplx.data(), nbins, k); return cpx<T>{(T)d.re, (T)d.im}; };
+    if (kind == PC_TRIVIAL) {
+      for (int i = 0; i < ntones; i++) out[i] = bin((long)(i + 1) * tonestep);
+    } else if (kind == PC_IMPLICIT) {
+      const size_t step = (size_t)(std::floor(double(nbins) * (spacing / fs)));
+      const size_t off = (size_t)(std::floor(double(nbins) * (offset / fs)));
+      for (size_t n = 0; n < (size_t)ntones; n++) {
+        const size_t idx = off + n * step;
+        out[n] = idx >= (size_t)nbins ? cpx<T>{0, 0} : bin((long)idx);
+      }
+    } else {
+      const size_t step = (size_t)(std::floor(nbins * (spacing / fs)));
+      for (size_t n = 0; n < (size_t)ntones; n++) {
+        const size_t idx = n * step;
+        if (idx >= (size_t)nbins) { out[n] = cpx<T>{0, 0}; continue; }   /* left untouched by the reference */
+        out[n] = bin((long)idx);
+      }
+    }
+    return samplecount;
+  }
+};
+
+/* PCal::getNew for real data (pcal.cpp:139-203) */
+static int pcal_kind(double bw_hz, double spacing_hz, int offset_hz, double minres) {
+  const double fs = 2.0 * bw_hz;
+  int No, Np, Nt;
+  if (offset_hz == -1) { No = Np = Nt = 0; }
+  else {
+    No = (int)(fs / gcd_d(fs, (double)offset_hz));
+    Np = (int)(fs / gcd_d(fs, spacing_hz));
+    Nt = calc_num_tones(bw_hz, (double)offset_hz, spacing_hz);
+    while (fs / (2 * No) > minres) No *= 2;
+  }
+  if (Nt == 0) return PC_DUMMY;
+  if (offset_hz == 0.0f) return PC_TRIVIAL;
+  if ((No % Np) == 0) return PC_IMPLICIT;
+  return PC_SHIFT;
+}
+
+static int pcal_ntones_of(int kind, double bw_hz, double spacing_hz, int offset_hz) {
+  if (kind == PC_TRIVIAL) return calc_num_tones(bw_hz, 0.0f, spacing_hz) - 1;
+  return calc_num_tones(bw_hz, (double)offset_hz, spacing_hz);
+}
+
+/* PCal::extractAndIntegrate_reference (pcal.cpp:253-331): the slow direct
+ * method the reference keeps for comparison.  out[ntones]; state is the
+ * running sample count. */
+static void pcal_direct(double fs, long offset, int ntones, uint64_t* samplecount, const float* data, size_t len,
+                        uint64_t sampleoffset, cpx<double>* out) {
+  const size_t Nbins = 2 * ntones;
+  const size_t maxtoneperiod = (size_t)(fs / gcd_l(offset, (long)fs));
+  const double dphi = 2 * M_PI * (-offset / fs);
+  std::vector<cpx<float>> acc(Nbins, cpx<float>{0, 0});
+  for (size_t n = 0; n < len; n++) {
+    const size_t no = n + sampleoffset;
+    const int b = (int)(no % Nbins);
+    const double phi = dphi * ((*samplecount + no) % maxtoneperiod);
+    acc[b].re += std::cos(phi) * data[n];
+    acc[b].im += std::sin(phi) * data[n];
+  }
+  *samplecount += len;
+  for (int i = 0; i < ntones; i++) out[i] = dft_bin(acc.data(), (long)Nbins, offset == 0 ? i + 1 : i);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Core::processdata restated                                               */
 /* ------------------------------------------------------------------------ */
 template <typename T>
@@ -773,6 +1056,21 @@ struct CoreRun {
       w0[d].assign(ds.nrecbands, 0); w1[d].assign(ds.nrecbands, 0);
     }
     std::vector<float> dweight(nds);
+    /* this thread's phase-cal extractors per (datastream, recorded band),
+     * created as Mode's constructor does (mode.cpp:412-428) */
+    std::vector<std::vector<PcalExtractor<T>>> pcx(nds);
+    std::vector<int> datasamples(nds, 0);
+    for (int d = 0; d < nds; d++) {
+      const fxg_datastream& ds = j->datastreams[d];
+      if (lay.pcal_ntones[d].empty()) continue;
+      const double bwhz = 1e6 * j->freqs[ds.recfreq_index[0]].bw_mhz;
+      pcx[d].resize(ds.nrecbands);
+      for (int k = 0; k < ds.nrecbands; k++) {
+        const int off = lay.pcal_offhz[d][ds.recband_freq[k]];
+        pcx[d][k].init(pcal_kind(bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6), bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6);
+      }
+      datasamples[d] = static_cast<int>(st[d].ns / (geom[d].sampletime * 1e3) + 0.5);   /* mode.cpp:1572 */
+    }
     /* this thread's cross products before averaging (threadcrosscorrs):
      * [freq][baseline][product][nchan] at xoff[f][b] */
     std::vector<std::vector<int64_t>> xoff(j->nfreqs, std::vector<int64_t>(j->nbaselines, -1));
@@ -792,6 +1090,11 @@ struct CoreRun {
         dweight[d] = bg.status == 0 ? bg.weight : 0.0f;
         if (bg.status != 0) continue;
         const int C = geom[d].C;
+        /* phase cal of the block's L samples (mode.cpp:774-788) */
+        for (size_t k = 0; k < pcx[d].size(); k++) {
+          pcx[d][k].adjust((size_t)(datasamples[d] + bg.nearestsample));
+          pcx[d][k].extract(work[d].unpacked.data() + k * work[d].up.stride + work[d].up.munged, (size_t)geom[d].L);
+        }
         for (int i = 0; i < ds.nrecfreqs; i++) {
           int idx[FXG_MAX_BANDS]; int count = 0;
           for (int jb = 0; jb < ds.nrecbands; jb++) {
@@ -893,6 +1196,16 @@ struct CoreRun {
           ri += nch;
         }
       }
+      /* copyPCalTones (core.cpp:1124-1157) after Mode::finalisepcal */
+      if (!pcx[d].empty()) {
+        int64_t pi = lay.pcaloff[d];
+        for (int k = 0; k < ds.nrecbands; k++) {
+          const int nt = lay.pcal_ntones[d][ds.recband_freq[k]];
+          std::vector<cpx<T>> tones(std::max(nt, pcx[d][k].ntones) + 1, cpx<T>{0, 0});
+          pcx[d][k].final(tones.data());
+          for (int t = 0; t < nt; t++) { acc[2 * (size_t)pi] += tones[t].re; acc[2 * (size_t)pi + 1] += tones[t].im; pi++; }
+        }
+      }
       int64_t wi = lay.acwoff[d] * 2;
       for (int pass = 0; pass < lay.bandsperautocorr; pass++) {
         if (pass == 1 && !j->crosspol_autocorrs) break;
@@ -973,6 +1286,50 @@ int orc_plan_acweight_offset(const fxg_job_desc* j, int d, int64_t* off) {
   Layout L; int r = make_layout(j, &L); if (r) return r;
   if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "index");
   *off = L.acwoff[d]; return 0;
+}
+
+int orc_plan_pcal_offset(const fxg_job_desc* j, int d, int64_t* off) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "index");
+  *off = L.pcaloff[d]; return 0;
+}
+int orc_plan_pcal_tones(const fxg_job_desc* j, int d, int recfreq, int32_t* ntones, int32_t* offset_hz) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "index");
+  if (L.pcal_ntones[d].empty()) { *ntones = 0; *offset_hz = -1; return 0; }
+  if (recfreq < 0 || recfreq >= (int)L.pcal_ntones[d].size()) return fail(FXG_EINVAL, "index");
+  *ntones = L.pcal_ntones[d][recfreq]; *offset_hz = L.pcal_offhz[d][recfreq]; return 0;
+}
+
+/* One phase-cal extractor over a sample stream, as the reference's unit test
+ * drives it (pcal.cpp:1590-1640): kind -1 = PCal::getNew's choice with the
+ * given minimum resolution, else PC_*; nseg calls, call i at sample offset
+ * so[i] with data + start[i], len[i] samples.  out[*ntones] (complex float
+ * pairs); returns the extractor kind. */
+int orc_pcal_extract(double bw_hz, double spacing_hz, int offset_hz, int kind, double minres, int nseg,
+                     const int64_t* so, const int64_t* start, const int64_t* len, const float* data,
+                     float* out, int32_t* ntones, int64_t* samples) {
+  if (kind < 0) kind = pcal_kind(bw_hz, spacing_hz, offset_hz, minres);
+  PcalExtractor<float> x;
+  x.init(kind, bw_hz, spacing_hz, offset_hz, minres);
+  for (int i = 0; i < nseg; i++) { x.adjust((size_t)so[i]); x.extract(data + start[i], (size_t)len[i]); }
+  std::vector<cpx<float>> o(x.ntones + 1);
+  *samples = (int64_t)x.final(o.data());
+  *ntones = x.ntones;
+  for (int i = 0; i < x.ntones; i++) { out[2 * i] = o[i].re; out[2 * i + 1] = o[i].im; }
+  return kind;
+}
+
+/* PCal::extractAndIntegrate_reference on one call of len samples. */
+int orc_pcal_direct(double bw_hz, double spacing_hz, int offset_hz, int64_t sampleoffset, const float* data,
+                    int64_t len, double* out, int32_t* ntones) {
+  const int nt = calc_num_tones(bw_hz, (double)offset_hz, spacing_hz) - (offset_hz == 0 ? 1 : 0);
+  std::vector<cpx<double>> o(nt + 1);
+  uint64_t count = 0;
+  pcal_direct(2 * bw_hz, offset_hz, nt, &count, data, (size_t)len, (uint64_t)sampleoffset, o.data());
+  for (int i = 0; i < nt; i++) { out[2 * i] = o[i].re; out[2 * i + 1] = o[i].im; }
+  *ntones = nt;
+  return 0;
 }
 
 /* Block geometry of one station/block (the Mode::process prologue). */

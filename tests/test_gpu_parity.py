@@ -376,3 +376,56 @@ def test_uneven_workload_jobs(gpu, oracle, spec):
         assert rel_err(got[:xend], ref[:xend]) < tol, (xmac, rel_err(got[:xend], ref[:xend]))
         assert rel_err(got[a0:a1], ref[a0:a1]) < tol
         g.close()
+
+
+# phase-cal configurations: the implicit-shift extractor (fractional-MHz band
+# edge), the shifting extractor (No % Np != 0), lower sideband, and a comb
+# shifted onto DC (the first-tone-at-DC extractor)
+PCAL_CASES = {
+    "implicit": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0),
+    "shifting": dict(base_freq_mhz=8401.0, pcal_interval_mhz=3.0),
+    "lsb": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0, lsb=(1, 1)),
+    "dc": dict(base_freq_mhz=8400.0, pcal_interval_mhz=3.0, pcal_base_mhz=-3.0, nsubbands=1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(PCAL_CASES))
+def test_phase_cal(gpu, oracle, case):
+    """Phase-cal tones of every recorded band (Mode::process extraction,
+    copyPCalTones) against the oracle's restated extractors, FP32, with a
+    flagged block, a bad frame and a comb injected at known phases."""
+    lib, ctx = gpu
+    kw = dict(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=24)
+    kw.update(PCAL_CASES[case])
+    spec = JobSpec(**kw)
+    job, keep = spec.build()
+    ntones, offs = [], []
+    for sb in range(spec.nsubbands):
+        nt, off = C.c_int32(), C.c_int32()
+        check(lib, lib.fxg_plan_pcal_tones(C.byref(job), 0, sb, C.byref(nt), C.byref(off)), "pcal tones")
+        ntones.append(nt.value)
+        offs.append(max(off.value, 0))
+    comb = dict(amp=0.5, spacing_hz=1e6 * spec.pcal_interval_mhz, offsets_hz=offs, ntones=ntones, slope_deg=5.0)
+    sd = make_data(spec, invalid_frames={1: [2]}, invalid_blocks={2: [3, 4]}, pcal=comb)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd, fp64=False, nthreads=8)
+    p0 = C.c_int64()
+    check(lib, lib.fxg_plan_pcal_offset(C.byref(job), 0, C.byref(p0)), "pcal offset")
+    p0 = p0.value
+    assert g.info.result_length - p0 == spec.nstations * spec.npol * sum(ntones)
+    assert rel_err(got[p0:], ref[p0:]) < FP32_TOL, rel_err(got[p0:], ref[p0:])
+    assert np.max(np.abs(got[p0:] - ref[p0:])) < 1e-4 * np.max(np.abs(ref[p0:]))
+    # the rest of the result is unchanged by the extraction
+    assert rel_err(got[:p0], ref[:p0]) < FP32_TOL
+    if case in ("implicit", "shifting", "lsb"):
+        # station 0, band 0: tone k of the comb has phase -90 + 5 k degrees
+        ph = np.degrees(np.angle(got[p0:p0 + ntones[0]]))
+        exp = -90.0 + 5.0 * np.arange(ntones[0])
+        fs_edge = offs[0] + np.arange(ntones[0]) * comb["spacing_hz"] < 1e6 * spec.bw_mhz
+        d = (ph - exp + 180) % 360 - 180
+        # 12288 samples of 2-bit data: a few degrees of noise per tone; a sign,
+        # bin or sample-offset mistake would move the phases by tens of degrees
+        assert np.all(np.abs(d[fs_edge]) < 20.0), d
+        assert np.mean(np.abs(d[fs_edge])) < 6.0, d
+    g.close()
