@@ -19,6 +19,7 @@
 #include <mutex>
 #include <string>
 #include <deque>
+#include <functional>
 #include <vector>
 
 using namespace fxg;
@@ -90,6 +91,11 @@ struct fxg_job {
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
   DBuf d_zero;                    // 64 zero bytes: the rows past the data for streaming loads
   int xmac_streaming = 0;         // direct-to-LDS MFMA XMAC where the layout allows
+  // the subintegration as one graph launch, captured per launch shape
+  int use_graph = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_key;
+  long long graph_captures = 0;
   // asynchronous submissions: (ticket, completion event) in submission order
   std::deque<std::pair<int64_t, hipEvent_t>> pending;
   std::vector<hipEvent_t> free_events;
@@ -353,6 +359,8 @@ static void job_free(fxg_job* j) {
   j->pending.clear();
   for (hipEvent_t e : j->free_events) hipEventDestroy(e);
   j->free_events.clear();
+  if (j->graph_exec) (void)hipGraphExecDestroy(j->graph_exec);
+  j->graph_exec = nullptr;
   if (j->stream) hipStreamDestroy(j->stream);
 }
 
@@ -494,6 +502,18 @@ int fxg_job_set_xmac_streaming(fxg_job* j, int on) {
   return 0;
 }
 
+int fxg_job_set_graph(fxg_job* j, int on) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  j->use_graph = on ? 1 : 0;
+  return 0;
+}
+
+int fxg_job_graph_captures(const fxg_job* j, int64_t* captures) {
+  if (!j || !captures) return set_err(FXG_EINVAL, "null argument");
+  *captures = j->graph_captures;
+  return 0;
+}
+
 int fxg_job_set_keep_taps(fxg_job* j, int keep) {
   if (!j) return set_err(FXG_EINVAL, "null job");
   j->keep_taps = keep;
@@ -513,21 +533,26 @@ static void xmac_shape(int nin, int nchan, XmacArgs* a) {
   a->TB = std::min(tb, 16);
 }
 
-// Frame validity, corner turn and FP64 block geometry of one subintegration
-// (everything before the F stage).
-static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
-                          int data_on_device) {
+// What the launches of one subintegration depend on besides the job's fixed
+// buffers: a graph captured for one shape replays for every subintegration of
+// the same shape.
+struct LaunchShape {
+  long long maxframes = 0, maxsamples = 0;
+  int maxdatabytes = 0;
+  int fastW = 0;        // 2-bit register corner turn: W words per sample-time, else 0
+};
+
+// Raw data (host inputs), the per-datastream subint table and the valid
+// flags of one subintegration, copied to the device on stream s.
+static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                         int data_on_device, hipStream_t s, LaunchShape* sh) {
   Plan& P = j->plan;
-  const fxg_job_desc& D = j->desc;
-  hipStream_t s = j->stream;
   int rc;
-  const int nblocks = D.blockspersend;
-  // ---- per-datastream subint table
   std::vector<DevStationSub> ss(P.nds);
   std::vector<int> flags;
   long long fbase = 0, maxframes = 0, ctbytes = 0, maxsamples = 0;
   int maxdatabytes = 0;
-  int fastW = -1;   // 2-bit register corner turn: W words per sample-time for every datastream, else 0
+  int fastW = -1;
   for (int d = 0; d < P.nds; d++) {
     const fxg_station_input& in = st[d];
     DevStationSub& q = ss[d];
@@ -567,6 +592,7 @@ static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
     q.flagoff = (int)flags.size();
     for (int k = 0; k < q.nflags; k++) flags.push_back(in.validflags[k]);
     for (int k = 0; k < 3; k++) q.interp[k] = in.interpolator[k];
+    q.offsec = sub->sec; q.offns = sub->ns; q.scanstart = sub->scan_start_sec; q.pad = 0;
   }
   if ((rc = j->d_ss.ensure(sizeof(DevStationSub) * ss.size())) ||
       (rc = j->d_flags.ensure(sizeof(int) * std::max<size_t>(1, flags.size()))) ||
@@ -575,18 +601,37 @@ static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
     return rc;
   HIPCHK(hipMemcpyAsync(j->d_ss.p, ss.data(), sizeof(DevStationSub) * ss.size(), hipMemcpyHostToDevice, s));
   if (!flags.empty()) HIPCHK(hipMemcpyAsync(j->d_flags.p, flags.data(), sizeof(int) * flags.size(), hipMemcpyHostToDevice, s));
+  sh->maxframes = maxframes;
+  sh->maxsamples = maxsamples;
+  sh->maxdatabytes = maxdatabytes;
+  sh->fastW = fastW > 0 ? fastW : 0;
+  return 0;
+}
 
+// Frame validity, corner turn and FP64 block geometry (everything before the
+// F stage), launched on stream s.
+static int launch_front(fxg_job* j, const LaunchShape& sh, hipStream_t s) {
+  Plan& P = j->plan;
+  int rc;
   if ((rc = j->ev_begin(FXG_STAGE_FRAMES))) return rc;
-  HIPCHK(launch_frames(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ftab.as<int2>(), P.nds, maxframes, s));
+  HIPCHK(launch_frames(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ftab.as<int2>(), P.nds,
+                       sh.maxframes, s));
   HIPCHK(launch_cornerturn(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_ct.as<unsigned char>(),
-                           P.nds, maxframes, maxdatabytes, fastW > 0 ? fastW : 0, maxsamples, s));
+                           P.nds, sh.maxframes, sh.maxdatabytes, sh.fastW, sh.maxsamples, s));
   if ((rc = j->ev_end())) return rc;
-  DevSubint dsub{sub->sec, sub->ns, sub->scan_start_sec, nblocks};
   if ((rc = j->ev_begin(FXG_STAGE_PROLOGUE))) return rc;
   HIPCHK(launch_prologue(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_flags.as<int>(),
-                         j->d_ftab.as<int2>(), dsub, j->d_blocks.as<DevBlock>(), P.nds, s));
+                         j->d_ftab.as<int2>(), j->desc.blockspersend, j->d_blocks.as<DevBlock>(), P.nds, s));
   if ((rc = j->ev_end())) return rc;
   return 0;
+}
+
+static int prepare_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                          int data_on_device) {
+  LaunchShape sh;
+  int rc = upload_subint(j, sub, st, data_on_device, j->stream, &sh);
+  if (rc) return rc;
+  return launch_front(j, sh, j->stream);
 }
 
 static void fill_fstage_args(fxg_job* j, FStageArgs* fa) {
@@ -623,20 +668,13 @@ static std::vector<unsigned char> subint_key(const fxg_job* j, const fxg_subint_
   return k;
 }
 
-// Everything of one subintegration, enqueued on the job's stream.
-static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
-                          int data_on_device, float* results, int results_on_device) {
-  if (!j || !sub || !st || !results) return set_err(FXG_EINVAL, "null argument");
+// F and X stages chunk by chunk, weights, the cleared result, scatter and
+// phase cal, launched on stream s into dres (a device result buffer).
+static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
   Plan& P = j->plan;
   const fxg_job_desc& D = j->desc;
-  hipStream_t s = j->stream;
-  HIPCHK(hipSetDevice(j->ctx->device));
   int rc;
   const int nblocks = D.blockspersend;
-  j->prepared_key.clear();
-  if ((rc = prepare_subint(j, sub, st, data_on_device))) return rc;
-
-  // ---- F and X stages, chunk by chunk
   const int prec = D.xmac_precision;
   FStageArgs fa;
   fill_fstage_args(j, &fa);
@@ -683,9 +721,9 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
   HIPCHK(launch_weights(j->d_blocks.as<DevBlock>(), nblocks, j->d_blpairs.as<int2>(), P.nbl, P.nds,
                         j->d_weights.as<float>(), s));
   if ((rc = j->ev_end())) return rc;
-  const size_t resbytes = sizeof(float) * 2 * (size_t)P.info.result_length;
-  float* dres = results_on_device ? results : j->d_results.as<float>();
-  HIPCHK(hipMemsetAsync(dres, 0, resbytes, s));
+  // the slot is zeroed before accumulation (core.cpp:319) by a kernel, so
+  // that the sequence can live in a graph
+  HIPCHK(launch_zero(dres, 2 * P.info.result_length, s));
   float vscale = 1.0f / (j->scale * j->scale);
   if ((rc = j->ev_begin(FXG_STAGE_SCATTER))) return rc;
   HIPCHK(launch_scatter(j->d_segs.as<DevSeg>(), (int)P.segs.size(), P.maxseglen, j->d_V.as<float2>(),
@@ -702,7 +740,72 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
     HIPCHK(launch_pcal(pa, (int)P.pcal.size(), s));
     if ((rc = j->ev_end())) return rc;
   }
+  return 0;
+}
+
+// identity of a captured launch sequence: the shape, the result buffer and
+// the per-subint buffers the kernels take as arguments
+static std::vector<unsigned char> graph_key(const fxg_job* j, const LaunchShape& sh, const float* dres) {
+  std::vector<unsigned char> k;
+  auto put = [&](const void* p, size_t n) { const unsigned char* b = (const unsigned char*)p; k.insert(k.end(), b, b + n); };
+  put(&sh, sizeof(sh));
+  put(&dres, sizeof(dres));
+  const void* ptrs[4] = {j->d_ss.p, j->d_flags.p, j->d_ftab.p, j->d_ct.p};
+  put(ptrs, sizeof(ptrs));
+  put(&j->xmac_streaming, sizeof(int));
+  return k;
+}
+
+// Record fn's launches into a graph on stream s (fn enqueues on s and on
+// streams that fork from it), instantiate it into *exec.
+static int capture(hipStream_t s, hipGraphExec_t* exec, const std::function<int()>& fn) {
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = fn();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return set_err(FXG_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) { *exec = nullptr; return set_err(FXG_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+  return 0;
+}
+
+// Everything of one subintegration, enqueued on the job's stream: eagerly,
+// or as one graph launch replayed while the launch shape stays the same.
+static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                          int data_on_device, float* results, int results_on_device) {
+  if (!j || !sub || !st || !results) return set_err(FXG_EINVAL, "null argument");
+  Plan& P = j->plan;
+  hipStream_t s = j->stream;
+  HIPCHK(hipSetDevice(j->ctx->device));
+  int rc;
+  j->prepared_key.clear();
+  LaunchShape sh;
+  if ((rc = upload_subint(j, sub, st, data_on_device, s, &sh))) return rc;
+  float* dres = results_on_device ? results : j->d_results.as<float>();
+  if (j->use_graph && !j->profiling) {
+    std::vector<unsigned char> key = graph_key(j, sh, dres);
+    if (!j->graph_exec || key != j->graph_key) {
+      if (j->graph_exec) { (void)hipGraphExecDestroy(j->graph_exec); j->graph_exec = nullptr; }
+      j->graph_key.clear();
+      rc = capture(s, &j->graph_exec, [&]() {
+        int r = launch_front(j, sh, s);
+        return r ? r : launch_back(j, dres, s);
+      });
+      if (rc) return rc;
+      j->graph_key = key;
+      j->graph_captures++;
+    }
+    HIPCHK(hipGraphLaunch(j->graph_exec, s));
+    j->tap_chunk0 = ((j->desc.blockspersend - 1) / P.chunk_blocks) * P.chunk_blocks;
+    j->tap_chunkn = j->desc.blockspersend - j->tap_chunk0;
+  } else {
+    if ((rc = launch_front(j, sh, s))) return rc;
+    if ((rc = launch_back(j, dres, s))) return rc;
+  }
   if (!results_on_device) {
+    const size_t resbytes = sizeof(float) * 2 * (size_t)P.info.result_length;
     if ((rc = j->ev_begin(FXG_STAGE_D2H))) return rc;
     HIPCHK(hipMemcpyAsync(results, dres, resbytes, hipMemcpyDeviceToHost, s));
     if ((rc = j->ev_end())) return rc;
@@ -763,6 +866,127 @@ int fxg_subint_wait(fxg_job* j, int64_t ticket) {
     j->pending.pop_front();
   }
   j->ev_collect();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batches: one subintegration of each of several jobs in one graph launch;
+// every job's sequence is a branch forked from the batch stream
+// ---------------------------------------------------------------------------
+struct fxg_batch {
+  fxg_ctx* ctx = nullptr;
+  std::vector<fxg_job*> jobs;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> key;
+  long long captures = 0;
+};
+
+static void batch_free(fxg_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  for (hipEvent_t e : b->join) if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int fxg_batch_create(fxg_ctx* c, fxg_job* const* jobs, int njobs, fxg_batch** out) {
+  if (!c || !jobs || njobs < 1 || !out) return set_err(FXG_EINVAL, "null argument");
+  for (int k = 0; k < njobs; k++) {
+    if (!jobs[k] || jobs[k]->ctx != c) return set_err(FXG_EINVAL, "every job must belong to the context");
+    for (int m = 0; m < k; m++)
+      if (jobs[m] == jobs[k]) return set_err(FXG_EINVAL, "a job appears twice in the batch");
+  }
+  HIPCHK(hipSetDevice(c->device));
+  fxg_batch* b = new fxg_batch();
+  b->ctx = c;
+  b->jobs.assign(jobs, jobs + njobs);
+  b->join.assign(njobs, nullptr);
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; ok && k < njobs; k++) ok = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) { batch_free(b); return set_err(FXG_EHIP, "batch stream / events"); }
+  *out = b;
+  return 0;
+}
+
+int fxg_batch_destroy(fxg_batch* b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  return 0;
+}
+
+int fxg_batch_graph_captures(const fxg_batch* b, int64_t* captures) {
+  if (!b || !captures) return set_err(FXG_EINVAL, "null argument");
+  *captures = b->captures;
+  return 0;
+}
+
+int fxg_batch_process(fxg_batch* b, const fxg_subint_desc* subs, const fxg_station_input* const* stations,
+                      int data_on_device, float* const* results, int results_on_device) {
+  if (!b || !subs || !stations || !results) return set_err(FXG_EINVAL, "null argument");
+  const int n = (int)b->jobs.size();
+  for (int k = 0; k < n; k++) {
+    if (!stations[k] || !results[k]) return set_err(FXG_EINVAL, "null station or result array");
+    if (b->jobs[k]->profiling) return set_err(FXG_ESTATE, "stage profiling is not available inside a batch");
+  }
+  HIPCHK(hipSetDevice(b->ctx->device));
+  hipStream_t s = b->stream;
+  int rc;
+  // work already queued on the jobs' own streams comes first
+  for (int k = 0; k < n; k++) {
+    HIPCHK(hipEventRecord(b->join[k], b->jobs[k]->stream));
+    HIPCHK(hipStreamWaitEvent(s, b->join[k], 0));
+  }
+  std::vector<LaunchShape> sh(n);
+  std::vector<float*> dres(n);
+  std::vector<unsigned char> key;
+  for (int k = 0; k < n; k++) {
+    fxg_job* j = b->jobs[k];
+    j->prepared_key.clear();
+    if ((rc = upload_subint(j, &subs[k], stations[k], data_on_device, s, &sh[k]))) return rc;
+    dres[k] = results_on_device ? results[k] : j->d_results.as<float>();
+    std::vector<unsigned char> kk = graph_key(j, sh[k], dres[k]);
+    key.insert(key.end(), kk.begin(), kk.end());
+  }
+  if (!b->exec || key != b->key) {
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    b->key.clear();
+    rc = capture(s, &b->exec, [&]() -> int {
+      HIPCHK(hipEventRecord(b->fork, s));
+      for (int k = 0; k < n; k++) {
+        fxg_job* j = b->jobs[k];
+        HIPCHK(hipStreamWaitEvent(j->stream, b->fork, 0));
+        int r = launch_front(j, sh[k], j->stream);
+        if (!r) r = launch_back(j, dres[k], j->stream);
+        // the branch is joined even after a failed launch, so the capture ends cleanly
+        HIPCHK(hipEventRecord(b->join[k], j->stream));
+        HIPCHK(hipStreamWaitEvent(s, b->join[k], 0));
+        if (r) return r;
+      }
+      return 0;
+    });
+    if (rc) return rc;
+    b->key = key;
+    b->captures++;
+  }
+  HIPCHK(hipGraphLaunch(b->exec, s));
+  for (int k = 0; k < n; k++) {
+    fxg_job* j = b->jobs[k];
+    if (!results_on_device)
+      HIPCHK(hipMemcpyAsync(results[k], dres[k], sizeof(float) * 2 * (size_t)j->plan.info.result_length,
+                            hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  for (fxg_job* j : b->jobs) {
+    j->tap_chunk0 = ((j->desc.blockspersend - 1) / j->plan.chunk_blocks) * j->plan.chunk_blocks;
+    j->tap_chunkn = j->desc.blockspersend - j->tap_chunk0;
+    j->have_subint = 1;
+  }
   return 0;
 }
 

@@ -49,6 +49,9 @@ struct DevStationSub {
   long long ctbase;     // byte offset of band 0's stream in the corner-turn buffer
   long long ctstride;   // bytes between consecutive bands' streams (multiple of 4)
   double interp[3];
+  // the subintegration's time (procslots[].offsets, Model::getScanStartSec),
+  // the same in every datastream's entry
+  int offsec, offns, scanstart, pad;
 };
 
 // ---- per block, per datastream (Mode::process prologue result) -----------
@@ -61,11 +64,6 @@ struct DevBlock {
   double a, b, fracwall;
   double c;             // order 2: constant term minus integerdelay; order 0: delay at the block centre
   double wall;          // walltimesecs of the block start
-};
-
-// ---- subint-wide scalars ---------------------------------------------------
-struct DevSubint {
-  int offsec, offns, scanstart, nblocks;
 };
 
 // ---- result scatter segments -----------------------------------------------

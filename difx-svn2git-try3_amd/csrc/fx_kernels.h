@@ -66,7 +66,7 @@ hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsi
                              long long maxframes, int maxdatabytes, int fastW, long long maxsamples,
                              hipStream_t stream);
 hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
-                           const int2* ftab, DevSubint sub, DevBlock* out, int nst, hipStream_t stream);
+                           const int2* ftab, int nblocks, DevBlock* out, int nst, hipStream_t stream);
 hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
 hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const int2* ftab, const float* lut8,
                          long long first, int nsamp, int nchan, long long validend, float* out,
@@ -80,6 +80,8 @@ hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* z
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream);
 hipError_t launch_pcal(const PcalArgs& a, int npcal, hipStream_t stream);
+// p[0, n) = 0 with a kernel (graph-safe clearing)
+hipError_t launch_zero(float* p, long long n, hipStream_t stream);
 hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w,
                           float vscale, float* res, hipStream_t stream);
 

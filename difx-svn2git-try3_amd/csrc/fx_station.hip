@@ -105,11 +105,11 @@ __device__ __forceinline__ long long valid_in_frame(const DevStation& S, int2 z,
 // ---------------------------------------------------------------------------
 __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSub* __restrict__ ss,
                            const int* __restrict__ flags, const int2* __restrict__ ftab,
-                           DevSubint sub, DevBlock* __restrict__ out) {
+                           int nblocks, DevBlock* __restrict__ out) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   const int index = blockIdx.x * blockDim.x + threadIdx.x;
-  if (index >= sub.nblocks) return;
+  if (index >= nblocks) return;
   const DevStation S = st[s];
   const DevStationSub Q = ss[s];
   DevBlock g;
@@ -121,23 +121,23 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
     int w = index / FXG_FLAGS_PER_INT;
     ok = w < Q.nflags && ((flags[Q.flagoff + w] >> (index % FXG_FLAGS_PER_INT)) & 1);
   }
-  if (!ok) { g.status = 1; out[(long long)s * sub.nblocks + index] = g; return; }
+  if (!ok) { g.status = 1; out[(long long)s * nblocks + index] = g; return; }
   const double* ip = Q.interp;
   const int fftchannels = S.L;
   double fftcentre = index + 0.5;
   double averagedelay = ip[0] * fftcentre * fftcentre + ip[1] * fftcentre + ip[2];
   double fftstartmicrosec = index * fftchannels * S.sampletime;
-  double starttime = (sub.offsec - Q.datasec) * 1000000.0 +
-                     ((long long)sub.offns - (long long)Q.datans) / 1000.0 + fftstartmicrosec - averagedelay;
+  double starttime = (Q.offsec - Q.datasec) * 1000000.0 +
+                     ((long long)Q.offns - (long long)Q.datans) / 1000.0 + fftstartmicrosec - averagedelay;
   int nearestsample = (int)(starttime / S.sampletime + 0.5);
-  double walltimesecs = (sub.scanstart + sub.offsec) + sub.offns / 1.0e9 + fftstartmicrosec / 1.0e6;
+  double walltimesecs = (Q.scanstart + Q.offsec) + Q.offns / 1.0e9 + fftstartmicrosec / 1.0e6;
   g.intwall = (int)walltimesecs;
   g.fracwall = walltimesecs - g.intwall;
   g.wall = walltimesecs;
   g.nearestsample = nearestsample;
   if (nearestsample < -1 ||
       ((((long long)nearestsample + fftchannels) / 1) * S.bpbnum) / S.bpbden > Q.nbytes) {
-    g.status = 2; out[(long long)s * sub.nblocks + index] = g; return;
+    g.status = 2; out[(long long)s * nblocks + index] = g; return;
   }
   if (nearestsample == -1) nearestsample = 0;
   g.nearestsample = nearestsample;
@@ -169,7 +169,7 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   }
   g.weight = (float)good / (float)S.L;
   g.allvalid = good == S.L ? 1 : 0;
-  if (!(g.weight > 0.0f)) { g.status = 3; out[(long long)s * sub.nblocks + index] = g; return; }
+  if (!(g.weight > 0.0f)) { g.status = 3; out[(long long)s * nblocks + index] = g; return; }
   double nearestsampletime = nearestsample * S.sampletime;
   g.fse = (float)(starttime - nearestsampletime);
   if (S.order == 0) {                        // post-F (mode.cpp:791-793)
@@ -192,7 +192,7 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
     c -= integerdelay;
     g.a = a; g.b = b; g.c = c; g.integerdelay = integerdelay;
   }
-  out[(long long)s * sub.nblocks + index] = g;
+  out[(long long)s * nblocks + index] = g;
 }
 
 // ---------------------------------------------------------------------------
@@ -985,9 +985,9 @@ hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ft
 }
 
 hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
-                           const int2* ftab, DevSubint sub, DevBlock* out, int nst, hipStream_t stream) {
-  dim3 grid((unsigned)((sub.nblocks + 127) / 128), nst);
-  hipLaunchKernelGGL(k_prologue, grid, dim3(128), 0, stream, st, ss, flags, ftab, sub, out);
+                           const int2* ftab, int nblocks, DevBlock* out, int nst, hipStream_t stream) {
+  dim3 grid((unsigned)((nblocks + 127) / 128), nst);
+  hipLaunchKernelGGL(k_prologue, grid, dim3(128), 0, stream, st, ss, flags, ftab, nblocks, out);
   return hipGetLastError();
 }
 

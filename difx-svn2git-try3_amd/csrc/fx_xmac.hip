@@ -195,6 +195,23 @@ hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// p[i] = 0 for i < n: four floats per thread where aligned
+__global__ __launch_bounds__(256) void k_zero(float* __restrict__ p, long long n) {
+  const long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (i + 3 < n && ((uintptr_t)(p + i) & 15) == 0) {
+    *(float4*)(p + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    for (long long k = i; k < n && k < i + 4; k++) p[k] = 0.0f;
+  }
+}
+
+hipError_t launch_zero(float* p, long long n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = (n + 1023) / 1024;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream) {
   int n = nbl + nds;

@@ -318,6 +318,13 @@ def load_gpu_lib():
     lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]
     lib.fxg_tap_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.fxg_job_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.fxg_job_graph_captures.argtypes = [C.c_void_p, P(C.c_int64)]
+    lib.fxg_batch_create.argtypes = [C.c_void_p, P(C.c_void_p), C.c_int, P(C.c_void_p)]
+    lib.fxg_batch_destroy.argtypes = [C.c_void_p]
+    lib.fxg_batch_process.argtypes = [C.c_void_p, P(SubintDesc), P(P(StationInput)), C.c_int,
+                                      P(C.c_void_p), C.c_int]
+    lib.fxg_batch_graph_captures.argtypes = [C.c_void_p, P(C.c_int64)]
     _gpu_lib = lib
     return lib
 

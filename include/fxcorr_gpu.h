@@ -310,6 +310,28 @@ int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
  * 0 (default) stages them through registers. */
 int fxg_job_set_xmac_streaming(fxg_job* job, int on);
 
+/* 1: a subintegration is one graph launch (captured on the first call and
+ * again whenever the launch shape changes: data sizes, result buffer);
+ * 0 (default): the kernels are launched one by one.  Stage profiling
+ * launches one by one either way.  captures counts the graphs captured. */
+int fxg_job_set_graph(fxg_job* job, int on);
+int fxg_job_graph_captures(const fxg_job* job, int64_t* captures);
+
+/* ---- batches: one subintegration of several jobs in one graph launch ---
+ * No counterpart in the reference (one configuration per Core slot): the
+ * jobs' launch sequences are branches of one graph forked from the batch's
+ * stream, so small jobs overlap on the device.  stations[k] and results[k]
+ * are job k's station array and result buffer (layout of job k).  A job in
+ * a batch must not be used elsewhere while fxg_batch_process runs; the call
+ * returns when every result is complete. */
+typedef struct fxg_batch fxg_batch;
+int fxg_batch_create(fxg_ctx* ctx, fxg_job* const* jobs, int njobs, fxg_batch** batch);
+int fxg_batch_destroy(fxg_batch* batch);
+int fxg_batch_process(fxg_batch* batch, const fxg_subint_desc* subs,
+                      const fxg_station_input* const* stations, int data_on_device,
+                      float* const* results, int results_on_device);
+int fxg_batch_graph_captures(const fxg_batch* batch, int64_t* captures);
+
 /* Debug taps valid after fxg_subint_process with keep_taps set. */
 int fxg_job_set_keep_taps(fxg_job* job, int keep);
 int fxg_tap_block_info(fxg_job* job, int datastream, int block, fxg_block_info* out);
