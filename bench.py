@@ -80,8 +80,9 @@ class Job:
               "fxg_subint_process")
 
     def stage_times(self, reset=True):
-        ms = (C.c_double * 8)()
-        n = (C.c_int64 * 8)()
+        from fxcorr import STAGE_NAMES
+        ms = (C.c_double * len(STAGE_NAMES))()
+        n = (C.c_int64 * len(STAGE_NAMES))()
         self.lib.fxg_job_stage_times(self.h, ms, n, 1 if reset else 0)
         return list(ms), list(n)
 
@@ -89,12 +90,25 @@ class Job:
         self.lib.fxg_job_destroy(self.h)
 
 
-def time_job(lib, ctx, spec, sd, raws, steps, warmup, torch):
+def time_job(lib, ctx, spec, sd, raws, steps, warmup, torch, autotune=False, graph=False):
     from fxsynth import station_inputs_device
+    from fxcorr import Schedule, TuneResult, check
     job = Job(lib, ctx, spec)
     sub = sd.subint_desc()
     st, keep = station_inputs_device(sd, raws)
     dres = torch.empty(2 * job.info.result_length, dtype=torch.float32, device="cuda:0")
+    tuning = None
+    if autotune:
+        res = (TuneResult * 32)()
+        nres = C.c_int()
+        check(lib, lib.fxg_job_autotune(job.h, C.byref(sub), st, 1, 1, res, 32, C.byref(nres)), "autotune")
+        tuning = [{"schedule": res[i].schedule.as_dict(), "ms": res[i].ms, "rel_err": res[i].rel_err,
+                   "accepted": bool(res[i].accepted)} for i in range(min(nres.value, 32))]
+        log(f"  autotune: {len(tuning)} schedules tried")
+    sched = Schedule()
+    check(lib, lib.fxg_job_get_schedule(job.h, C.byref(sched)), "get_schedule")
+    if graph:
+        check(lib, lib.fxg_job_set_graph(job.h, 1), "set_graph")
     for _ in range(warmup):
         job.run(sub, st, dres.data_ptr())
     torch.cuda.synchronize()
@@ -115,6 +129,7 @@ def time_job(lib, ctx, spec, sd, raws, steps, warmup, torch):
     info = job.info
     job.close()
     del dres
+    time_job.schedule = {"schedule": sched.as_dict(), "tuning": tuning, "graph": graph}
     return el, lat, ms, n, info
 
 
@@ -347,6 +362,9 @@ def main():
     ap.add_argument("--load", type=float, default=1.5, help="serving: offered load as a multiple of real time")
     ap.add_argument("--duration", type=float, default=8.0, help="serving: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=2, help="serving: concurrent slots per job")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the XMAC schedule and pass length with the library's autotuner first")
+    ap.add_argument("--graph", action="store_true", help="run each subintegration as one graph launch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -395,10 +413,12 @@ def main():
     samples = spec.nstations * spec.nrecbands * spec.blockspersend * spec.fftchannels
 
     log(f"timing {args.precision}: {args.warmup} warmup + {args.steps} steps")
-    el, lat, ms, n, info = time_job(lib, ctx, spec, sd, raws, args.steps, args.warmup, torch)
+    el, lat, ms, n, info = time_job(lib, ctx, spec, sd, raws, args.steps, args.warmup, torch,
+                                    autotune=args.autotune, graph=args.graph)
+    launch = time_job.schedule
     value = samples * args.steps / el / 1e9
     log(f"  {value:.1f} Gsamples/s  ({el / args.steps * 1e3:.1f} ms/subint)")
-    stage = {fxcorr.STAGE_NAMES[i]: {"ms": ms[i], "launches": n[i]} for i in range(8) if n[i]}
+    stage = {fxcorr.STAGE_NAMES[i]: {"ms": ms[i], "launches": n[i]} for i in range(len(ms)) if n[i]}
 
     fp32 = None
     if not args.no_fp32 and args.precision != "fp32":
@@ -407,7 +427,7 @@ def main():
         el32, lat32, ms32, n32, info32 = time_job(lib, ctx, spec32, sd, raws, max(1, args.steps // 2), 1, torch)
         v32 = samples * max(1, args.steps // 2) / el32 / 1e9
         fp32 = {"value": v32, "ms_per_step": el32 / max(1, args.steps // 2) * 1e3,
-                "stages_ms": {fxcorr.STAGE_NAMES[i]: ms32[i] for i in range(8) if n32[i]}}
+                "stages_ms": {fxcorr.STAGE_NAMES[i]: ms32[i] for i in range(len(ms32)) if n32[i]}}
         log(f"  fp32 {v32:.1f} Gsamples/s")
 
     # ---- roofline of the dominant kernel (from the profiled step)
@@ -481,7 +501,7 @@ def main():
                                f"{args.blocks} FFT blocks of 8192 per subint ({args.blocks * 8192 / 128e6 * 1e3:.0f} ms)",
                    "stations": spec.nstations, "subbands": spec.nsubbands, "pols": spec.npol,
                    "channels": spec.nchan, "blocks_per_subint": spec.blockspersend,
-                   "xmac": args.precision, "fft": "fp32"},
+                   "xmac": args.precision, "fft": "fp32", "launch": launch},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stages_ms": {k: v["ms"] for k, v in stage.items()},
         "plain_fp32": fp32,

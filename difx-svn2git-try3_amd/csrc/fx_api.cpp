@@ -91,6 +91,8 @@ struct fxg_job {
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
   DBuf d_zero;                    // 64 zero bytes: the rows past the data for streaming loads
   int xmac_streaming = 0;         // direct-to-LDS MFMA XMAC where the layout allows
+  int xmac_ct = 0, xmac_tb = 0;   // FP32 XMAC channel tile / LDS stage (0: automatic)
+  int chunk_blocks = 0;           // blocks per F/X pass (<= the plan's; 0: the plan's)
   // the subintegration as one graph launch, captured per launch shape
   int use_graph = 0;
   hipGraphExec_t graph_exec = nullptr;
@@ -496,7 +498,7 @@ int fxg_job_plan_info(const fxg_job* j, fxg_plan_info* info) {
   return 0;
 }
 
-int fxg_job_set_xmac_streaming(fxg_job* j, int on) {
+int fxg_job_set_xmac_streaming(fxg_job* j, int on) {   // one entry of the schedule
   if (!j) return set_err(FXG_EINVAL, "null job");
   j->xmac_streaming = on ? 1 : 0;
   return 0;
@@ -521,16 +523,19 @@ int fxg_job_set_keep_taps(fxg_job* j, int keep) {
 }
 
 // XMAC tiling for nin inputs: CT channels per workgroup, 8x8 tiles
-static void xmac_shape(int nin, int nchan, XmacArgs* a) {
+// (ct_req, tb_req: a schedule's choice, used where it fits; 0 = automatic)
+static void xmac_shape(int nin, int nchan, XmacArgs* a, int ct_req = 0, int tb_req = 0) {
   a->G = (nin + 7) / 8;
   a->ntiles = a->G * (a->G + 1) / 2;
   int ct = 16;
   while (ct > 1 && ct * a->ntiles > 576) ct >>= 1;
   while (ct > 1 && ct > nchan) ct >>= 1;
+  if (ct_req > 0 && (ct_req & (ct_req - 1)) == 0 && ct_req * a->ntiles <= 576 && ct_req <= std::max(1, nchan)) ct = ct_req;
   a->CT = ct;
   int per_block = a->G * 8 * ct * (int)sizeof(float2);
-  int tb = std::max(1, 65536 / per_block);
-  a->TB = std::min(tb, 16);
+  int tb = std::min(std::max(1, 65536 / per_block), 16);
+  if (tb_req > 0 && tb_req * per_block <= 65536) tb = tb_req;
+  a->TB = tb;
 }
 
 // What the launches of one subintegration depend on besides the job's fixed
@@ -676,10 +681,11 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
   int rc;
   const int nblocks = D.blockspersend;
   const int prec = D.xmac_precision;
+  const int chunk = j->chunk_blocks > 0 ? std::min(j->chunk_blocks, P.chunk_blocks) : P.chunk_blocks;
   FStageArgs fa;
   fill_fstage_args(j, &fa);
-  for (int c0 = 0; c0 < nblocks; c0 += P.chunk_blocks) {
-    int cn = std::min(P.chunk_blocks, nblocks - c0);
+  for (int c0 = 0; c0 < nblocks; c0 += chunk) {
+    int cn = std::min(chunk, nblocks - c0);
     fa.chunk0 = c0;
     if ((rc = j->ev_begin(FXG_STAGE_FSTAGE))) return rc;
     if (j->nb2) {
@@ -700,7 +706,7 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       xa.nin = (int)xf.in_station.size();
       xa.nchan = xf.nchan;
       xa.nblocks = cn;
-      xmac_shape(xa.nin, xa.nchan, &xa);
+      xmac_shape(xa.nin, xa.nchan, &xa, j->xmac_ct, j->xmac_tb);
       xa.V = j->d_V.as<float2>() + xf.voff;
       xa.accumulate = c0 > 0;
       if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
@@ -752,7 +758,8 @@ static std::vector<unsigned char> graph_key(const fxg_job* j, const LaunchShape&
   put(&dres, sizeof(dres));
   const void* ptrs[4] = {j->d_ss.p, j->d_flags.p, j->d_ftab.p, j->d_ct.p};
   put(ptrs, sizeof(ptrs));
-  put(&j->xmac_streaming, sizeof(int));
+  const int sched[4] = {j->xmac_streaming, j->xmac_ct, j->xmac_tb, j->chunk_blocks};
+  put(sched, sizeof(sched));
   return k;
 }
 
@@ -798,8 +805,6 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
       j->graph_captures++;
     }
     HIPCHK(hipGraphLaunch(j->graph_exec, s));
-    j->tap_chunk0 = ((j->desc.blockspersend - 1) / P.chunk_blocks) * P.chunk_blocks;
-    j->tap_chunkn = j->desc.blockspersend - j->tap_chunk0;
   } else {
     if ((rc = launch_front(j, sh, s))) return rc;
     if ((rc = launch_back(j, dres, s))) return rc;
@@ -811,6 +816,123 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
     if ((rc = j->ev_end())) return rc;
   }
   j->have_subint = 1;
+  return 0;
+}
+
+int fxg_job_set_schedule(fxg_job* j, const fxg_schedule* sc) {
+  if (!j || !sc) return set_err(FXG_EINVAL, "null argument");
+  if (sc->xmac_ct < 0 || sc->xmac_tb < 0 || sc->chunk_blocks < 0) return set_err(FXG_EINVAL, "negative schedule entry");
+  j->xmac_streaming = sc->xmac_streaming ? 1 : 0;
+  j->xmac_ct = sc->xmac_ct;
+  j->xmac_tb = sc->xmac_tb;
+  j->chunk_blocks = sc->chunk_blocks;
+  return 0;
+}
+
+int fxg_job_get_schedule(const fxg_job* j, fxg_schedule* sc) {
+  if (!j || !sc) return set_err(FXG_EINVAL, "null argument");
+  sc->xmac_streaming = j->xmac_streaming;
+  sc->xmac_ct = j->xmac_ct;
+  sc->xmac_tb = j->xmac_tb;
+  sc->chunk_blocks = j->chunk_blocks;
+  return 0;
+}
+
+// Candidate schedules one dimension at a time around `base`: the XMAC
+// staging or tile first, then the pass length.
+static std::vector<fxg_schedule> xmac_candidates(const fxg_job* j, const fxg_schedule& base) {
+  const Plan& P = j->plan;
+  std::vector<fxg_schedule> out;
+  bool mfma = false, f32 = false;
+  for (const XmacFreq& xf : P.xf) {
+    const int nin = (int)xf.in_station.size();
+    if (j->desc.xmac_precision != FXG_XMAC_FP32 && xmac_mfma_supported(nin, xf.nchan)) mfma = true;
+    else f32 = true;
+  }
+  if (mfma) {
+    for (int st = 0; st < 2; st++) { fxg_schedule c = base; c.xmac_streaming = st; out.push_back(c); }
+  }
+  if (f32) {
+    for (int ct : {0, 4, 8, 16})
+      for (int tb : {0, 4, 8, 16}) {
+        if ((ct == 0) != (tb == 0)) continue;
+        fxg_schedule c = base; c.xmac_ct = ct; c.xmac_tb = tb; out.push_back(c);
+      }
+  }
+  if (out.empty()) out.push_back(base);
+  return out;
+}
+
+static std::vector<fxg_schedule> chunk_candidates(const fxg_job* j, const fxg_schedule& base) {
+  std::vector<fxg_schedule> out;
+  const int full = j->plan.chunk_blocks;
+  for (int cb : {0, 1024, 256, 64, 16}) {
+    if (cb > 0 && cb >= full) continue;
+    fxg_schedule c = base; c.chunk_blocks = cb; out.push_back(c);
+  }
+  return out;
+}
+
+int fxg_job_autotune(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st, int data_on_device,
+                     int reps, fxg_tune_result* results, int max_results, int* nresults) {
+  if (!j || !sub || !st) return set_err(FXG_EINVAL, "null argument");
+  if (reps < 1) reps = 1;
+  HIPCHK(hipSetDevice(j->ctx->device));
+  hipStream_t s = j->stream;
+  int rc;
+  const int saved_prof = j->profiling;
+  j->profiling = 0;
+  j->prepared_key.clear();
+  LaunchShape sh;
+  if ((rc = upload_subint(j, sub, st, data_on_device, s, &sh))) { j->profiling = saved_prof; return rc; }
+  const size_t nres = 2 * (size_t)j->plan.info.result_length;
+  std::vector<float> ref, cur(nres);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(FXG_EHIP, "hipEventCreate"); }
+  std::vector<fxg_tune_result> tried;
+  fxg_schedule best{};
+  fxg_job_get_schedule(j, &best);
+  double best_ms = 1e300;
+  auto run = [&](const fxg_schedule& c, fxg_tune_result* r) -> int {
+    fxg_job_set_schedule(j, &c);
+    float* dres = j->d_results.as<float>();
+    int q;
+    if ((q = launch_front(j, sh, s)) || (q = launch_back(j, dres, s))) return q;     // untimed
+    HIPCHK(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; k++)
+      if ((q = launch_front(j, sh, s)) || (q = launch_back(j, dres, s))) return q;
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipMemcpyAsync(cur.data(), dres, sizeof(float) * nres, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    r->schedule = c;
+    r->ms = ms / reps;
+    if (ref.empty()) ref = cur;
+    double num = 0, den = 0;
+    for (size_t i = 0; i < nres; i++) { double d = (double)cur[i] - ref[i]; num += d * d; den += (double)ref[i] * ref[i]; }
+    r->rel_err = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+    r->accepted = r->rel_err <= 1e-5;
+    r->pad = 0;
+    return 0;
+  };
+  for (int phase = 0; phase < 2 && !rc; phase++) {
+    std::vector<fxg_schedule> cand = phase == 0 ? xmac_candidates(j, best) : chunk_candidates(j, best);
+    for (const fxg_schedule& c : cand) {
+      fxg_tune_result r{};
+      if ((rc = run(c, &r))) break;
+      tried.push_back(r);
+      if (r.accepted && r.ms < best_ms) { best_ms = r.ms; best = c; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  j->profiling = saved_prof;
+  fxg_job_set_schedule(j, &best);
+  if (rc) return rc;
+  if (nresults) *nresults = (int)tried.size();
+  for (int i = 0; results && i < (int)tried.size() && i < max_results; i++) results[i] = tried[i];
   return 0;
 }
 
@@ -982,11 +1104,7 @@ int fxg_batch_process(fxg_batch* b, const fxg_subint_desc* subs, const fxg_stati
                             hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
-  for (fxg_job* j : b->jobs) {
-    j->tap_chunk0 = ((j->desc.blockspersend - 1) / j->plan.chunk_blocks) * j->plan.chunk_blocks;
-    j->tap_chunkn = j->desc.blockspersend - j->tap_chunk0;
-    j->have_subint = 1;
-  }
+  for (fxg_job* j : b->jobs) j->have_subint = 1;
   return 0;
 }
 

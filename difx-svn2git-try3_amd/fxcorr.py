@@ -101,6 +101,19 @@ class SubintDesc(C.Structure):
                 ("scan_start_sec", C.c_int32), ("model_ok", C.c_int32), ("pad", C.c_int32)]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("xmac_streaming", C.c_int32), ("xmac_ct", C.c_int32), ("xmac_tb", C.c_int32),
+                ("chunk_blocks", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("schedule", Schedule), ("ms", C.c_double), ("rel_err", C.c_double),
+                ("accepted", C.c_int32), ("pad", C.c_int32)]
+
+
 class BlockInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("nearestsample", C.c_int32), ("integerdelay", C.c_int32),
                 ("weight", C.c_float), ("fracsampleerror", C.c_float), ("intwalltime", C.c_int32),
@@ -318,6 +331,10 @@ def load_gpu_lib():
     lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]
     lib.fxg_tap_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.fxg_job_set_schedule.argtypes = [C.c_void_p, P(Schedule)]
+    lib.fxg_job_get_schedule.argtypes = [C.c_void_p, P(Schedule)]
+    lib.fxg_job_autotune.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int, C.c_int,
+                                     P(TuneResult), C.c_int, P(C.c_int)]
     lib.fxg_job_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_graph_captures.argtypes = [C.c_void_p, P(C.c_int64)]
     lib.fxg_batch_create.argtypes = [C.c_void_p, P(C.c_void_p), C.c_int, P(C.c_void_p)]

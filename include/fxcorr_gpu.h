@@ -310,6 +310,38 @@ int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
  * 0 (default) stages them through registers. */
 int fxg_job_set_xmac_streaming(fxg_job* job, int on);
 
+/* ---- launch schedules and the autotuner --------------------------------
+ * No counterpart in the reference (its FFT and XMAC are library calls with
+ * fixed strategies).  A schedule picks, per job, the reduced-precision XMAC's
+ * staging (through registers or streamed into LDS with direct loads), the
+ * FP32 XMAC's tile (channels per workgroup, blocks per LDS stage) and the
+ * blocks per F/X pass (spectra resident between the F stage and the XMAC).
+ * Every schedule computes the same products; FP32 sums grouped differently
+ * by a different pass length agree to rounding. */
+typedef struct {
+  int32_t xmac_streaming;  /* 0 register staging, 1 direct-to-LDS streaming    */
+  int32_t xmac_ct;         /* FP32 XMAC channels per workgroup, 0 = automatic  */
+  int32_t xmac_tb;         /* FP32 XMAC blocks per LDS stage, 0 = automatic    */
+  int32_t chunk_blocks;    /* blocks per F/X pass, 0 = the plan's chunk_blocks */
+} fxg_schedule;
+
+typedef struct {
+  fxg_schedule schedule;
+  double  ms;              /* mean device time of the subintegration's kernels */
+  double  rel_err;         /* ||result - first candidate's|| / ||first's||     */
+  int32_t accepted;        /* rel_err within 1e-5                              */
+  int32_t pad;
+} fxg_tune_result;
+
+int fxg_job_set_schedule(fxg_job* job, const fxg_schedule* schedule);
+int fxg_job_get_schedule(const fxg_job* job, fxg_schedule* schedule);
+/* Times the candidate schedules on one subintegration (reps timed runs each,
+ * after one untimed), keeps the fastest accepted one on the job and reports
+ * up to max_results candidates in order of trial. */
+int fxg_job_autotune(fxg_job* job, const fxg_subint_desc* sub, const fxg_station_input* stations,
+                     int data_on_device, int reps, fxg_tune_result* results, int max_results,
+                     int* nresults);
+
 /* 1: a subintegration is one graph launch (captured on the first call and
  * again whenever the launch shape changes: data sizes, result buffer);
  * 0 (default): the kernels are launched one by one.  Stage profiling

@@ -105,3 +105,31 @@ def test_batch_equals_each_job_alone(gpu):
         lib.fxg_batch_destroy(b)
         for h in hs:
             h.close()
+
+
+@pytest.mark.parametrize("name", ["c2", "fp16_pcal"])
+def test_autotune_keeps_results(gpu, name):
+    """Every schedule the tuner tries computes the same result (within 1e-5);
+    the one it keeps runs eagerly and as a graph to the same output."""
+    lib, ctx = gpu
+    spec = SPECS[name]
+    job, keep = spec.build()
+    g = GpuJob(lib, ctx, job)
+    sd = data(spec, 7)
+    base = g.process(sd)
+    sub = sd.subint_desc()
+    st, keep2 = sd.station_inputs()
+    res = (fxcorr.TuneResult * 32)()
+    n = C.c_int()
+    check(lib, lib.fxg_job_autotune(g.h, C.byref(sub), st, 0, 2, res, 32, C.byref(n)), "autotune")
+    assert n.value >= 2
+    for i in range(n.value):
+        assert res[i].accepted and res[i].rel_err <= 1e-5 and res[i].ms > 0, res[i].schedule.as_dict()
+    sched = fxcorr.Schedule()
+    check(lib, lib.fxg_job_get_schedule(g.h, C.byref(sched)), "get_schedule")
+    tuned = g.process(sd)
+    rel = np.linalg.norm(tuned - base) / np.linalg.norm(base)
+    assert rel <= 1e-5, (sched.as_dict(), rel)
+    check(lib, lib.fxg_job_set_graph(g.h, 1), "set_graph")
+    assert np.array_equal(g.process(sd), tuned)
+    g.close()
