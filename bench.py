@@ -343,6 +343,95 @@ def serving_bench(args):
     host.fxs_destroy(srv)
 
 
+def uneven_bench(args):
+    """Six uneven jobs (4..30 stations, 64..32768 channels, 2/8-bit, 4..16
+    subbands of 16..128 MHz, two zoom bands each) of one 0.128 s
+    subintegration each, all in one graph launch per round (fxg_batch), data
+    resident in HBM; also each job alone, one after another."""
+    import torch
+    import fxcorr
+    from fxcorr import check
+    from fxsynth import make_subint_device, station_inputs_device, uneven_workload_specs
+
+    lib = fxcorr.load_gpu_lib()
+    ctx = C.c_void_p()
+    check(lib, lib.fxg_ctx_create(0, C.byref(ctx)), "fxg_ctx_create")
+    specs = uneven_workload_specs(args.subint, xmac=args.precision)
+    rng = np.random.default_rng(args.seed)
+    jobs, keep, subs, sts, dres, samples, labels = [], [], [], [], [], [], []
+    for k, (spec, label) in enumerate(specs):
+        log(f"job {label}: {spec.blockspersend} blocks of {spec.fftchannels}")
+        delays = [int(v) for v in rng.integers(-1500, 1500, spec.nstations)]
+        sd, raws = make_subint_device(spec, seed=args.seed + k, rho=rhos_for(spec.nsubbands, args.seed + k),
+                                      delay_samples=delays)
+        j = Job(lib, ctx, spec)
+        st, kst = station_inputs_device(sd, raws)
+        out = torch.empty(2 * j.info.result_length, dtype=torch.float32, device="cuda:0")
+        if args.autotune:
+            res = (fxcorr.TuneResult * 32)()
+            n = C.c_int()
+            sub = sd.subint_desc()
+            check(lib, lib.fxg_job_autotune(j.h, C.byref(sub), st, 1, 1, res, 32, C.byref(n)), "autotune")
+        jobs.append(j)
+        keep.append((sd, raws, st, kst, out))
+        subs.append(sd.subint_desc())
+        sts.append(st)
+        dres.append(out)
+        samples.append(spec.nstations * spec.nrecbands * spec.blockspersend * spec.fftchannels)
+        labels.append(label)
+    torch.cuda.synchronize()
+    total = sum(samples)
+
+    # each job alone, one after another
+    alone = []
+    for j, sub, st, out in zip(jobs, subs, sts, dres):
+        j.run(sub, st, out.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        j.run(sub, st, out.data_ptr())
+        torch.cuda.synchronize()
+        alone.append(time.perf_counter() - t)
+    seq_value = total / sum(alone) / 1e9
+
+    # all six in one graph launch per round
+    n = len(jobs)
+    arr = (C.c_void_p * n)(*[j.h.value for j in jobs])
+    b = C.c_void_p()
+    check(lib, lib.fxg_batch_create(ctx, arr, n, C.byref(b)), "batch_create")
+    subarr = (fxcorr.SubintDesc * n)(*subs)
+    starr = (C.POINTER(fxcorr.StationInput) * n)(*[C.cast(st, C.POINTER(fxcorr.StationInput)) for st in sts])
+    resarr = (C.c_void_p * n)(*[o.data_ptr() for o in dres])
+    for _ in range(args.warmup):
+        check(lib, lib.fxg_batch_process(b, subarr, starr, 1, resarr, 1), "batch_process")
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        check(lib, lib.fxg_batch_process(b, subarr, starr, 1, resarr, 1), "batch_process")
+        lat.append(time.perf_counter() - a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = total * args.steps / el / 1e9
+    lat_ms = np.array(lat) * 1e3
+    log(f"  batch {value:.1f} Gsamples/s ({el / args.steps * 1e3:.1f} ms/round); one by one {seq_value:.1f}")
+    lib.fxg_batch_destroy(b)
+    out = {
+        "metric": "input Gsamples/s, six uneven jobs in one graph launch",
+        "value": value, "unit": "Gsamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision, "data": "synthetic, resident in HBM",
+        "config": {"workload": "6 uneven jobs, one subint of each per step",
+                   "jobs": labels, "subint_s": args.subint, "samples_per_step": total},
+        "one_by_one": {"value": seq_value, "ms": [a * 1e3 for a in alone]},
+        "realtime_rate": total / args.subint / 1e9,
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+    }
+    print(json.dumps(out), flush=True)
+    for j in jobs:
+        j.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +454,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="pick the XMAC schedule and pass length with the library's autotuner first")
     ap.add_argument("--graph", action="store_true", help="run each subintegration as one graph launch")
+    ap.add_argument("--uneven", action="store_true",
+                    help="the six-job uneven workload in one graph launch per step instead of the 32-station job")
+    ap.add_argument("--subint", type=float, default=0.128, help="uneven: subintegration length in seconds")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -374,6 +466,13 @@ def main():
             print("bench.py runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
             sys.exit(2)
         serving_bench(args)
+        return
+
+    if args.uneven:
+        if args.gpus != 1:
+            print("bench.py runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
+            sys.exit(2)
+        uneven_bench(args)
         return
 
     if args.copy_calibration:

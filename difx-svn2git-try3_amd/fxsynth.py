@@ -107,6 +107,18 @@ def vdif_header(sec: int, frameno: int, framebytes: int, nchan: int, nbits: int,
     return w.view(np.uint8)
 
 
+def vdif_headers(datasec: int, first_frameno: int, nframes: int, frames_per_sec: int, framebytes: int,
+                 nchan: int, nbits: int, station: int) -> np.ndarray:
+    """vdif_header for nframes consecutive frames: [nframes, 32] bytes."""
+    fn = first_frameno + np.arange(nframes, dtype=np.int64)
+    w = np.zeros((nframes, 8), dtype=np.uint32)
+    w[:, 0] = ((datasec + fn // frames_per_sec) & 0x3FFFFFFF).astype(np.uint32)
+    w[:, 1] = ((fn % frames_per_sec) & 0xFFFFFF).astype(np.uint32) | np.uint32(20 << 24)
+    w[:, 2] = ((framebytes // 8) & 0xFFFFFF) | ((int(math.log2(nchan)) & 0x1F) << 24)
+    w[:, 3] = (station & 0xFFFF) | (((nbits - 1) & 0x1F) << 26)
+    return w.view(np.uint8).reshape(nframes, 32)
+
+
 def mark5b_header(frameno: int, invalid: bool = False) -> np.ndarray:
     w = np.zeros(4, dtype=np.uint32)
     w[0] = 0xABADDEED
@@ -292,12 +304,9 @@ def make_subint_device(spec: JobSpec, seed: int = 1, rho=0.05, delay_samples=Non
         del codes, flat
         out = torch.empty((nframes, spec.framebytes), dtype=torch.uint8, device=device)
         out[:, spec.headerbytes:] = payload.view(nframes, databytes)
-        hdr = np.empty((nframes, spec.headerbytes), dtype=np.uint8)
-        for f in range(nframes):
-            fn = first_frameno + f
-            hdr[f] = vdif_header(datasec + fn // frames_per_sec, fn % frames_per_sec, spec.framebytes,
-                                 nchan, spec.nbits, s)[: spec.headerbytes]
-        out[:, : spec.headerbytes] = torch.from_numpy(hdr).to(device)
+        hdr = vdif_headers(datasec, first_frameno, nframes, frames_per_sec, spec.framebytes, nchan,
+                           spec.nbits, s)[:, : spec.headerbytes]
+        out[:, : spec.headerbytes] = torch.from_numpy(np.ascontiguousarray(hdr)).to(device)
         raws.append(out.view(-1))
         vf = np.zeros(nflags, dtype=np.int64)
         for b in range(spec.blockspersend):
