@@ -59,6 +59,7 @@ struct fxg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;   // for standalone calls
   DBuf lut8;
+  DBuf mux_src, mux_hdr, mux_mask, mux_off;   // VDIF multiplexing: staged input and frame tables
 };
 
 struct fxg_job {
@@ -146,6 +147,43 @@ struct fxg_job {
     evused.swap(keep);
   }
 };
+
+namespace fxg {
+
+int set_error(int code, const char* msg) { return set_err(code, msg); }
+
+// stage 2 of VDIF multiplexing on the context's stream (the frame tables
+// come from the host-side stage 1 in fx_mux.cpp)
+int ctx_mux_launch(fxg_ctx* c, const uint8_t* src, const uint8_t* src_device, long long srcsize, uint8_t* dest,
+                   const uint32_t* hdr, const unsigned long long* mask, const long long* off, long long nframes,
+                   int nthreads, int unitbits, int units, int outwords, int outframe, unsigned long long good) {
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  int rc;
+  const uint8_t* dsrc = src_device;
+  if (!dsrc) {
+    if ((rc = c->mux_src.ensure((size_t)std::max(1LL, srcsize)))) return rc;
+    HIPCHK(hipMemcpyAsync(c->mux_src.p, src, (size_t)srcsize, hipMemcpyHostToDevice, s));
+    dsrc = c->mux_src.as<uint8_t>();
+  }
+  if ((rc = c->mux_hdr.ensure(sizeof(uint32_t) * 8 * (size_t)nframes)) ||
+      (rc = c->mux_mask.ensure(sizeof(unsigned long long) * (size_t)nframes)) ||
+      (rc = c->mux_off.ensure(sizeof(long long) * (size_t)nframes * nthreads)))
+    return rc;
+  HIPCHK(hipMemcpyAsync(c->mux_hdr.p, hdr, sizeof(uint32_t) * 8 * (size_t)nframes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->mux_mask.p, mask, sizeof(unsigned long long) * (size_t)nframes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->mux_off.p, off, sizeof(long long) * (size_t)nframes * nthreads, hipMemcpyHostToDevice, s));
+  MuxArgs a{};
+  a.src = dsrc; a.dest = dest; a.srcoff = c->mux_off.as<long long>();
+  a.mask = c->mux_mask.as<unsigned long long>(); a.headers = c->mux_hdr.as<unsigned>();
+  a.nframes = (int)nframes; a.nthreads = nthreads; a.unitbits = unitbits; a.unitsper = units;
+  a.outwords = outwords; a.framebytes = outframe; a.goodmask = good;
+  HIPCHK(launch_vdifmux(a, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace fxg
 
 extern "C" {
 
@@ -252,6 +290,7 @@ int fxg_ctx_destroy(fxg_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamDestroy(c->stream);
   c->lut8.release();
+  for (DBuf* b : {&c->mux_src, &c->mux_hdr, &c->mux_mask, &c->mux_off}) b->release();
   delete c;
   return 0;
 }

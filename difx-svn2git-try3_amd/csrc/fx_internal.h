@@ -142,6 +142,12 @@ struct Plan {
 constexpr int kPcalMaxFloats = 32768;
 
 int build_plan(const fxg_job_desc* d, Plan* p, std::string* err);
+// the library's error string (fxg_last_error) and code
+int set_error(int code, const char* msg);
+// stage 2 of VDIF multiplexing (fx_api.cpp, kernel in fx_mux.hip)
+int ctx_mux_launch(fxg_ctx* c, const uint8_t* src, const uint8_t* src_device, long long srcsize, uint8_t* dest,
+                   const uint32_t* hdr, const unsigned long long* mask, const long long* off, long long nframes,
+                   int nthreads, int unitbits, int units, int outwords, int outframe, unsigned long long good);
 int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d);
 
 }  // namespace fxg

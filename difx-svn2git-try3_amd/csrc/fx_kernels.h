@@ -60,6 +60,19 @@ struct PcalArgs {
   int ngroups;                // block groups per band
 };
 
+struct MuxArgs {
+  const unsigned char* src;     // input bytes on the device
+  unsigned char* dest;          // output frames
+  const long long* srcoff;      // [frame][thread] payload byte offsets into src
+  const unsigned long long* mask;   // [frame] threads present
+  const unsigned* headers;      // [frame][8] output header words
+  int nframes, nthreads, unitbits, unitsper;   // unitsper: thread units per sample-time (power of two)
+  int outwords;                 // payload words per output frame
+  int framebytes;               // output frame bytes
+  unsigned long long goodmask;
+};
+hipError_t launch_vdifmux(const MuxArgs& a, hipStream_t stream);
+
 hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
                          long long maxframes, hipStream_t stream);
 hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,

@@ -101,6 +101,28 @@ class SubintDesc(C.Structure):
                 ("scan_start_sec", C.c_int32), ("model_ok", C.c_int32), ("pad", C.c_int32)]
 
 
+FXG_MUX_MAX_THREADS = 31
+FXG_MUX_GOTOEND, FXG_MUX_RESPECTGRANULARITY, FXG_MUX_ENABLEVALIDITY = 0x01, 0x02, 0x04
+
+
+class VdifMuxDesc(C.Structure):
+    _fields_ = [("input_frame_bytes", C.c_int32), ("frames_per_second", C.c_int32),
+                ("bits_per_sample", C.c_int32), ("nthreads", C.c_int32),
+                ("thread_ids", C.c_int32 * FXG_MUX_MAX_THREADS), ("channels_per_thread", C.c_int32),
+                ("nsort", C.c_int32), ("ngap", C.c_int32), ("flags", C.c_int32)]
+
+
+class VdifMuxStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "valid_frames", "invalid_frames", "discarded_frames", "wrong_thread_frames", "skipped_bytes",
+        "fill_bytes", "duplicate_frames", "bytes_processed", "good_frames", "partial_frames", "filler_frames",
+        "src_used", "dest_used", "output_frame_bytes", "output_frames", "start_frame_number",
+        "output_frame_granularity", "epoch")]
+
+    def as_list(self):
+        return [getattr(self, n) for n, _ in self._fields_]
+
+
 class Schedule(C.Structure):
     _fields_ = [("xmac_streaming", C.c_int32), ("xmac_ct", C.c_int32), ("xmac_tb", C.c_int32),
                 ("chunk_blocks", C.c_int32)]
@@ -331,6 +353,10 @@ def load_gpu_lib():
     lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]
     lib.fxg_tap_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.fxg_vdifmux_output_shape.argtypes = [P(VdifMuxDesc), P(C.c_int32), P(C.c_int32)]
+    lib.fxg_vdifmux.restype = C.c_int64
+    lib.fxg_vdifmux.argtypes = [C.c_void_p, P(VdifMuxDesc), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                C.c_int64, C.c_int64, P(VdifMuxStats)]
     lib.fxg_job_set_schedule.argtypes = [C.c_void_p, P(Schedule)]
     lib.fxg_job_get_schedule.argtypes = [C.c_void_p, P(Schedule)]
     lib.fxg_job_autotune.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int, C.c_int,
@@ -390,6 +416,9 @@ def load_oracle():
     _declare_common(lib, "orc_")
     lib.orc_lut.argtypes = [C.c_int, C.c_int, P(C.c_float)]
     lib.orc_calcstridelength.argtypes = [C.c_int]
+    lib.orc_vdifmux.restype = C.c_int
+    lib.orc_vdifmux.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                P(C.c_int), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, P(C.c_int64)]
     lib.orc_unpack.argtypes = [P(Datastream), C.c_void_p, C.c_int64, C.c_int64, C.c_int,
                                C.c_void_p, P(C.c_float)]
     lib.orc_block_info.argtypes = [P(JobDesc), P(SubintDesc), P(StationInput), C.c_int, C.c_int,

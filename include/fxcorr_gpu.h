@@ -310,6 +310,52 @@ int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
  * 0 (default) stages them through registers. */
 int fxg_job_set_xmac_streaming(fxg_job* job, int on);
 
+/* ---- VDIF thread multiplexing (libraries/vdifio/src/vdifmux.c) ----------
+ * Multi-thread VDIF (one or more channels per thread) becomes single-thread
+ * multi-channel VDIF that the station kernels read: configurevdifmux
+ * (vdifmux.c:118-211, setvdifmuxinputchannels :213-253) and vdifmux
+ * (:507-1044).  The frame search and ordering (stage 1) reads only headers
+ * and runs on the host; the corner turn of the payloads (stage 2,
+ * cornerturners.c) runs on the GPU into device memory.  Fanout 1, non-legacy
+ * headers, real data, no EDV4 validity propagation; an output frame missing
+ * a thread is marked invalid and its payload zeroed. */
+#define FXG_MUX_MAX_THREADS        31
+#define FXG_MUX_GOTOEND          0x01  /* VDIF_MUX_FLAG_GOTOEND            */
+#define FXG_MUX_RESPECTGRANULARITY 0x02 /* VDIF_MUX_FLAG_RESPECTGRANULARITY */
+#define FXG_MUX_ENABLEVALIDITY   0x04  /* VDIF_MUX_FLAG_ENABLEVALIDITY     */
+
+typedef struct {
+  int32_t input_frame_bytes;     /* single-thread frame, 32-byte header included   */
+  int32_t frames_per_second;     /* per thread                                     */
+  int32_t bits_per_sample;
+  int32_t nthreads;
+  int32_t thread_ids[FXG_MUX_MAX_THREADS];
+  int32_t channels_per_thread;   /* 1 unless setvdifmuxinputchannels               */
+  int32_t nsort;                 /* out-of-order frames tolerated                  */
+  int32_t ngap;                  /* frame-number gap that ends the call            */
+  int32_t flags;                 /* FXG_MUX_*                                      */
+} fxg_vdifmux_desc;
+
+/* struct vdif_mux_statistics of one call (vdifio.h:391-425) */
+typedef struct {
+  int64_t valid_frames, invalid_frames, discarded_frames, wrong_thread_frames;
+  int64_t skipped_bytes, fill_bytes, duplicate_frames, bytes_processed;
+  int64_t good_frames, partial_frames, filler_frames;
+  int64_t src_used, dest_used, output_frame_bytes, output_frames, start_frame_number;
+  int64_t output_frame_granularity, epoch;
+} fxg_vdifmux_stats;
+
+/* Output frame size and channel count (configurevdifmux). */
+int fxg_vdifmux_output_shape(const fxg_vdifmux_desc* desc, int32_t* output_frame_bytes,
+                             int32_t* output_channels);
+/* vdifmux into dest_device (dest_bytes of device memory): src_host holds the
+ * input for the header scan; src_device is the same bytes on the device, or
+ * null to have them copied in.  start_frame < 0: from the lowest frame
+ * found.  Returns the bytes of input consumed (>= 0) or an FXG_E* code. */
+int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* desc, const uint8_t* src_host,
+                    const uint8_t* src_device, int64_t src_bytes, uint8_t* dest_device,
+                    int64_t dest_bytes, int64_t start_frame, fxg_vdifmux_stats* stats);
+
 /* ---- launch schedules and the autotuner --------------------------------
  * No counterpart in the reference (its FFT and XMAC are library calls with
  * fixed strategies).  A schedule picks, per job, the reduced-precision XMAC's

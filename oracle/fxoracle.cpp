@@ -1427,3 +1427,283 @@ int orc_delay_interpolator(const double* poly, int polyorder, double offsettime,
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------------ */
+/* VDIF thread multiplexing: libraries/vdifio/src/vdifmux.c:84-212         */
+/* (configurevdifmux, setvdifmuxinputchannels), :346-410                    */
+/* (getLowestFrameNumber), :411-470 (getHighestFrameNumber), :507-1044      */
+/* (vdifmux), with the corner turners of cornerturners.c stated by their    */
+/* common rule: output sample-time t, channel unit k (nBit bits = one       */
+/* thread's channels at t) = thread k's unit t, zero for the padding units  */
+/* up to the next power of two.  One fanout, non-legacy headers, real data, */
+/* no EDV4 validity propagation.                                             */
+/* ------------------------------------------------------------------------ */
+namespace orcmux {
+
+static const uint32_t FILL = 0x11223344UL;
+static const int BAD_THREAD = 20000;
+
+struct Mux {
+  int inputFrameSize, inputFramesPerSecond, bitsPerSample, nThread, nSort, nGap, flags;
+  int inputChannelsPerThread, nOutputChan, inputDataSize, outputDataSize, outputFrameSize;
+  int frameGranularity;
+  uint64_t goodMask;
+  int chanIndex[1024];
+};
+
+static uint32_t w32(const uint8_t* p, int k) { uint32_t v; std::memcpy(&v, p + 4 * k, 4); return v; }
+static void put32(uint8_t* p, int k, uint32_t v) { std::memcpy(p + 4 * k, &v, 4); }
+static int hframebytes(const uint8_t* h) { return (int)(w32(h, 2) & 0xFFFFFF) * 8; }
+static int hnchan(const uint8_t* h) { return 1 << ((w32(h, 2) >> 24) & 0x1F); }
+static int hnbits(const uint8_t* h) { return (int)((w32(h, 3) >> 26) & 0x1F) + 1; }
+static int hinvalid(const uint8_t* h) { return (int)(w32(h, 0) >> 31); }
+static int hthread(const uint8_t* h) { return (int)((w32(h, 3) >> 16) & 0x3FF); }
+static int hseconds(const uint8_t* h) { return (int)(w32(h, 0) & 0x3FFFFFFF); }
+static int hframe(const uint8_t* h) { return (int)(w32(h, 1) & 0xFFFFFF); }
+static int hepoch(const uint8_t* h) { return (int)((w32(h, 1) >> 24) & 0x3F); }
+
+static unsigned gcdu(unsigned u, unsigned v) {      /* vdifmux.c:50-82 (the same value) */
+  while (v) { unsigned t = u % v; u = v; v = t; }
+  return u;
+}
+
+static int configure(Mux* m, int frameSize, int fps, int bits, int nThread, const int* ids, int nSort, int nGap,
+                     int flags, int chansPerThread) {
+  if (nThread < 1 || nThread > 31) return -3;
+  m->inputFrameSize = frameSize; m->inputFramesPerSecond = fps; m->bitsPerSample = bits;
+  m->inputChannelsPerThread = chansPerThread;
+  m->frameGranularity = (int)(fps / gcdu((unsigned)fps, 1000000000u));
+  m->nSort = nSort; m->nGap = nGap < nSort ? nSort : nGap;
+  m->nThread = nThread; m->flags = flags;
+  int nt = nThread, n = 1;
+  while (n < nt) n *= 2;
+  m->nOutputChan = n * chansPerThread;
+  m->inputDataSize = frameSize - 32;
+  /* configurevdifmux sets the output size before setvdifmuxinputchannels
+   * multiplies the channel count, so it counts threads only */
+  m->outputDataSize = m->inputDataSize * n;
+  m->outputFrameSize = m->outputDataSize + 32;
+  for (int i = 0; i < 1024; i++) m->chanIndex[i] = BAD_THREAD;
+  for (int i = 0; i < nThread; i++) {
+    if (ids[i] < 0 || ids[i] > 1023) return -2;
+    m->chanIndex[ids[i]] = i;
+  }
+  m->goodMask = (1ULL << nThread) - 1;
+  return 0;
+}
+
+/* skip rules shared by the frame scanners and stage 1: returns the step to
+ * the next candidate, 0 if a valid frame of the stream starts at i */
+static int skip_step(const Mux& m, const uint8_t* cur, int* kind) {
+  if (w32(cur, m.inputFrameSize / 4 - 1) == FILL) { *kind = 1; return m.inputFrameSize; }
+  if (w32(cur, 0) == FILL) { *kind = 2; return 8; }
+  if (hframebytes(cur) != m.inputFrameSize || hnchan(cur) != m.inputChannelsPerThread || hnbits(cur) != m.bitsPerSample) {
+    *kind = 3; return 4;
+  }
+  if ((m.flags & 0x04) && hinvalid(cur) > 0) { *kind = 4; return m.inputFrameSize; }
+  *kind = 0;
+  return 0;
+}
+
+static int64_t frame_number(const Mux& m, const uint8_t* h) {
+  return (int64_t)hseconds(h) * m.inputFramesPerSecond + hframe(h);
+}
+
+static int64_t lowest_frame(const uint8_t* src, int srcSize, const Mux& m) {
+  int64_t lowest = -1; int nFrame = 0;
+  for (int i = 0; i <= srcSize - m.inputFrameSize;) {
+    int kind, st = skip_step(m, src + i, &kind);
+    if (st) { i += st; continue; }
+    int64_t fn = frame_number(m, src + i);
+    if (lowest < 0 || fn < lowest) lowest = fn;
+    ++nFrame; i += m.inputFrameSize;
+    if (nFrame >= m.nSort) break;
+  }
+  return lowest;
+}
+
+static int64_t highest_frame(const uint8_t* src, int srcSize, const Mux& m) {
+  int64_t highest = -1;
+  for (int i = 0; i <= srcSize - m.inputFrameSize;) {
+    int kind, st = skip_step(m, src + i, &kind);
+    if (st) { i += st; continue; }
+    int64_t fn = frame_number(m, src + i);
+    if (highest < 0 || fn > highest) highest = fn;
+    i += m.inputFrameSize;
+  }
+  return highest;
+}
+
+/* the corner-turn rule: output unit (t, k) of nBit bits */
+static void cornerturn(const Mux& m, uint8_t* out, const std::vector<const uint8_t*>& thr) {
+  const int nBit = m.bitsPerSample * m.inputChannelsPerThread;
+  const int nUnits = m.nOutputChan / m.inputChannelsPerThread;
+  const long long totalbits = 8LL * m.outputDataSize;
+  std::memset(out, 0, m.outputDataSize);
+  for (long long ob = 0; ob < totalbits; ob += nBit) {
+    const long long u = ob / nBit;
+    const long long t = u / nUnits;
+    const int k = (int)(u % nUnits);
+    if (k >= m.nThread) continue;
+    for (int b = 0; b < nBit; b++) {
+      const long long ib = t * nBit + b;
+      const int bit = (thr[k][ib >> 3] >> (ib & 7)) & 1;
+      out[(ob + b) >> 3] |= (uint8_t)(bit << ((ob + b) & 7));
+    }
+  }
+}
+
+/* stats: [0] nValidFrame [1] nInvalidFrame [2] nDiscardedFrame [3] nWrongThread [4] nSkippedByte
+ * [5] nFillByte [6] nDuplicateFrame [7] bytesProcessed [8] nGoodFrame [9] nPartialFrame
+ * [10] nFillerFrame [11] srcUsed [12] destUsed [13] outputFrameSize [14] nOutputFrame
+ * [15] startFrameNumber [16] outputFrameGranularity [17] epoch
+ * An invalid output frame's payload is zeroed here; the reference leaves its
+ * scratch pointer table there (a decoder skips invalid frames either way). */
+static int mux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, const Mux& m, int64_t startOut,
+               int64_t* st) {
+  int nValidFrame = 0, nSkip = 0, nFill = 0, nDup = 0, nInvalidFrame = 0, nDiscard = 0, nSeqUnproc = 0;
+  int highestDestIndex = 0, bytesProcessed = -1, firstGoodByte = -1, nEnd = 0;
+  int nGoodOutput = 0, nBadOutput = 0, nPartialOutput = 0, nWrongThread = 0;
+  uint8_t outHeader[32]; std::memset(outHeader, 0, 32);
+  int epoch = -1, highestSortedDestIndex = -1, vhUnset = 1, framesIn = 0;
+  int nSort = m.nSort;
+  const int N = srcSize - m.inputFrameSize;
+  if (3 * srcSize <= 4 * nSort * m.inputFrameSize) nSort = (3 * srcSize) / (4 * m.inputFrameSize);
+  const int maxSrcIndex = (m.flags & 0x01) ? srcSize - m.inputFrameSize : srcSize - nSort * m.inputFrameSize;
+  if ((m.flags & 0x02) && startOut > 0) {
+    int64_t r = startOut % m.frameGranularity;
+    if (r != 0) startOut += m.frameGranularity - r;
+  }
+  const int maxDestIndex = destSize / m.outputFrameSize - 1;
+  int64_t startFrameNumber = startOut;
+  if (startFrameNumber < 0) {
+    startFrameNumber = lowest_frame(src, srcSize, m);
+    if (startFrameNumber < 0) return -3;
+  }
+  std::vector<uint64_t> mask(std::max(0, maxDestIndex + 1), 0);
+  std::vector<std::vector<const uint8_t*>> ptr(std::max(0, maxDestIndex + 1), std::vector<const uint8_t*>(m.nThread, nullptr));
+  int i;
+  for (i = 0; i <= N;) {
+    const uint8_t* cur = src + i;
+    int kind, stp = skip_step(m, cur, &kind);
+    if (stp) {
+      if (kind == 1) nFill += m.inputFrameSize;
+      else if (kind == 2) nFill += 8;
+      else if (kind == 3) nSkip += 4;
+      else ++nInvalidFrame;
+      i += stp;
+      continue;
+    }
+    if (firstGoodByte < 0) firstGoodByte = i;
+    const int chanId = m.chanIndex[hthread(cur)];
+    if (chanId == BAD_THREAD) { i += m.inputFrameSize; ++nWrongThread; continue; }
+    const int64_t frameNumber = frame_number(m, cur);
+    if (vhUnset) {
+      std::memcpy(outHeader, cur, 16);
+      std::memset(outHeader + 16, 0, 16);
+      uint32_t w2 = w32(outHeader, 2), w3 = w32(outHeader, 3), w0 = w32(outHeader, 0);
+      unsigned logc = 0; for (int c = m.nOutputChan; c > 1; c /= 2) logc++;
+      w2 = (w2 & ~(0x1Fu << 24)) | ((logc & 0x1F) << 24);
+      w3 = (w3 & ~(0x3FFu << 16));
+      w2 = (w2 & ~0xFFFFFFu) | ((uint32_t)(m.outputFrameSize / 8) & 0xFFFFFF);
+      w0 &= 0x7FFFFFFFu;
+      put32(outHeader, 0, w0); put32(outHeader, 2, w2); put32(outHeader, 3, w3);
+      epoch = hepoch(outHeader);
+      vhUnset = 0;
+    }
+    ++framesIn;
+    const int64_t destIndex64 = frameNumber - startFrameNumber;
+    const int destIndex = (int)destIndex64;
+    if (destIndex64 < 0) { i += m.inputFrameSize; ++nDiscard; continue; }
+    if (i > maxSrcIndex && highestSortedDestIndex < 0) highestSortedDestIndex = highestDestIndex;
+    if (destIndex64 > maxDestIndex) {
+      ++nSeqUnproc;
+      if (framesIn < nSort) { i += m.inputFrameSize; ++nDiscard; continue; }
+      if (bytesProcessed == -1) bytesProcessed = i;
+      i += m.inputFrameSize;
+      ++nEnd;
+      if (nEnd >= nSort) break;
+    } else {
+      --nSeqUnproc;
+      if (destIndex > highestDestIndex + m.nGap && m.nGap > 0) {
+        if (nValidFrame > nSort) {
+          if (bytesProcessed == -1) bytesProcessed = i;
+          ++nEnd;
+          if (nEnd >= nSort) break;
+        }
+      }
+      if (mask[destIndex] & (1ULL << chanId)) ++nDup;
+      else {
+        mask[destIndex] |= 1ULL << chanId;
+        ptr[destIndex][chanId] = cur + 32;
+        ++nValidFrame;
+        if (destIndex > highestDestIndex) highestDestIndex = destIndex;
+      }
+      i += m.inputFrameSize;
+    }
+  }
+  if (bytesProcessed == -1) bytesProcessed = i;
+  auto backoff = [&](int f) {
+    if (mask[f] != m.goodMask) {
+      highestDestIndex = f - 1;
+      for (int t = 0; t < m.nThread; ++t)
+        if (mask[f] & (1ULL << t)) {
+          int d = (int)(ptr[f][t] - src - 32);
+          if (d < bytesProcessed) bytesProcessed = d;
+          --nValidFrame;
+        }
+    }
+  };
+  if (highestSortedDestIndex >= 0) {
+    for (int f = highestDestIndex; f > highestSortedDestIndex; --f) backoff(f);
+  } else {
+    const int minDestIndex = highestDestIndex - nSort / m.nThread;
+    if (minDestIndex >= 0) for (int f = highestDestIndex; f >= minDestIndex; --f) backoff(f);
+  }
+  if (i < N && highestDestIndex < maxDestIndex && (highestDestIndex > maxDestIndex - m.nGap || m.nGap == 0))
+    highestDestIndex = maxDestIndex;
+  int seconds = (int)(startFrameNumber / m.inputFramesPerSecond);
+  int frameNum = (int)(startFrameNumber % m.inputFramesPerSecond);
+  for (int f = 0; f <= highestDestIndex; ++f) {
+    uint8_t* frame = dest + (size_t)m.outputFrameSize * f;
+    std::memcpy(frame, outHeader, 32);
+    put32(frame, 0, (w32(frame, 0) & ~0x3FFFFFFFu) | ((uint32_t)seconds & 0x3FFFFFFF));
+    put32(frame, 1, (w32(frame, 1) & ~0xFFFFFFu) | ((uint32_t)frameNum & 0xFFFFFF));
+    if (mask[f] == m.goodMask) {
+      cornerturn(m, frame + 32, ptr[f]);
+      ++nGoodOutput;
+    } else {
+      put32(frame, 0, w32(frame, 0) | 0x80000000u);
+      std::memset(frame + 32, 0, m.outputDataSize);
+      ++nBadOutput;
+    }
+    if (++frameNum >= m.inputFramesPerSecond) { ++seconds; frameNum -= m.inputFramesPerSecond; }
+  }
+  int64_t srcUsed = bytesProcessed, nOut = nGoodOutput + nBadOutput + nPartialOutput, startFN = startFrameNumber;
+  if (nOut == 0 || bytesProcessed == 0 || firstGoodByte < 0 || nSeqUnproc >= nSort) {
+    int64_t h = highest_frame(src, srcSize, m);
+    startFN = h < 0 ? startFN + (srcSize / m.inputFrameSize) / m.nThread + 1 : h + 1;
+    nOut = 0;
+    srcUsed = srcSize - (srcSize % m.inputFrameSize);
+    bytesProcessed = (int)srcUsed;
+  }
+  const int64_t vals[18] = {nValidFrame, nInvalidFrame, nDiscard, nWrongThread, nSkip, nFill, nDup,
+                            bytesProcessed, nGoodOutput, nPartialOutput, nBadOutput, srcUsed,
+                            (int64_t)(nGoodOutput + nBadOutput + nPartialOutput) * m.outputFrameSize,
+                            m.outputFrameSize, nOut, startFN, m.frameGranularity, epoch};
+  for (int k = 0; k < 18; k++) st[k] = vals[k];
+  return bytesProcessed > firstGoodByte ? bytesProcessed : firstGoodByte;
+}
+
+}  // namespace orcmux
+
+extern "C" int orc_vdifmux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, int inputFrameSize,
+                           int framesPerSecond, int bitsPerSample, int nThread, const int* threadIds, int nSort,
+                           int nGap, int flags, int channelsPerThread, int64_t startOutputFrame, int64_t* stats) {
+  if (flags & ~0x07) return -4;                      /* legacy, complex, EDV4 propagation: not restated */
+  orcmux::Mux m;
+  int r = orcmux::configure(&m, inputFrameSize, framesPerSecond, bitsPerSample, nThread, threadIds, nSort, nGap, flags,
+                            channelsPerThread);
+  if (r) return r;
+  return orcmux::mux(dest, destSize, src, srcSize, m, startOutputFrame, stats);
+}
