@@ -123,6 +123,13 @@ class VdifMuxStats(C.Structure):
         return [getattr(self, n) for n, _ in self._fields_]
 
 
+class DsFormat(C.Structure):
+    _fields_ = [("framebytes", C.c_int32), ("payloadbytes", C.c_int32), ("framespersecond", C.c_double),
+                ("framegranularity", C.c_int32), ("bytespersamplenum", C.c_int32),
+                ("bytespersampledenom", C.c_int32), ("numchannels", C.c_int32), ("blockspersend", C.c_int32),
+                ("pad", C.c_int32), ("bw_mhz", C.c_double), ("databytes", C.c_int64)]
+
+
 class Schedule(C.Structure):
     _fields_ = [("xmac_streaming", C.c_int32), ("xmac_ct", C.c_int32), ("xmac_tb", C.c_int32),
                 ("chunk_blocks", C.c_int32)]
@@ -399,6 +406,14 @@ def load_host_lib():
     lib.fxs_submit.argtypes = [C.c_void_p, C.c_int, P(SubintDesc), P(StationInput), C.c_void_p, P(C.c_int64)]
     lib.fxs_wait.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64)]
     lib.fxs_stats.argtypes = [C.c_void_p, P(ServeStats)]
+    lib.fxds_data_bytes.restype = C.c_int64
+    lib.fxds_data_bytes.argtypes = [P(JobDesc), C.c_int, C.c_double]
+    lib.fxds_format_of.argtypes = [P(JobDesc), C.c_int, C.c_double, P(DsFormat)]
+    lib.fxds_create.argtypes = [P(DsFormat), P(C.c_void_p)]
+    lib.fxds_destroy.argtypes = [C.c_void_p]
+    lib.fxds_set_buffer.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32]
+    lib.fxds_control.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                 P(StationInput), P(C.c_int32), C.c_int32, P(C.c_int64)]
     _host_lib = lib
     return lib
 
