@@ -37,8 +37,9 @@ PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 
 def profiled_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (profiles/c3_hbm_traffic.txt), or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_hbm_traffic.txt")
+    FETCH_SIZE / WRITE_SIZE summary of the 32-station workload
+    (profiles/hbm_traffic_32st_4096ch.txt), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_32st_4096ch.txt")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_dispatch = "):
@@ -372,6 +373,8 @@ def uneven_bench(args):
             n = C.c_int()
             sub = sd.subint_desc()
             check(lib, lib.fxg_job_autotune(j.h, C.byref(sub), st, 1, 1, res, 32, C.byref(n)), "autotune")
+        if args.graph:
+            check(lib, lib.fxg_job_set_graph(j.h, 1), "set_graph")
         jobs.append(j)
         keep.append((sd, raws, st, kst, out))
         subs.append(sd.subint_desc())
@@ -451,9 +454,10 @@ def main():
     ap.add_argument("--load", type=float, default=1.5, help="serving: offered load as a multiple of real time")
     ap.add_argument("--duration", type=float, default=8.0, help="serving: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=2, help="serving: concurrent slots per job")
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick the XMAC schedule and pass length with the library's autotuner first")
-    ap.add_argument("--graph", action="store_true", help="run each subintegration as one graph launch")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the library's autotuner (XMAC schedule and pass length)")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="launch the kernels one by one instead of one graph per subintegration")
     ap.add_argument("--uneven", action="store_true",
                     help="the six-job uneven workload in one graph launch per step instead of the 32-station job")
     ap.add_argument("--subint", type=float, default=0.128, help="uneven: subintegration length in seconds")
@@ -523,9 +527,11 @@ def main():
     if not args.no_fp32 and args.precision != "fp32":
         log("timing plain fp32")
         spec32 = c3_spec("fp32", args.blocks, args.chunk)
-        el32, lat32, ms32, n32, info32 = time_job(lib, ctx, spec32, sd, raws, max(1, args.steps // 2), 1, torch)
+        el32, lat32, ms32, n32, info32 = time_job(lib, ctx, spec32, sd, raws, max(1, args.steps // 2), 1, torch,
+                                                  autotune=args.autotune, graph=args.graph)
+        launch32 = time_job.schedule
         v32 = samples * max(1, args.steps // 2) / el32 / 1e9
-        fp32 = {"value": v32, "ms_per_step": el32 / max(1, args.steps // 2) * 1e3,
+        fp32 = {"value": v32, "ms_per_step": el32 / max(1, args.steps // 2) * 1e3, "launch": launch32,
                 "stages_ms": {fxcorr.STAGE_NAMES[i]: ms32[i] for i in range(len(ms32)) if n32[i]}}
         log(f"  fp32 {v32:.1f} Gsamples/s")
 
@@ -555,7 +561,11 @@ def main():
         avg_s = stage["xmac"]["ms"] / stage["xmac"]["launches"] / 1e3
         achieved = flops / avg_s / 1e12
         peak = PEAK_TFLOPS[args.precision]
-        kname = "fxg::k_xmac_mfma<2, false, 16>" if args.precision == "fp16" else "fxg::k_xmac_mfma<2, true, 16>"
+        bf = "true" if args.precision == "bf16" else "false"
+        if launch["schedule"]["xmac_streaming"]:
+            kname = f"fxg::k_xmac_mfma_dl<2, {bf}>"
+        else:
+            kname = f"fxg::k_xmac_mfma<2, {bf}, 16>"
         roof = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": profiled_traffic(kname), "avg_launch_ms": avg_s * 1e3}
 

@@ -84,6 +84,7 @@ struct fxg_job {
   DBuf d_spec, d_V;
   DBuf d_z;                       // FFTs above 16384 points: the sub-FFT scratch
   DBuf d_pcal, d_pcalaux, d_pcaltone, d_pcalpart;   // phase-cal tables and partial sums
+  DBuf d_vpart;                   // block-split XMAC partial matrices (zeroed once)
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
@@ -91,7 +92,7 @@ struct fxg_job {
   std::vector<unsigned char> prepared_key;
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
   DBuf d_zero;                    // 64 zero bytes: the rows past the data for streaming loads
-  int xmac_streaming = 0;         // direct-to-LDS MFMA XMAC where the layout allows
+  int xmac_streaming = 1;         // direct-to-LDS MFMA XMAC where the layout allows
   int xmac_ct = 0, xmac_tb = 0;   // FP32 XMAC channel tile / LDS stage (0: automatic)
   int chunk_blocks = 0;           // blocks per F/X pass (<= the plan's; 0: the plan's)
   // the subintegration as one graph launch, captured per launch shape
@@ -391,7 +392,7 @@ static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
                   &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z,
-                  &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart};
+                  &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart, &j->d_vpart};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -505,6 +506,11 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     return fail(set_err(FXG_EHIP, "band index upload failed"));
   if ((rc = j->d_zero.ensure(64))) return fail(rc);
   if (hipMemset(j->d_zero.p, 0, 64) != hipSuccess) return fail(set_err(FXG_EHIP, "zero row"));
+  if (P.vpart_elems > 0) {
+    if ((rc = j->d_vpart.ensure(sizeof(float2) * (size_t)P.vpart_elems))) return fail(rc);
+    if (hipMemset(j->d_vpart.p, 0, sizeof(float2) * (size_t)P.vpart_elems) != hipSuccess)
+      return fail(set_err(FXG_EHIP, "partial matrix clear"));
+  }
   if (!P.pcal.empty()) {
     if ((rc = j->d_pcal.ensure(sizeof(DevPcal) * P.pcal.size())) ||
         (rc = j->d_pcalaux.ensure(sizeof(float) * P.pcal_aux.size())) ||
@@ -562,6 +568,16 @@ int fxg_job_set_keep_taps(fxg_job* j, int keep) {
 }
 
 // XMAC tiling for nin inputs: CT channels per workgroup, 8x8 tiles
+// Grid rows for an XMAC launch of wgx channel groups over nb blocks: enough
+// workgroups to cover the chip when the channel groups alone do not, at
+// least 8 blocks per row, within the partial buffer (cap elements).
+static int xmac_splits(int wgx, int nb, long long vsz, long long cap) {
+  if (wgx >= 256 || nb < 16 || vsz <= 0) return 1;
+  long long n = std::min<long long>((512 + wgx - 1) / wgx, nb / 8);
+  n = std::min<long long>(n, cap / vsz);
+  return (int)std::max<long long>(1, n);
+}
+
 // (ct_req, tb_req: a schedule's choice, used where it fits; 0 = automatic)
 static void xmac_shape(int nin, int nchan, XmacArgs* a, int ct_req = 0, int tb_req = 0) {
   a->G = (nin + 7) / 8;
@@ -746,10 +762,18 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       xa.nchan = xf.nchan;
       xa.nblocks = cn;
       xmac_shape(xa.nin, xa.nchan, &xa, j->xmac_ct, j->xmac_tb);
-      xa.V = j->d_V.as<float2>() + xf.voff;
-      xa.accumulate = c0 > 0;
+      const bool mfma = prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan);
+      float2* V = j->d_V.as<float2>() + xf.voff;
+      // few channel groups: split the blocks over grid rows too, into partial
+      // matrices summed afterwards in row order
+      const long long vsz = (long long)xa.nin * xa.nin * xa.nchan;
+      const int nsplit = xmac_splits(mfma ? xa.nchan / 16 : (xa.nchan + xa.CT - 1) / xa.CT, cn, vsz, P.vpart_elems);
+      xa.split_blocks = (cn + nsplit - 1) / nsplit;
+      const int rows = (cn + xa.split_blocks - 1) / xa.split_blocks;
+      if (rows > 1) { xa.V = j->d_vpart.as<float2>(); xa.vsplit = vsz; xa.accumulate = 0; }
+      else { xa.V = V; xa.vsplit = 0; xa.accumulate = c0 > 0; }
       if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
-      if (prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan)) {
+      if (mfma) {
         if (j->xmac_streaming && xmac_mfma_streaming_ok(xa, xf.in_off.data()))
           HIPCHK(launch_xmac_mfma_streaming(xa, prec, j->d_zero.p, s));
         else
@@ -757,6 +781,7 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       } else {
         HIPCHK(launch_xmac_f32(xa, prec, s));
       }
+      if (rows > 1) HIPCHK(launch_xmac_reduce(j->d_vpart.as<float2>(), rows, vsz, V, vsz, c0 > 0, s));
       if ((rc = j->ev_end())) return rc;
     }
     j->tap_chunk0 = c0;

@@ -136,6 +136,8 @@ struct Plan {
   int pcal_groups = 0;                                 // block groups per band
   long long pcal_partfloats = 0;
   int pcal_maxtones = 0;
+  // block-split XMAC: complex elements of the partial-matrix buffer
+  long long vpart_elems = 0;
 };
 
 // the largest accumulation of one band's phase-cal bins (LDS floats)

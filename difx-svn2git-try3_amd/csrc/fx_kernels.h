@@ -41,6 +41,10 @@ struct XmacArgs {
   int CT, TB, ntiles, G;      // channel tile, blocks per stage, tiles, 8-groups
   float2* V;                  // [nin][nin][nchan]
   int accumulate;
+  // block split: grid row y takes blocks [y*split_blocks, (y+1)*split_blocks)
+  // and writes V + y*vsplit (partial sums, reduced by launch_xmac_reduce)
+  int split_blocks;
+  long long vsplit;
 };
 
 struct PcalArgs {
@@ -90,6 +94,9 @@ hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream);
 // the direct-to-LDS double-buffered variant; needs 16-byte aligned rows
 bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host);
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream);
+// V[i] (+)= sum over splits of P[s][i], i < n, in split order
+hipError_t launch_xmac_reduce(const float2* P, int nsplit, long long vsplit, float2* V, long long n, int accumulate,
+                              hipStream_t stream);
 hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
                           hipStream_t stream);
 hipError_t launch_pcal(const PcalArgs& a, int npcal, hipStream_t stream);

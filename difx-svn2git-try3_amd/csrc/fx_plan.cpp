@@ -609,6 +609,16 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
   chunk = std::min(chunk, d->blockspersend);
   P.chunk_blocks = chunk;
   P.info.chunk_blocks = chunk;
+  // partial matrices for XMAC launches with few channel groups (block split):
+  // up to 512 workgroups' worth, at most 256 MiB
+  for (const XmacFreq& x : P.xf) {
+    const long long nin = (long long)x.in_station.size();
+    const long long vsz = nin * nin * x.nchan;
+    const long long wgx = std::max(1, x.nchan / 16);   // the widest channel group used
+    if (wgx >= 256 || chunk < 16) continue;
+    const long long rows = std::min<long long>((512 + wgx - 1) / wgx, chunk / 8);
+    P.vpart_elems = std::max(P.vpart_elems, std::min(rows * vsz, (256LL << 20) / 8 / std::max(1LL, vsz) * vsz));
+  }
   P.info.device_bytes = plan_device_bytes(P, d);
   return 0;
 }
@@ -620,13 +630,14 @@ int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d) {
   long long res = p.info.result_length * 8;
   long long tables = (long long)p.segs.size() * sizeof(DevSeg) + p.bands.size() * sizeof(DevBand) + (1 << 20);
   long long z = p.L > 16384 ? (long long)p.nbands_total * p.chunk_blocks * p.L * 8 : 0;
+  long long vpart = p.vpart_elems * 8;
   long long pcal = p.pcal_partfloats * 4 + (long long)p.pcal_aux.size() * 4 +
                    (long long)p.pcal.size() * sizeof(DevPcal) + (long long)p.pcal_tonebin.size() * 4;
   // corner-turned samples and the frame table (about the raw data of the subint)
   long long ct = 0;
   for (const DevStation& st : p.st)
     ct += (long long)(d->blockspersend + 2) * st.L * st.nbits * st.nchan / 8 + st.framebytes * 4;
-  return spec + v + blocks + res + tables + z + pcal + ct + ct / 8;
+  return spec + v + blocks + res + tables + z + pcal + vpart + ct + ct / 8;
 }
 
 }  // namespace fxg

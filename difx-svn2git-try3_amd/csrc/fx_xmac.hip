@@ -40,6 +40,9 @@ __global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
   const int nthreads = blockDim.x;
   const int CT = A.CT, TB = A.TB, NP = A.G * 8;
   const int c0 = blockIdx.x * CT;
+  const int sb0 = blockIdx.y * A.split_blocks;               // this row's blocks
+  const int nb = min(A.nblocks - sb0, A.split_blocks);
+  float2* const Vout = A.V + (long long)blockIdx.y * A.vsplit;
   const int cl = tid % CT;
   const int tile = tid / CT;
   int ta = 0, tb = 0;
@@ -55,20 +58,20 @@ __global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
 #pragma unroll
     for (int k = 0; k < 8; k++) acc[i][k] = make_float2(0.f, 0.f);
 
-  for (int t0 = 0; t0 < A.nblocks; t0 += TB) {
+  for (int t0 = 0; t0 < nb; t0 += TB) {
     const int nstage = TB * NP * CT;
     for (int e = tid; e < nstage; e += nthreads) {
       int c = e % CT;
       int a = (e / CT) % NP;
       int tt = e / (CT * NP);
       float2 v = make_float2(0.f, 0.f);
-      if (a < A.nin && t0 + tt < A.nblocks && c0 + c < A.nchan)
-        v = load_spec(spec, A.in_off[a] + (long long)(t0 + tt) * A.block_stride + c0 + c);
+      if (a < A.nin && t0 + tt < nb && c0 + c < A.nchan)
+        v = load_spec(spec, A.in_off[a] + (long long)(sb0 + t0 + tt) * A.block_stride + c0 + c);
       sh[e] = v;
     }
     __syncthreads();
     if (active) {
-      const int nt = min(TB, A.nblocks - t0);
+      const int nt = min(TB, nb - t0);
       for (int tt = 0; tt < nt; tt++) {
         const float2* row = sh + (long long)tt * NP * CT;
         float2 xb[8];
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
     for (int k = 0; k < 8; k++) {
       int b = tb * 8 + k;
       if (b >= A.nin) continue;
-      float2* p = A.V + ((long long)a * A.nin + b) * A.nchan + c0 + cl;
+      float2* p = Vout + ((long long)a * A.nin + b) * A.nchan + c0 + cl;
       if (A.accumulate) { float2 o = *p; o.x += acc[i][k].x; o.y += acc[i][k].y; *p = o; }
       else *p = acc[i][k];
     }
@@ -185,13 +188,36 @@ __global__ void k_scatter(const DevSeg* __restrict__ segs, int nseg, const float
 
 hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream) {
   size_t shmem = (size_t)a.TB * a.G * 8 * a.CT * sizeof(float2);
-  dim3 grid((unsigned)((a.nchan + a.CT - 1) / a.CT));
+  dim3 grid((unsigned)((a.nchan + a.CT - 1) / a.CT), (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks));
   int threads = a.CT * a.ntiles;
   threads = ((threads + 63) / 64) * 64;
   if (threads > 576) return hipErrorInvalidValue;
   if (prec == FXG_XMAC_FP32) hipLaunchKernelGGL(k_xmac_f32<float2>, grid, dim3(threads), shmem, stream, a);
   else if (prec == FXG_XMAC_FP16) hipLaunchKernelGGL(k_xmac_f32<__half2>, grid, dim3(threads), shmem, stream, a);
   else hipLaunchKernelGGL(k_xmac_f32<__hip_bfloat162>, grid, dim3(threads), shmem, stream, a);
+  return hipGetLastError();
+}
+
+// V[i] (+)= sum_s P[s*vsplit + i]: the block-split partial sums, added in
+// split order so that the result does not depend on scheduling
+__global__ __launch_bounds__(256) void k_xmac_reduce(const float2* __restrict__ P, int nsplit, long long vsplit,
+                                                     float2* __restrict__ V, long long n, int accumulate) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float2 acc = accumulate ? V[i] : make_float2(0.f, 0.f);
+  for (int s = 0; s < nsplit; s++) {
+    const float2 p = P[(long long)s * vsplit + i];
+    acc.x += p.x;
+    acc.y += p.y;
+  }
+  V[i] = acc;
+}
+
+hipError_t launch_xmac_reduce(const float2* P, int nsplit, long long vsplit, float2* V, long long n, int accumulate,
+                              hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xmac_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, P, nsplit, vsplit, V, n,
+                     accumulate);
   return hipGetLastError();
 }
 

@@ -60,6 +60,9 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
   const int grp = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;
   const int c0 = grp * CG;
   const int half = l >> 5, row = l & 31;
+  const int sb0 = blockIdx.y * A.split_blocks;               // this row's blocks
+  const int nb = min(A.nblocks - sb0, A.split_blocks);
+  float2* const Vout = A.V + (long long)blockIdx.y * A.vsplit;
 
   f16_t accR[NTILE], accI[NTILE];
 #pragma unroll
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
 #pragma unroll
     for (int e = 0; e < 16; e++) { accR[q][e] = 0.0f; accI[q][e] = 0.0f; }
 
-  for (int t0 = 0; t0 < A.nblocks; t0 += TT) {
+  for (int t0 = 0; t0 < nb; t0 += TT) {
     // ---- stage: NR inputs x TT blocks x CG channels, 16 bytes (4 channels) per item
     constexpr int NITEM = NR * TT * (CG / 4);
     for (int it = tid; it < NITEM; it += NTH) {
@@ -76,8 +79,8 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
       const int t = (it / QN) % TT;
       const int a = it / (QN * TT);
       uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (a < A.nin && t0 + t < A.nblocks) {
-        const long long e = A.in_off[a] + (long long)(t0 + t) * A.block_stride + c0 + 4 * q;
+      if (a < A.nin && t0 + t < nb) {
+        const long long e = A.in_off[a] + (long long)(sb0 + t0 + t) * A.block_stride + c0 + 4 * q;
         if ((e & 3) == 0) {
           v = *(const uint4*)(spec + e);
         } else {
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
       for (int r = 0; r < 16; r++) {
         const int a = 32 * I + 8 * (r >> 2) + 4 * half + (r & 3);
         if (a >= A.nin || b >= A.nin) continue;
-        float2* p = A.V + ((long long)a * A.nin + b) * A.nchan + c;
+        float2* p = Vout + ((long long)a * A.nin + b) * A.nchan + c;
         float2 v = make_float2(accR[q][r], accI[q][r]);
         if (A.accumulate) { float2 o = *p; v.x += o.x; v.y += o.y; }
         *p = v;
@@ -160,6 +163,9 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
   const int c0 = grp * CG;
   const int half = l >> 5, row = l & 31;
   const int lrow = l >> 2, lq = l & 3;   // load lane: block row and 4-channel quad
+  const int sb0 = blockIdx.y * A.split_blocks;               // this row's blocks
+  const int nb = min(A.nblocks - sb0, A.split_blocks);
+  float2* const Vout = A.V + (long long)blockIdx.y * A.vsplit;
   // the inputs this wavefront loads, with their offsets read once into
   // scalar registers (a vector load here would make every later wait for
   // it also wait for the loads in flight)
@@ -176,8 +182,8 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
 #pragma unroll
     for (int u = 0; u < LPW; u++) {
       const void* src = zero16;
-      if (ioff[u] >= 0 && t < A.nblocks)
-        src = spec + 4 * (ioff[u] + (long long)t * A.block_stride + 4 * lq);
+      if (ioff[u] >= 0 && t < nb)
+        src = spec + 4 * (ioff[u] + (long long)(sb0 + t) * A.block_stride + 4 * lq);
       __builtin_amdgcn_global_load_lds(src,
                                        (__attribute__((address_space(3))) void*)(lds + buf * STAGE + (wu * LPW + u) * SDW),
                                        16, 0, 0);
@@ -190,7 +196,7 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
 #pragma unroll
     for (int e = 0; e < 16; e++) { accR[q][e] = 0.0f; accI[q][e] = 0.0f; }
 
-  const int nstage = (A.nblocks + TT - 1) / TT;
+  const int nstage = (nb + TT - 1) / TT;
   issue(0, 0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -233,7 +239,7 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
       for (int r = 0; r < 16; r++) {
         const int a = 32 * I + 8 * (r >> 2) + 4 * half + (r & 3);
         if (a >= A.nin || b >= A.nin) continue;
-        float2* p = A.V + ((long long)a * A.nin + b) * A.nchan + c;
+        float2* p = Vout + ((long long)a * A.nin + b) * A.nchan + c;
         float2 v = make_float2(accR[q][r], accI[q][r]);
         if (A.accumulate) { float2 o = *p; v.x += o.x; v.y += o.y; }
         *p = v;
@@ -252,7 +258,7 @@ bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host) {
 
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream) {
   if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
-  dim3 grid((unsigned)(a.nchan / 16));
+  dim3 grid((unsigned)(a.nchan / 16), (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks));
   const bool bf = prec == FXG_XMAC_BF16;
   const uint4* z = (const uint4*)zero16;
   if (a.nin <= 32) {
@@ -268,7 +274,7 @@ hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* z
 hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream) {
   if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
   constexpr int CG = 16;      // 64-byte row segments; one workgroup (148 KB of LDS) per CU
-  dim3 grid((unsigned)(a.nchan / CG));
+  dim3 grid((unsigned)(a.nchan / CG), (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks));
   const bool bf = prec == FXG_XMAC_BF16;
   if (a.nin <= 32) {
     if (bf) hipLaunchKernelGGL((k_xmac_mfma<1, true, CG>), grid, dim3(64 * CG), 0, stream, a);

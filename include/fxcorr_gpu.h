@@ -305,9 +305,9 @@ int fxg_job_set_profiling(fxg_job* job, int on);
  * reset != 0 clears the accumulators after reading. */
 int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
 
-/* Reduced-precision XMAC staging: 1 streams the spectra from HBM
+/* Reduced-precision XMAC staging: 1 (default) streams the spectra from HBM
  * into LDS with direct loads, double-buffered, where the layout allows;
- * 0 (default) stages them through registers. */
+ * 0 stages them through registers. */
 int fxg_job_set_xmac_streaming(fxg_job* job, int on);
 
 /* ---- VDIF thread multiplexing (libraries/vdifio/src/vdifmux.c) ----------
