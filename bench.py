@@ -169,12 +169,16 @@ def vis_errors(lib, ctx, orc, seed, blocks, threads, torch):
         check(orc, orc.orc_plan_autocorr_offset(C.byref(job64), d, C.byref(off)), "ac")
         acs.append(ref[off.value: off.value + nb * nch].real.reshape(nb, nch))
     out = {}
-    for prec in ("fp16", "bf16", "fp32"):
+    for prec in ("fp16", "bf16", "fp8", "fp32"):
         spec = c3_spec(prec, blocks)
         job = Job(lib, ctx, spec)
         dres = torch.empty(2 * job.info.result_length, dtype=torch.float32, device="cuda:0")
         sti, keep2 = station_inputs_device(sd, raws)
         job.run(sub, sti, dres.data_ptr())
+        if prec == "fp8":
+            # per-channel power-of-two scales from this subintegration's power, then again
+            check(lib, lib.fxg_job_calibrate_scales(job.h), "calibrate")
+            job.run(sub, sti, dres.data_ptr())
         got = dres.cpu().numpy().view(np.complex64).astype(np.complex128)
         job.close()
         d = got[:xend] - ref[:xend]
@@ -441,7 +445,7 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--blocks", type=int, default=2000, help="FFT blocks per subintegration")
-    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32"])
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp8", "fp32"])
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison")
     ap.add_argument("--no-verify", action="store_true", help="skip the FP64-oracle error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

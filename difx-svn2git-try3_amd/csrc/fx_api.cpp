@@ -85,6 +85,7 @@ struct fxg_job {
   DBuf d_z;                       // FFTs above 16384 points: the sub-FFT scratch
   DBuf d_pcal, d_pcalaux, d_pcaltone, d_pcalpart;   // phase-cal tables and partial sums
   DBuf d_vpart;                   // block-split XMAC partial matrices (zeroed once)
+  DBuf d_cexp, d_eoff;            // fp8: scale exponents [gband][channel], per-input offsets
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
@@ -392,7 +393,8 @@ static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
                   &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z,
-                  &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart, &j->d_vpart};
+                  &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart, &j->d_vpart, &j->d_cexp,
+                  &j->d_eoff};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -442,16 +444,20 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
 
   const int prec = desc->xmac_precision;
   // spectrum scale for reduced precision: 2^-round(log2(sqrt(L)))
-  j->scale = prec == FXG_XMAC_FP32 ? 1.0f : std::ldexp(1.0f, -(int)std::lround(0.5 * std::log2((double)P.L)));
+  // (fp8: per-channel exponents instead, below)
+  j->scale = (prec == FXG_XMAC_FP32 || prec == FXG_XMAC_FP8)
+                 ? 1.0f : std::ldexp(1.0f, -(int)std::lround(0.5 * std::log2((double)P.L)));
 
   // spectra offsets: [global band][chunk block][C]
   const long long chunk = P.chunk_blocks;
   for (size_t g = 0; g < P.bands.size(); g++) P.bands[g].specoff = (long long)g * chunk * P.C;
   std::vector<long long> inoff;
+  std::vector<int> eoff;            // fp8: [gband][channel] index of each input's channel 0
   for (XmacFreq& x : P.xf) {
     j->inoff_base.push_back((long long)inoff.size());
     for (size_t i = 0; i < x.in_station.size(); i++) {
       int gband = P.st[x.in_station[i]].band0 + x.in_band[i];
+      eoff.push_back(gband * P.C + (int)x.in_off[i]);
       x.in_off[i] = P.bands[gband].specoff + x.in_off[i];
       inoff.push_back(x.in_off[i]);
     }
@@ -469,7 +475,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     double ang = -kTwoPi * (double)(64 * h) / (double)P.L;
     tw[64 + h] = make_float2((float)std::cos(ang), (float)std::sin(ang));
   }
-  size_t specbytes = (size_t)P.nbands_total * chunk * P.C * (prec == FXG_XMAC_FP32 ? 8 : 4);
+  size_t specbytes = (size_t)P.nbands_total * chunk * P.C * spectrum_bytes(prec);
   const size_t zbytes = P.L > kFStageMaxL ? (size_t)P.nbands_total * chunk * P.L * sizeof(float2) : 0;
   if ((rc = j->d_st.ensure(sizeof(DevStation) * P.st.size())) ||
       (rc = j->d_bands.ensure(sizeof(DevBand) * P.bands.size())) ||
@@ -506,6 +512,22 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     return fail(set_err(FXG_EHIP, "band index upload failed"));
   if ((rc = j->d_zero.ensure(64))) return fail(rc);
   if (hipMemset(j->d_zero.p, 0, 64) != hipSuccess) return fail(set_err(FXG_EHIP, "zero row"));
+  if (prec == FXG_XMAC_FP8) {
+    // default scales: RMS of an L-point transform of the quantised samples
+    // (variance 4.31 for 2-bit levels at +-0.98 sigma thresholds; about 6 for
+    // the 8-bit levels of unit-variance data) mapped to 16
+    std::vector<signed char> ex((size_t)P.nbands_total * P.C);
+    for (size_t g = 0; g < P.bands.size(); g++) {
+      const double var = P.st[P.bands[g].station].nbits == 8 ? 6.0 : 4.31;
+      const int e = (int)std::lround(std::log2(16.0 / std::sqrt((double)P.L * var)));
+      std::fill(ex.begin() + g * P.C, ex.begin() + (g + 1) * P.C, (signed char)e);
+    }
+    if ((rc = j->d_cexp.ensure(ex.size())) || (rc = j->d_eoff.ensure(sizeof(int) * std::max<size_t>(1, eoff.size()))))
+      return fail(rc);
+    if (hipMemcpy(j->d_cexp.p, ex.data(), ex.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(j->d_eoff.p, eoff.data(), sizeof(int) * eoff.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_err(FXG_EHIP, "scale table upload failed"));
+  }
   if (P.vpart_elems > 0) {
     if ((rc = j->d_vpart.ensure(sizeof(float2) * (size_t)P.vpart_elems))) return fail(rc);
     if (hipMemset(j->d_vpart.p, 0, sizeof(float2) * (size_t)P.vpart_elems) != hipSuccess)
@@ -546,6 +568,38 @@ int fxg_job_plan_info(const fxg_job* j, fxg_plan_info* info) {
 int fxg_job_set_xmac_streaming(fxg_job* j, int on) {   // one entry of the schedule
   if (!j) return set_err(FXG_EINVAL, "null job");
   j->xmac_streaming = on ? 1 : 0;
+  return 0;
+}
+
+int fxg_job_calibrate_scales(fxg_job* j) {
+  if (!j) return set_err(FXG_EINVAL, "null job");
+  if (j->desc.xmac_precision != FXG_XMAC_FP8) return set_err(FXG_EINVAL, "not an fp8 job");
+  if (!j->have_subint) return set_err(FXG_ESTATE, "no subintegration processed");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  HIPCHK(hipStreamSynchronize(j->stream));
+  const Plan& P = j->plan;
+  std::vector<signed char> ex((size_t)P.nbands_total * P.C);
+  HIPCHK(hipMemcpy(ex.data(), j->d_cexp.p, ex.size(), hipMemcpyDeviceToHost));
+  const double nb = std::max(1, j->desc.blockspersend);
+  std::vector<float2> row;
+  for (const XmacFreq& x : P.xf) {
+    const int nin = (int)x.in_station.size();
+    row.resize(x.nchan);
+    for (int a = 0; a < nin; a++) {
+      const int ds = x.in_station[a];
+      if (x.in_total[a] >= j->dss[ds].nrecbands) continue;      // zoom inputs: their parent's own frequency
+      const size_t gband = (size_t)P.st[ds].band0 + x.in_band[a];
+      HIPCHK(hipMemcpy(row.data(), j->d_V.as<float2>() + x.voff + ((long long)a * nin + a) * x.nchan,
+                       sizeof(float2) * x.nchan, hipMemcpyDeviceToHost));
+      for (int c = 0; c < x.nchan; c++) {
+        const double pw = row[c].x / nb;                          // mean |X|^2 per block
+        if (!(pw > 0) || !std::isfinite(pw)) continue;
+        const long e = std::lround(std::log2(16.0 / std::sqrt(pw)));
+        ex[gband * P.C + c] = (signed char)std::max(-100L, std::min(100L, e));
+      }
+    }
+  }
+  HIPCHK(hipMemcpy(j->d_cexp.p, ex.data(), ex.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -710,6 +764,7 @@ static void fill_fstage_args(fxg_job* j, FStageArgs* fa) {
   fa->nblocks_subint = j->desc.blockspersend;
   fa->zbuf = j->d_z.p ? j->d_z.as<float2>() : nullptr;
   fa->chunkn = j->plan.chunk_blocks;
+  fa->cexp = j->d_cexp.p ? j->d_cexp.as<signed char>() : nullptr;
 }
 
 // identity of a subintegration's inputs: the prepared state is reused while it matches
@@ -762,7 +817,9 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       xa.nchan = xf.nchan;
       xa.nblocks = cn;
       xmac_shape(xa.nin, xa.nchan, &xa, j->xmac_ct, j->xmac_tb);
+      const bool fp8 = prec == FXG_XMAC_FP8;
       const bool mfma = prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan);
+      if (fp8) { xa.cexp = j->d_cexp.as<signed char>(); xa.eoff = j->d_eoff.as<int>() + j->inoff_base[x]; }
       float2* V = j->d_V.as<float2>() + xf.voff;
       // few channel groups: split the blocks over grid rows too, into partial
       // matrices summed afterwards in row order
@@ -773,7 +830,9 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       if (rows > 1) { xa.V = j->d_vpart.as<float2>(); xa.vsplit = vsz; xa.accumulate = 0; }
       else { xa.V = V; xa.vsplit = 0; xa.accumulate = c0 > 0; }
       if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
-      if (mfma) {
+      if (fp8) {
+        HIPCHK(launch_xmac_fp8(xa, s));
+      } else if (mfma) {
         if (j->xmac_streaming && xmac_mfma_streaming_ok(xa, xf.in_off.data()))
           HIPCHK(launch_xmac_mfma_streaming(xa, prec, j->d_zero.p, s));
         else
@@ -1243,6 +1302,19 @@ int fxg_tap_spectrum(fxg_job* j, int ds, int band, int block, float* out) {
   long long off = B.specoff + (long long)(block - j->tap_chunk0) * P.C;
   if (j->desc.xmac_precision == FXG_XMAC_FP32) {
     HIPCHK(hipMemcpy(out, j->d_spec.as<float2>() + off, sizeof(float2) * P.C, hipMemcpyDeviceToHost));
+  } else if (j->desc.xmac_precision == FXG_XMAC_FP8) {
+    std::vector<uint8_t> h(2 * (size_t)P.C);
+    std::vector<signed char> ex((size_t)P.C);
+    HIPCHK(hipMemcpy(h.data(), j->d_spec.as<uint16_t>() + off, 2 * (size_t)P.C, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ex.data(), j->d_cexp.as<signed char>() + (size_t)(P.st[ds].band0 + band) * P.C, (size_t)P.C,
+                     hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); i++) {
+      // e4m3: sign, 4 exponent bits (bias 7), 3 mantissa bits; 0x7f/0xff NaN
+      const int sign = h[i] >> 7, e = (h[i] >> 3) & 15, m = h[i] & 7;
+      float v = e == 0 ? std::ldexp((float)m, -9) : std::ldexp((float)(8 + m), e - 10);
+      if (e == 15 && m == 7) v = std::nanf("");
+      out[i] = std::ldexp(sign ? -v : v, -ex[i / 2]);
+    }
   } else {
     std::vector<uint16_t> h(2 * (size_t)P.C);
     HIPCHK(hipMemcpy(h.data(), j->d_spec.as<uint32_t>() + off, 4 * (size_t)P.C, hipMemcpyDeviceToHost));

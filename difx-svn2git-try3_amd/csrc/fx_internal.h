@@ -140,6 +140,9 @@ struct Plan {
   long long vpart_elems = 0;
 };
 
+// bytes of one spectrum value (complex) entering the cross-multiply
+inline int spectrum_bytes(int prec) { return prec == FXG_XMAC_FP32 ? 8 : prec == FXG_XMAC_FP8 ? 2 : 4; }
+
 // the largest accumulation of one band's phase-cal bins (LDS floats)
 constexpr int kPcalMaxFloats = 32768;
 

@@ -31,6 +31,7 @@ struct FStageArgs {
   int chunk0;                 // first block of the chunk
   float2* zbuf;               // FFTs above 16384 points: [band][chunk block][16][L/16] scratch
   int chunkn;                 // blocks per chunk (the scratch's block dimension)
+  const signed char* cexp;    // fp8: power-of-two scale exponent per [global band][channel]
 };
 
 struct XmacArgs {
@@ -45,6 +46,9 @@ struct XmacArgs {
   // and writes V + y*vsplit (partial sums, reduced by launch_xmac_reduce)
   int split_blocks;
   long long vsplit;
+  // fp8: scale exponents, and per input the offset of its channel 0 in them
+  const signed char* cexp;
+  const int* eoff;
 };
 
 struct PcalArgs {
@@ -92,6 +96,8 @@ hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream);
 bool xmac_mfma_supported(int nin, int nchan);
 hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream);
 // the direct-to-LDS double-buffered variant; needs 16-byte aligned rows
+// fp8 e4m3 spectra on the microscaled matrix cores (nin <= 64, nchan % 16 == 0)
+hipError_t launch_xmac_fp8(const XmacArgs& a, hipStream_t stream);
 bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host);
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream);
 // V[i] (+)= sum over splits of P[s][i], i < n, in split order

@@ -176,7 +176,7 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
   REQUIRE(d->nbaselines >= 0, FXG_EINVAL, "nbaselines negative");
   REQUIRE(d->blockspersend > 0, FXG_EINVAL, "blockspersend must be positive");
   REQUIRE(d->fringe_order >= 0 && d->fringe_order <= 2, FXG_EUNSUPPORTED, "fringe rotation order must be 0, 1 or 2");
-  REQUIRE(d->xmac_precision >= FXG_XMAC_FP32 && d->xmac_precision <= FXG_XMAC_BF16, FXG_EINVAL,
+  REQUIRE(d->xmac_precision >= FXG_XMAC_FP32 && d->xmac_precision <= FXG_XMAC_FP8, FXG_EINVAL,
           "xmac_precision");
   for (int f = 0; f < d->nfreqs; f++) {
     REQUIRE(d->freqs[f].nchan > 0, FXG_EINVAL, "freq nchan must be positive");
@@ -594,6 +594,10 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
   }
 
   REQUIRE(seg_ok, FXG_EINVAL, "a result product names a band that no XMAC frequency holds");
+  if (d->xmac_precision == FXG_XMAC_FP8)
+    for (const XmacFreq& x : P.xf)
+      REQUIRE(x.in_station.size() <= 64 && x.nchan % 16 == 0, FXG_EUNSUPPORTED,
+              "fp8 cross-multiply needs at most 64 inputs and a multiple of 16 channels per frequency");
   P.info.result_length = at;
   P.info.arraystridelen = P.arraystride[0];
   P.info.xmacstridelen = P.xmacstride;
@@ -602,7 +606,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
   P.info.fftchannels = P.L;
 
   // ---- spectra chunking: keep the spectra of at most ~16 GiB resident
-  long long per_block = (long long)P.nbands_total * P.C * (d->xmac_precision == FXG_XMAC_FP32 ? 8 : 4);
+  long long per_block = (long long)P.nbands_total * P.C * spectrum_bytes(d->xmac_precision);
   if (P.L > 16384) per_block += (long long)P.nbands_total * P.L * 8;     // the sub-FFT scratch
   long long budget = 16LL << 30;
   int chunk = d->max_chunk_blocks > 0 ? d->max_chunk_blocks : (int)std::max(1LL, budget / per_block);
@@ -624,7 +628,8 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
 }
 
 int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d) {
-  long long spec = (long long)p.nbands_total * p.C * p.chunk_blocks * (d->xmac_precision == FXG_XMAC_FP32 ? 8 : 4);
+  long long spec = (long long)p.nbands_total * p.C * p.chunk_blocks * spectrum_bytes(d->xmac_precision);
+  if (d->xmac_precision == FXG_XMAC_FP8) spec += (long long)p.nbands_total * p.C;   // the scale exponents
   long long v = p.vsize * 8;
   long long blocks = (long long)p.nds * d->blockspersend * (long long)sizeof(DevBlock);
   long long res = p.info.result_length * 8;

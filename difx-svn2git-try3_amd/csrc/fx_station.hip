@@ -376,39 +376,46 @@ __global__ __launch_bounds__(256) void k_cornerturn(const DevStation* __restrict
 // ---------------------------------------------------------------------------
 // k_cornerturn2<W>: the same corner turn for 2-bit frames whose sample-times
 // are W whole 32-bit words (16 W bands), used when every datastream has that
-// shape and 4W-byte aligned frames.  Each thread turns 16 consecutive
-// sample-times of all bands with a 2-bit transpose in registers: W vector
-// loads per sample-time, one coalesced word store per band.
+// shape and 4W-byte aligned frames.  A workgroup stages 4096 consecutive
+// sample-times through LDS with coalesced loads; each thread then turns 16
+// of them for all bands with a 2-bit transpose in registers and stores one
+// coalesced word per band.
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restrict__ st,
                                                      const DevStationSub* __restrict__ ss,
                                                      unsigned char* __restrict__ ct) {
   using VW = typename std::conditional<W == 1, unsigned, typename std::conditional<W == 2, uint2, uint4>::type>::type;
+  constexpr int TPW = 16 * 256;                        // sample-times per workgroup
+  __shared__ VW buf[TPW + TPW / 16];                   // element i at i + i/16
   const int s = blockIdx.y;
   const DevStation S = st[s];
   const DevStationSub Q = ss[s];
   const long long spf = S.databytes / (4 * W);          // sample-times per frame
   const long long total = Q.nframes * spf;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long tau0 = g * 16;
-  if (tau0 >= total) return;
-  long long f = tau0 / spf;
-  long long off = tau0 - f * spf;
-  uint32_t w[W][16];
-#pragma unroll
-  for (int t = 0; t < 16; t++) {
+  const long long base = (long long)blockIdx.x * TPW;
+  if (base >= total) return;                            // the whole workgroup
+  for (int i = threadIdx.x; i < TPW; i += 256) {
+    const long long tau = base + i;
     VW v;
-    if (tau0 + t < total) {
-      const VW* p = (const VW*)(Q.data + f * S.framebytes + S.headerbytes);
-      v = p[off];
+    if (tau < total) {
+      const long long f = tau / spf;
+      v = ((const VW*)(Q.data + f * S.framebytes + S.headerbytes))[tau - f * spf];
     } else {
       memset(&v, 0, sizeof v);
     }
+    buf[i + (i >> 4)] = v;
+  }
+  __syncthreads();
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g * 16 >= total) return;
+  uint32_t w[W][16];
+#pragma unroll
+  for (int t = 0; t < 16; t++) {
+    const VW v = buf[threadIdx.x * 17 + t];
     const uint32_t* pv = (const uint32_t*)&v;
 #pragma unroll
     for (int c = 0; c < W; c++) w[c][t] = pv[c];
-    if (++off == spf) { off = 0; f++; }
   }
 #pragma unroll
   for (int c = 0; c < W; c++) {
@@ -496,9 +503,36 @@ __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStatio
   }
 }
 
+// fp8 e4m3 pair (re low byte, im high byte) of y * 2^e, saturated at the
+// largest finite value 448
+__device__ __forceinline__ unsigned short fp8_pair(float2 y, int e) {
+  const float a = fminf(fmaxf(ldexpf(y.x, e), -448.0f), 448.0f);
+  const float b = fminf(fmaxf(ldexpf(y.y, e), -448.0f), 448.0f);
+  return (unsigned short)(__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xFFFF);
+}
+struct Fp8Out {
+  unsigned short* p;          // the band's spectrum of this block
+  const signed char* e;       // the band's channel exponents
+};
+
 __device__ __forceinline__ void put_spec(float2* o, int c, float2 y) { o[c] = y; }
 __device__ __forceinline__ void put_spec(__half2* o, int c, float2 y) { o[c] = __floats2half2_rn(y.x, y.y); }
 __device__ __forceinline__ void put_spec(__hip_bfloat162* o, int c, float2 y) { o[c] = __float22bfloat162_rn(y); }
+__device__ __forceinline__ void put_spec(const Fp8Out& o, int c, float2 y) { o.p[c] = fp8_pair(y, o.e[c]); }
+
+// zero spectrum of one (band, block): C values of the job's spectrum type
+__device__ __forceinline__ void zero_spec(void* spectra, int prec, long long outbase, int C, int tid, int nt) {
+  if (prec == FXG_XMAC_FP32) {
+    float2* o = (float2*)spectra + outbase;
+    for (int c = tid; c < C; c += nt) o[c] = make_float2(0.f, 0.f);
+  } else if (prec == FXG_XMAC_FP8) {
+    unsigned short* o = (unsigned short*)spectra + outbase;
+    for (int c = tid; c < C; c += nt) o[c] = 0;
+  } else {
+    unsigned* o = (unsigned*)spectra + outbase;
+    for (int c = tid; c < C; c += nt) o[c] = 0u;
+  }
+}
 
 // Last pass of radix 2 (L = 2 * 16^a) for thread j: butterflies k = j + u L/16,
 // u < 8, with W_L^k = W_L^j W_16^u; only the kept sideband's output of each
@@ -508,7 +542,7 @@ __device__ __forceinline__ void put_spec(__hip_bfloat162* o, int c, float2 y) { 
 template <int L, typename TO>
 __device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* twlo, const float2* twhi,
                                              const float2* fsub, const float2* fstep, int s, int slog,
-                                             float scale, bool lsb, TO* out, int j) {
+                                             float scale, bool lsb, TO out, int j) {
   constexpr int NTF = L / 16, NSL = L / 2, RSTR = NTF + NTF / 16;
   const float2 wj = tw_of(twlo, twhi, j);
   float2 f0 = fsub[j & (s - 1)];
@@ -562,13 +596,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   const long long outbase = B.specoff + (long long)cb * C;
 
   if (G.status != 0) {
-    if (A.prec == FXG_XMAC_FP32) {
-      float2* o = (float2*)A.spectra + outbase;
-      for (int c = tid; c < C; c += NT) o[c] = make_float2(0.f, 0.f);
-    } else {
-      unsigned* o = (unsigned*)A.spectra + outbase;
-      for (int c = tid; c < C; c += NT) o[c] = 0u;
-    }
+    zero_spec(A.spectra, A.prec, outbase, C, tid, NT);
     return;
   }
 
@@ -736,12 +764,15 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   //      (USB [0, C), real LSB [C, 2C)), then the fractional-sample rotation
   if (RL == 2 && tid < NTF && rotfast) {
     if (A.prec == FXG_XMAC_FP32)
-      last_pass_r2<L, float2>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0, (float2*)A.spectra + outbase, tid);
+      last_pass_r2<L, float2*>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0, (float2*)A.spectra + outbase, tid);
     else if (A.prec == FXG_XMAC_FP16)
-      last_pass_r2<L, __half2>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0, (__half2*)A.spectra + outbase, tid);
+      last_pass_r2<L, __half2*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0, (__half2*)A.spectra + outbase, tid);
+    else if (A.prec == FXG_XMAC_FP8)
+      last_pass_r2<L, Fp8Out>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0,
+                              Fp8Out{(unsigned short*)A.spectra + outbase, A.cexp + (long long)gband * C}, tid);
     else
-      last_pass_r2<L, __hip_bfloat162>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0,
-                                       (__hip_bfloat162*)A.spectra + outbase, tid);
+      last_pass_r2<L, __hip_bfloat162*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0,
+                                        (__hip_bfloat162*)A.spectra + outbase, tid);
   } else if (tid < NTF) {
     const int j = tid;
     const int mh = B.lsb ? RL / 2 : 0;
@@ -771,6 +802,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
           ((float2*)A.spectra)[outbase + c] = y;
         } else if (A.prec == FXG_XMAC_FP16) {
           ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
+        } else if (A.prec == FXG_XMAC_FP8) {
+          ((unsigned short*)A.spectra)[outbase + c] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
         } else {
           ((__hip_bfloat162*)A.spectra)[outbase + c] =
               __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
@@ -934,6 +967,7 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   if (G.status != 0) {
     for (int q = 0; q < 8; q++) {
       if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + k2 + M * q] = make_float2(0.f, 0.f);
+      else if (A.prec == FXG_XMAC_FP8) ((unsigned short*)A.spectra)[outbase + k2 + M * q] = 0;
       else ((unsigned*)A.spectra)[outbase + k2 + M * q] = 0u;
     }
     return;
@@ -949,7 +983,7 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   for (int n1 = 0; n1 < 16; n1++) v[n1] = z[(long long)n1 * M];
   DFT<16>::run(v);
   const int mh = B.lsb ? 8 : 0;          // USB keeps k1 < 8, real LSB k1 >= 8
-  const float scale = A.prec == FXG_XMAC_FP32 ? 1.0f : A.scale;
+  const float scale = (A.prec == FXG_XMAC_FP32 || A.prec == FXG_XMAC_FP8) ? 1.0f : A.scale;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int c = k2 + M * q;
@@ -958,6 +992,8 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
     const float2 y = make_float2((x.x * fr.x - x.y * fr.y) * scale, (x.x * fr.y + x.y * fr.x) * scale);
     if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + c] = y;
     else if (A.prec == FXG_XMAC_FP16) ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x, y.y);
+    else if (A.prec == FXG_XMAC_FP8)
+      ((unsigned short*)A.spectra)[outbase + c] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
     else ((__hip_bfloat162*)A.spectra)[outbase + c] = __float22bfloat162_rn(y);
   }
 }

@@ -247,6 +247,155 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP8 variant: spectra stored as e4m3 (re, im) byte pairs, each (band,
+// channel) scaled by 2^e in the F stage; v_mfma_scale_f32_32x32x64_f8f6f4
+// takes 32 blocks per step (K = 64) and applies 2^-e per input row and
+// column through its E8M0 block scales (one per lane: the lane's 32 values
+// are one input at one channel), so the accumulators hold the unscaled
+// products.  Staging goes through registers: two blocks of eight channels
+// per item become one dword (two pairs) per channel in LDS, [channel][input]
+// [block pair], so each lane's operand is two 16-byte LDS reads.
+// ---------------------------------------------------------------------------
+typedef int i8v_t __attribute__((ext_vector_type(8)));
+
+// (re, im) -> (im, -re) on each packed fp8 pair (sign = bit 7 of the byte)
+__device__ __forceinline__ unsigned hat8(unsigned v) {
+  return (((v >> 8) & 0x00FF00FFu) | ((v << 8) & 0xFF00FF00u)) ^ 0x80008000u;
+}
+
+__device__ __forceinline__ f16_t mfma_fp8(const uint4 (&a)[2], const uint4 (&b)[2], f16_t c, int sa, int sb) {
+  const i8v_t av = {(int)a[0].x, (int)a[0].y, (int)a[0].z, (int)a[0].w, (int)a[1].x, (int)a[1].y, (int)a[1].z, (int)a[1].w};
+  const i8v_t bv = {(int)b[0].x, (int)b[0].y, (int)b[0].z, (int)b[0].w, (int)b[1].x, (int)b[1].y, (int)b[1].z, (int)b[1].w};
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, sa, 0, sb);
+}
+
+// eight consecutive (re, im) pairs from element e
+__device__ __forceinline__ uint4 load8pairs(const unsigned short* p, long long e) {
+  if ((e & 7) == 0) return *(const uint4*)(p + e);
+  uint4 v;
+  v.x = p[e] | ((unsigned)p[e + 1] << 16);
+  v.y = p[e + 2] | ((unsigned)p[e + 3] << 16);
+  v.z = p[e + 4] | ((unsigned)p[e + 5] << 16);
+  v.w = p[e + 6] | ((unsigned)p[e + 7] << 16);
+  return v;
+}
+
+template <int T, int CG>
+__global__ __launch_bounds__(64 * CG) void k_xmac_fp8(XmacArgs A) {
+  // CG channels (wavefronts) per workgroup: 16 for one 32-input tile, 8 for
+  // two (the accumulators and operands of a wavefront need 128+ registers)
+  constexpr int TT = 32;                 // blocks per K step
+  constexpr int NR = 32 * T;             // inputs, padded to the tile size
+  constexpr int NTILE = T * (T + 1) / 2;
+  constexpr int TP = TT / 2 + 4;         // dwords per input row (16 block pairs + pad)
+  constexpr int CHS = NR * TP + 4;       // dwords per channel
+  __shared__ __align__(16) unsigned lds[CG * CHS];
+
+  const unsigned short* spec = (const unsigned short*)A.spec;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int nwg = gridDim.x;
+  const int bid = blockIdx.x;
+  const int grp = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;   // XCD-aware groups
+  const int c0 = grp * CG;
+  const int c = c0 + w;                  // this wavefront's channel
+  const int half = l >> 5, row = l & 31;
+  const int sb0 = blockIdx.y * A.split_blocks;
+  const int nb = min(A.nblocks - sb0, A.split_blocks);
+  float2* const Vout = A.V + (long long)blockIdx.y * A.vsplit;
+
+  // E8M0 scale of this lane's row (input 32 I + row) at channel c: 2^-e
+  int sc[T];
+#pragma unroll
+  for (int I = 0; I < T; I++) {
+    const int a = 32 * I + row;
+    sc[I] = a < A.nin ? 127 - (int)A.cexp[A.eoff[a] + c] : 127;
+  }
+  f16_t accR[NTILE], accI[NTILE];
+#pragma unroll
+  for (int q = 0; q < NTILE; q++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) { accR[q][e] = 0.0f; accI[q][e] = 0.0f; }
+
+  for (int t0 = 0; t0 < nb; t0 += TT) {
+    // ---- stage: items (input, block pair, channel octet)
+    constexpr int QN = CG / 8;           // channel octets
+    constexpr int NITEM = NR * (TT / 2) * QN;
+    for (int it = tid; it < NITEM; it += 64 * CG) {
+      const int q = it % QN;
+      const int p = (it / QN) % (TT / 2);
+      const int a = it / (QN * (TT / 2));
+      uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+      if (a < A.nin) {
+        const long long e0 = A.in_off[a] + (long long)(sb0 + t0 + 2 * p) * A.block_stride + c0 + 8 * q;
+        if (t0 + 2 * p < nb) v0 = load8pairs(spec, e0);
+        if (t0 + 2 * p + 1 < nb) v1 = load8pairs(spec, e0 + A.block_stride);
+      }
+      const unsigned u0[4] = {v0.x, v0.y, v0.z, v0.w}, u1[4] = {v1.x, v1.y, v1.z, v1.w};
+      unsigned* d = lds + (8 * q) * CHS + a * TP + p;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const unsigned lo = (u0[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+        const unsigned hi = (u1[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+        d[k * CHS] = lo | (hi << 16);
+      }
+    }
+    __syncthreads();
+    // ---- one K step: lane (row, half) holds blocks 16 half .. 16 half + 15
+    const unsigned* base = lds + w * CHS;
+    uint4 f[T][2];
+#pragma unroll
+    for (int I = 0; I < T; I++) {
+      const unsigned* rp = base + (32 * I + row) * TP + 8 * half;
+      f[I][0] = *(const uint4*)rp;
+      f[I][1] = *(const uint4*)(rp + 4);
+    }
+    int q = 0;
+#pragma unroll
+    for (int I = 0; I < T; I++) {
+      uint4 fh[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        fh[h] = make_uint4(hat8(f[I][h].x), hat8(f[I][h].y), hat8(f[I][h].z), hat8(f[I][h].w));
+#pragma unroll
+      for (int J = I; J < T; J++, q++) {
+        accR[q] = mfma_fp8(f[I], f[J], accR[q], sc[I], sc[J]);
+        accI[q] = mfma_fp8(fh, f[J], accI[q], sc[I], sc[J]);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (c >= A.nchan) return;
+  int q = 0;
+#pragma unroll
+  for (int I = 0; I < T; I++)
+#pragma unroll
+    for (int J = I; J < T; J++, q++) {
+      const int b = 32 * J + row;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int a = 32 * I + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (a >= A.nin || b >= A.nin) continue;
+        float2* p = Vout + ((long long)a * A.nin + b) * A.nchan + c;
+        float2 v = make_float2(accR[q][r], accI[q][r]);
+        if (A.accumulate) { float2 o = *p; v.x += o.x; v.y += o.y; }
+        *p = v;
+      }
+    }
+}
+
+hipError_t launch_xmac_fp8(const XmacArgs& a, hipStream_t stream) {
+  if (!(a.nin <= 64 && a.nchan % 16 == 0) || !a.cexp || !a.eoff) return hipErrorInvalidValue;
+  const unsigned rows = (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks);
+  if (a.nin <= 32)
+    hipLaunchKernelGGL((k_xmac_fp8<1, 16>), dim3((unsigned)(a.nchan / 16), rows), dim3(1024), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_xmac_fp8<2, 8>), dim3((unsigned)(a.nchan / 8), rows), dim3(512), 0, stream, a);
+  return hipGetLastError();
+}
+
 bool xmac_mfma_supported(int nin, int nchan) { return nin <= 64 && nchan % 16 == 0; }
 
 bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host) {

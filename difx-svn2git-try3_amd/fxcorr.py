@@ -31,12 +31,13 @@ FXG_BLANKER_NONE = 1
 FXG_XMAC_FP32 = 0
 FXG_XMAC_FP16 = 1
 FXG_XMAC_BF16 = 2
+FXG_XMAC_FP8 = 3
 FXG_OK, FXG_EINVAL, FXG_ENOMEM, FXG_EHIP, FXG_EUNSUPPORTED, FXG_ESTATE, FXG_ENODEV = 0, -1, -2, -3, -4, -5, -6
 FXG_INVALID_SUBINT = -99999999
 FXG_FLAGS_PER_INT = 30
 STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h", "pcal"]
 
-XMAC_NAMES = {"fp32": FXG_XMAC_FP32, "fp16": FXG_XMAC_FP16, "bf16": FXG_XMAC_BF16}
+XMAC_NAMES = {"fp32": FXG_XMAC_FP32, "fp16": FXG_XMAC_FP16, "bf16": FXG_XMAC_BF16, "fp8": FXG_XMAC_FP8}
 
 
 class Freq(C.Structure):
@@ -369,6 +370,7 @@ def load_gpu_lib():
     lib.fxg_job_autotune.argtypes = [C.c_void_p, P(SubintDesc), P(StationInput), C.c_int, C.c_int,
                                      P(TuneResult), C.c_int, P(C.c_int)]
     lib.fxg_job_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.fxg_job_calibrate_scales.argtypes = [C.c_void_p]
     lib.fxg_job_graph_captures.argtypes = [C.c_void_p, P(C.c_int64)]
     lib.fxg_batch_create.argtypes = [C.c_void_p, P(C.c_void_p), C.c_int, P(C.c_void_p)]
     lib.fxg_batch_destroy.argtypes = [C.c_void_p]

@@ -66,6 +66,9 @@ extern "C" {
 #define FXG_XMAC_FP32       0   /* plain drop-in: cf32 spectra, FP32 FMA     */
 #define FXG_XMAC_FP16       1   /* fp16 spectra, MFMA, FP32 accumulation     */
 #define FXG_XMAC_BF16       2   /* bf16 spectra, MFMA, FP32 accumulation     */
+#define FXG_XMAC_FP8        3   /* fp8 e4m3 spectra with a power-of-two scale
+                                   per (band, channel), microscaled MFMA, FP32
+                                   accumulation                               */
 
 /* Sentinel used by Mode for a torched subintegration (mode.h:232). */
 #define FXG_INVALID_SUBINT  (-99999999)
@@ -409,6 +412,14 @@ int fxg_batch_process(fxg_batch* batch, const fxg_subint_desc* subs,
                       const fxg_station_input* const* stations, int data_on_device,
                       float* const* results, int results_on_device);
 int fxg_batch_graph_captures(const fxg_batch* batch, int64_t* captures);
+
+/* FP8 jobs: the power-of-two scale of every (band, channel) is set so that
+ * the channel's RMS spectrum magnitude maps to 16 in e4m3 (largest value
+ * 448): from the quantiser's variance at job creation, and from measured
+ * power by this call, which reads the autocorrelations of the last
+ * subintegration the job processed.  The products are exact in the scale
+ * (the matrix cores apply its inverse), so the scales change rounding only. */
+int fxg_job_calibrate_scales(fxg_job* job);
 
 /* Debug taps valid after fxg_subint_process with keep_taps set. */
 int fxg_job_set_keep_taps(fxg_job* job, int keep);

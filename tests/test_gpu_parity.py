@@ -429,3 +429,32 @@ def test_phase_cal(gpu, oracle, case):
         assert np.all(np.abs(d[fs_edge]) < 20.0), d
         assert np.mean(np.abs(d[fs_edge])) < 6.0, d
     g.close()
+
+
+FP8_TOL = 0.08   # e4m3 keeps 3 mantissa bits: ~4 % per product, averaged over the blocks
+
+
+@pytest.mark.parametrize("mk", ["c2", "wide", "zoom"])
+def test_visibilities_fp8_vs_fp64(gpu, oracle, mk):
+    """fp8 e4m3 spectra with per-(band, channel) power-of-two scales on the
+    microscaled matrix cores, against the FP64 oracle: with the default
+    scales and again after calibrating them on the subintegration's own
+    power; a wrong scale would be off by powers of two."""
+    lib, ctx = gpu
+    if mk == "zoom":
+        spec = c2_spec(xmac="fp8", zoom=((0, 128, 256), (1, 40, 64)))
+    else:
+        spec = {"c2": c2_spec, "wide": wide_spec}[mk](xmac="fp8")
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [1]})
+    g = GpuJob(lib, ctx, job)
+    ref = oracle_results(oracle, job, sd, fp64=True, nthreads=8)
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    first = g.process(sd)
+    check(lib, lib.fxg_job_calibrate_scales(g.h), "calibrate")
+    second = g.process(sd)
+    for got in (first, second):
+        assert rel_err(got[:xend], ref[:xend]) < FP8_TOL, rel_err(got[:xend], ref[:xend])
+        assert rel_err(got[a0:a1], ref[a0:a1]) < FP8_TOL / 2, rel_err(got[a0:a1], ref[a0:a1])
+    g.close()
