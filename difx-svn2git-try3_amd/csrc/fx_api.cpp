@@ -83,6 +83,7 @@ struct fxg_job {
   // big buffers
   DBuf d_spec, d_V;
   DBuf d_z;                       // FFTs above 16384 points: the sub-FFT scratch
+  DBuf d_rtab;                    // and the rotator tables of its blocks
   DBuf d_pcal, d_pcalaux, d_pcaltone, d_pcalpart;   // phase-cal tables and partial sums
   DBuf d_vpart;                   // block-split XMAC partial matrices (zeroed once)
   DBuf d_cexp, d_eoff;            // fp8: scale exponents [gband][channel], per-input offsets
@@ -392,7 +393,7 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
-                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z,
+                  &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z, &j->d_rtab,
                   &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart, &j->d_vpart, &j->d_cexp,
                   &j->d_eoff};
   for (DBuf* b : bufs) b->release();
@@ -476,7 +477,8 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     tw[64 + h] = make_float2((float)std::cos(ang), (float)std::sin(ang));
   }
   size_t specbytes = (size_t)P.nbands_total * chunk * P.C * spectrum_bytes(prec);
-  const size_t zbytes = P.L > kFStageMaxL ? (size_t)P.nbands_total * chunk * P.L * sizeof(float2) : 0;
+  const size_t zbytes = P.L > kFStageMaxL ? (size_t)P.nbands_total * P.zcap * P.L * sizeof(float2) : 0;
+  const size_t rtbytes = P.L > kFStageMaxL ? (size_t)P.nbands_total * P.zcap * P.rtabn * sizeof(float2) : 0;
   if ((rc = j->d_st.ensure(sizeof(DevStation) * P.st.size())) ||
       (rc = j->d_bands.ensure(sizeof(DevBand) * P.bands.size())) ||
       (rc = j->d_segs.ensure(sizeof(DevSeg) * std::max<size_t>(1, P.segs.size()))) ||
@@ -488,6 +490,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
       (rc = j->d_results.ensure(sizeof(float) * 2 * (size_t)std::max<long long>(1, P.info.result_length))) ||
       (rc = j->d_spec.ensure(specbytes)) ||
       (zbytes && (rc = j->d_z.ensure(zbytes))) ||
+      (rtbytes && (rc = j->d_rtab.ensure(rtbytes))) ||
       (rc = j->d_V.ensure(sizeof(float2) * (size_t)std::max<long long>(1, P.vsize))))
     return fail(rc);
   if (hipMemcpy(j->d_st.p, P.st.data(), sizeof(DevStation) * P.st.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -763,7 +766,10 @@ static void fill_fstage_args(fxg_job* j, FStageArgs* fa) {
   fa->scale = j->scale;
   fa->nblocks_subint = j->desc.blockspersend;
   fa->zbuf = j->d_z.p ? j->d_z.as<float2>() : nullptr;
-  fa->chunkn = j->plan.chunk_blocks;
+  fa->rtab = j->d_rtab.p ? j->d_rtab.as<float2>() : nullptr;
+  fa->zcap = j->plan.zcap;
+  fa->rtabn = j->plan.rtabn;
+  fa->zcb0 = 0;
   fa->cexp = j->d_cexp.p ? j->d_cexp.as<signed char>() : nullptr;
 }
 

@@ -138,6 +138,8 @@ struct Plan {
   int pcal_maxtones = 0;
   // block-split XMAC: complex elements of the partial-matrix buffer
   long long vpart_elems = 0;
+  // FFTs above 16384 points: blocks per sub-chunk and rotator-table entries per (band, block)
+  int zcap = 0, rtabn = 0;
 };
 
 // bytes of one spectrum value (complex) entering the cross-multiply

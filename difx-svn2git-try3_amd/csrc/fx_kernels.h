@@ -29,8 +29,12 @@ struct FStageArgs {
   float scale;                // applied before reduced-precision stores
   int nblocks_subint;
   int chunk0;                 // first block of the chunk
-  float2* zbuf;               // FFTs above 16384 points: [band][chunk block][16][L/16] scratch
-  int chunkn;                 // blocks per chunk (the scratch's block dimension)
+  // FFTs above 16384 points run zcap blocks of the chunk at a time:
+  float2* zbuf;               // sub-FFT scratch [band][zcap][16][L/16]
+  float2* rtab;               // rotator tables [band][zcap][rtabn] (sub, step, quad, frac sub, frac step)
+  int zcap;                   // blocks per sub-chunk
+  int rtabn;                  // table entries per (band, block)
+  int zcb0;                   // first chunk block of this sub-chunk
   const signed char* cexp;    // fp8: power-of-two scale exponent per [global band][channel]
 };
 

@@ -607,12 +607,17 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
 
   // ---- spectra chunking: keep the spectra of at most ~16 GiB resident
   long long per_block = (long long)P.nbands_total * P.C * spectrum_bytes(d->xmac_precision);
-  if (P.L > 16384) per_block += (long long)P.nbands_total * P.L * 8;     // the sub-FFT scratch
   long long budget = 16LL << 30;
   int chunk = d->max_chunk_blocks > 0 ? d->max_chunk_blocks : (int)std::max(1LL, budget / per_block);
   chunk = std::min(chunk, d->blockspersend);
   P.chunk_blocks = chunk;
   P.info.chunk_blocks = chunk;
+  // FFTs above 16384 points: a scratch of about 2 GiB takes zcap blocks of
+  // the chunk at a time, with their rotator tables
+  if (P.L > 16384) {
+    P.zcap = (int)std::max(1LL, std::min<long long>(chunk, (2LL << 30) / ((long long)P.nbands_total * P.L * 8)));
+    for (const DevStation& st : P.st) P.rtabn = std::max(P.rtabn, 3 * st.s + st.nfrsteps + st.nfracsteps);
+  }
   // partial matrices for XMAC launches with few channel groups (block split):
   // up to 512 workgroups' worth, at most 256 MiB
   for (const XmacFreq& x : P.xf) {
@@ -634,7 +639,7 @@ int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d) {
   long long blocks = (long long)p.nds * d->blockspersend * (long long)sizeof(DevBlock);
   long long res = p.info.result_length * 8;
   long long tables = (long long)p.segs.size() * sizeof(DevSeg) + p.bands.size() * sizeof(DevBand) + (1 << 20);
-  long long z = p.L > 16384 ? (long long)p.nbands_total * p.chunk_blocks * p.L * 8 : 0;
+  long long z = p.L > 16384 ? (long long)p.nbands_total * p.zcap * ((long long)p.L + p.rtabn) * 8 : 0;
   long long vpart = p.vpart_elems * 8;
   long long pcal = p.pcal_partfloats * 4 + (long long)p.pcal_aux.size() * 4 +
                    (long long)p.pcal.size() * sizeof(DevPcal) + (long long)p.pcal_tonebin.size() * 4;

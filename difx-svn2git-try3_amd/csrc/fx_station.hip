@@ -823,6 +823,23 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
 // over n1, keeps the sideband's eight outputs and applies the
 // fractional-sample rotation.
 // ---------------------------------------------------------------------------
+// rotator tables of each (band, block) of a sub-chunk, once, for the 16
+// workgroups of k_bigfft_a and the rows of k_bigfft_b that read them:
+// [sub s][step nfrsteps][quad s][frac sub s][frac step nfracsteps]
+template <int L>
+__global__ __launch_bounds__(256) void k_rotab(FStageArgs A) {
+  const int cbl = blockIdx.x;
+  const int blk = A.chunk0 + A.zcb0 + cbl;
+  const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
+  const DevBand B = A.bands[gband];
+  const DevStation S = A.st[B.station];
+  const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
+  if (G.status != 0) return;
+  float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn;
+  rotator_tables(B, S, G, L, threadIdx.x, 256, true, true, rt, rt + S.s, rt + 2 * S.s + S.nfrsteps,
+                 rt + 3 * S.s + S.nfrsteps, rt + S.s + S.nfrsteps);
+}
+
 template <int M, int NBITS>
 __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   using Sh = FftShape<M>;
@@ -834,8 +851,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   __shared__ float lut8s[NBITS == 8 ? 256 : 4];
 
   const int tid = threadIdx.x;
-  const int cb = blockIdx.x, n1 = blockIdx.z;
-  const int blk = A.chunk0 + cb;
+  const int cbl = blockIdx.x, n1 = blockIdx.z;   // block within the sub-chunk, decimation phase
+  const int blk = A.chunk0 + A.zcb0 + cbl;
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
   const DevStation S = A.st[B.station];
@@ -843,12 +860,17 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
   if (G.status != 0) return;                     // k_bigfft_b writes the zero spectrum
 
+  // the pre-F rotator tables k_rotab computed for this (band, block)
   float2* rsub = tabs;
   float2* rstep = tabs + S.s;
   float2* rquad = rstep + S.nfrsteps;
   for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
   for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
-  rotator_tables(B, S, G, L, tid, NT, true, false, rsub, rstep, nullptr, nullptr, rquad);
+  {
+    const float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn;
+    const int nrot = S.order == 0 ? 0 : (S.order == 2 ? 2 * S.s : S.s) + S.nfrsteps;
+    for (int i = tid; i < nrot; i += NT) tabs[i] = rt[i];
+  }
   if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
   else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
   __syncthreads();
@@ -928,7 +950,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   // ---- last pass, every output, times W_L^(n1 k2), to the scratch
   if (tid < NTF) {
     const int j = tid;
-    float2* z = A.zbuf + (((long long)gband * A.chunkn + cb) * 16 + n1) * M;
+    float2* z = A.zbuf + (((long long)gband * A.zcap + cbl) * 16 + n1) * M;
 #pragma unroll
     for (int u = 0; u < 16 / RL; u++) {
       const int k = j + u * NTF;
@@ -956,7 +978,8 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   constexpr int L = 16 * M, C = L / 2;
   __shared__ float2 tabs[fxg_rotator_table_entries(L)];
   const int tid = threadIdx.x;
-  const int cb = blockIdx.x;
+  const int cbl = blockIdx.x;                    // block within the sub-chunk
+  const int cb = A.zcb0 + cbl;                   // block within the chunk
   const int blk = A.chunk0 + cb;
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
@@ -974,10 +997,13 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   }
   float2* fsub = tabs;
   float2* fstep = tabs + S.s;
-  rotator_tables(B, S, G, L, tid, 256, false, true, nullptr, nullptr, fsub, fstep, nullptr);
+  {
+    const float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn + 2 * S.s + S.nfrsteps;
+    for (int i = tid; i < S.s + S.nfracsteps; i += 256) tabs[i] = rt[i];
+  }
   __syncthreads();
   const int s = S.s, slog = S.slog;
-  const float2* z = A.zbuf + ((long long)gband * A.chunkn + cb) * 16 * M + k2;
+  const float2* z = A.zbuf + ((long long)gband * A.zcap + cbl) * 16 * M + k2;
   float2 v[16];
 #pragma unroll
   for (int n1 = 0; n1 < 16; n1++) v[n1] = z[(long long)n1 * M];
@@ -1008,6 +1034,8 @@ template __global__ void k_bigfft_a<4096, 2>(FStageArgs);
 template __global__ void k_bigfft_a<4096, 8>(FStageArgs);
 template __global__ void k_bigfft_b<2048>(FStageArgs);
 template __global__ void k_bigfft_b<4096>(FStageArgs);
+template __global__ void k_rotab<32768>(FStageArgs);
+template __global__ void k_rotab<65536>(FStageArgs);
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1038,11 +1066,17 @@ hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int 
     CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192) CASE(16384)
 #undef CASE
 #define BIG(Mv) case 16 * Mv: { \
-      if (!a.zbuf) return hipErrorInvalidValue; \
-      dim3 ga(nblocks, nbands, 16), gb(nblocks, nbands, Mv / 256); \
-      if (nbits == 2) hipLaunchKernelGGL((k_bigfft_a<Mv, 2>), ga, dim3(FftShape<Mv>::NT), 0, stream, a); \
-      else hipLaunchKernelGGL((k_bigfft_a<Mv, 8>), ga, dim3(FftShape<Mv>::NT), 0, stream, a); \
-      hipLaunchKernelGGL((k_bigfft_b<Mv>), gb, dim3(256), 0, stream, a); \
+      if (!a.zbuf || !a.rtab || a.zcap < 1) return hipErrorInvalidValue; \
+      for (int zc = 0; zc < nblocks; zc += a.zcap) { \
+        FStageArgs b = a; \
+        b.zcb0 = zc; \
+        const int n = nblocks - zc < a.zcap ? nblocks - zc : a.zcap; \
+        dim3 gr(n, nbands), ga(n, nbands, 16), gb(n, nbands, Mv / 256); \
+        hipLaunchKernelGGL((k_rotab<16 * Mv>), gr, dim3(256), 0, stream, b); \
+        if (nbits == 2) hipLaunchKernelGGL((k_bigfft_a<Mv, 2>), ga, dim3(FftShape<Mv>::NT), 0, stream, b); \
+        else hipLaunchKernelGGL((k_bigfft_a<Mv, 8>), ga, dim3(FftShape<Mv>::NT), 0, stream, b); \
+        hipLaunchKernelGGL((k_bigfft_b<Mv>), gb, dim3(256), 0, stream, b); \
+      } \
       break; }
     BIG(2048) BIG(4096)
 #undef BIG
