@@ -28,7 +28,8 @@ HIP_FLAGS = [
     f"-I{os.path.join(REPO, 'include')}",
 ]
 
-SOURCES = ["fx_api.cpp", "fx_plan.cpp", "fx_station.hip", "fx_xmac.hip", "fx_xmac_mfma.hip", "fx_pcal.hip", "fx_mux.cpp", "fx_mux.hip"]
+SOURCES = ["fx_api.cpp", "fx_plan.cpp", "fx_station.hip", "fx_xmac.hip", "fx_xmac_mfma.hip", "fx_pcal.hip", "fx_mux.cpp",
+           "fx_mux.hip", "fx_kurtosis.hip"]
 
 
 def _run(cmd):

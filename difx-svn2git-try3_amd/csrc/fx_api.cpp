@@ -87,6 +87,9 @@ struct fxg_job {
   DBuf d_pcal, d_pcalaux, d_pcaltone, d_pcalpart;   // phase-cal tables and partial sums
   DBuf d_vpart;                   // block-split XMAC partial matrices (zeroed once)
   DBuf d_cexp, d_eoff;            // fp8: scale exponents [gband][channel], per-input offsets
+  // spectral kurtosis: sums [record][s1, s2][gband][C], records [record][gband][nout], flags
+  int kurt_on = 0, kurt_maxch = 32, kurt_nrec = 1, kurt_nout = 0, kurt_have = 0;
+  DBuf d_kurt, d_kurtout, d_kurtnz;
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
@@ -395,7 +398,7 @@ static void job_free(fxg_job* j) {
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
                   &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z, &j->d_rtab,
                   &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart, &j->d_vpart, &j->d_cexp,
-                  &j->d_eoff};
+                  &j->d_eoff, &j->d_kurt, &j->d_kurtout, &j->d_kurtnz};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
@@ -618,6 +621,60 @@ int fxg_job_graph_captures(const fxg_job* j, int64_t* captures) {
   return 0;
 }
 
+int fxg_job_set_kurtosis(fxg_job* j, int enable, int maxchannels, int nthreads) {
+  if (!j) return set_err(FXG_EINVAL, "null argument");
+  if (!enable) {
+    j->kurt_on = 0;
+    j->kurt_have = 0;
+    j->d_kurt.release(); j->d_kurtout.release(); j->d_kurtnz.release();
+    return 0;
+  }
+  if (maxchannels < 1) return set_err(FXG_EINVAL, "kurtosis channels must be positive");
+  if (nthreads < 1 || nthreads > 1024) return set_err(FXG_EINVAL, "kurtosis records (threads) must be in [1, 1024]");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  HIPCHK(hipStreamSynchronize(j->stream));
+  const Plan& P = j->plan;
+  const long long nb = (long long)P.bands.size();
+  const int nout = std::min(maxchannels, P.C);
+  int rc;
+  if ((rc = j->d_kurt.ensure(sizeof(float) * 2 * (size_t)(nthreads * nb * P.C))) ||
+      (rc = j->d_kurtout.ensure(sizeof(float) * (size_t)(nthreads * nb * nout))) ||
+      (rc = j->d_kurtnz.ensure(sizeof(int) * (size_t)(nthreads * nb)))) {
+    j->kurt_on = 0;
+    return rc;
+  }
+  j->kurt_on = 1;
+  j->kurt_maxch = maxchannels;
+  j->kurt_nrec = nthreads;
+  j->kurt_nout = nout;
+  j->kurt_have = 0;
+  return 0;
+}
+
+int fxg_job_get_kurtosis(fxg_job* j, int thread, int ds, int band, float* out, int maxout, int32_t* nchan,
+                         int32_t* valid) {
+  if (!j || !out || !nchan || !valid) return set_err(FXG_EINVAL, "null argument");
+  if (!j->kurt_on) return set_err(FXG_ESTATE, "kurtosis is off (fxg_job_set_kurtosis)");
+  if (!j->kurt_have) return set_err(FXG_ESTATE, "no subintegration processed since kurtosis was enabled");
+  const Plan& P = j->plan;
+  if (thread < 0 || thread >= j->kurt_nrec) return set_err(FXG_EINVAL, "thread");
+  if (ds < 0 || ds >= P.nds) return set_err(FXG_EINVAL, "datastream");
+  if (band < 0 || band >= P.st[ds].nchan) return set_err(FXG_EINVAL, "recorded band");
+  if (maxout < j->kurt_nout) return set_err(FXG_EINVAL, "output too short");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  HIPCHK(hipStreamSynchronize(j->stream));
+  const long long nb = (long long)P.bands.size();
+  const long long g = P.st[ds].band0 + band;
+  const long long glast = P.st[ds].band0 + P.st[ds].nchan - 1;
+  HIPCHK(hipMemcpy(out, j->d_kurtout.as<float>() + ((long long)thread * nb + g) * j->kurt_nout,
+                   sizeof(float) * (size_t)j->kurt_nout, hipMemcpyDeviceToHost));
+  int nz = 0;
+  HIPCHK(hipMemcpy(&nz, j->d_kurtnz.as<int>() + (long long)thread * nb + glast, sizeof(int), hipMemcpyDeviceToHost));
+  *nchan = j->kurt_nout;
+  *valid = nz;
+  return 0;
+}
+
 int fxg_job_set_keep_taps(fxg_job* j, int keep) {
   if (!j) return set_err(FXG_EINVAL, "null job");
   j->keep_taps = keep;
@@ -800,6 +857,16 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
   const int chunk = j->chunk_blocks > 0 ? std::min(j->chunk_blocks, P.chunk_blocks) : P.chunk_blocks;
   FStageArgs fa;
   fill_fstage_args(j, &fa);
+  KurtArgs ka{};
+  if (j->kurt_on) {
+    ka.spectra = j->d_spec.p; ka.prec = prec; ka.invscale = 1.0f / j->scale;
+    ka.cexp = j->d_cexp.p ? j->d_cexp.as<signed char>() : nullptr;
+    ka.nbands = (int)P.bands.size(); ka.C = P.C; ka.nout = j->kurt_nout;
+    ka.band_stride = (long long)P.chunk_blocks * P.C;
+    ka.nblocks = nblocks; ka.nrec = j->kurt_nrec; ka.s = j->d_kurt.as<float>();
+    // Mode::zeroKurtosis at the start of every subintegration (core.cpp:712-714)
+    HIPCHK(launch_zero(ka.s, 2LL * ka.nrec * ka.nbands * ka.C, s));
+  }
   for (int c0 = 0; c0 < nblocks; c0 += chunk) {
     int cn = std::min(chunk, nblocks - c0);
     fa.chunk0 = c0;
@@ -813,6 +880,12 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       HIPCHK(launch_fstage(P.L, 8, fa, j->nb8, cn, s));
     }
     if ((rc = j->ev_end())) return rc;
+    if (j->kurt_on) {
+      ka.chunk0 = c0; ka.nchunk = cn;
+      if ((rc = j->ev_begin(FXG_STAGE_KURTOSIS))) return rc;
+      HIPCHK(launch_kurt_accum(ka, s));
+      if ((rc = j->ev_end())) return rc;
+    }
     for (size_t x = 0; x < P.xf.size(); x++) {
       const XmacFreq& xf = P.xf[x];
       XmacArgs xa{};
@@ -852,6 +925,11 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
     j->tap_chunk0 = c0;
     j->tap_chunkn = cn;
   }
+  if (j->kurt_on) {
+    if ((rc = j->ev_begin(FXG_STAGE_KURTOSIS))) return rc;
+    HIPCHK(launch_kurt_final(ka, j->d_kurtout.as<float>(), j->d_kurtnz.as<int>(), s));
+    if ((rc = j->ev_end())) return rc;
+  }
   if ((rc = j->ev_begin(FXG_STAGE_WEIGHTS))) return rc;
   HIPCHK(launch_weights(j->d_blocks.as<DevBlock>(), nblocks, j->d_blpairs.as<int2>(), P.nbl, P.nds,
                         j->d_weights.as<float>(), s));
@@ -887,8 +965,11 @@ static std::vector<unsigned char> graph_key(const fxg_job* j, const LaunchShape&
   put(&dres, sizeof(dres));
   const void* ptrs[4] = {j->d_ss.p, j->d_flags.p, j->d_ftab.p, j->d_ct.p};
   put(ptrs, sizeof(ptrs));
-  const int sched[4] = {j->xmac_streaming, j->xmac_ct, j->xmac_tb, j->chunk_blocks};
+  const int sched[8] = {j->xmac_streaming, j->xmac_ct, j->xmac_tb, j->chunk_blocks,
+                        j->kurt_on, j->kurt_maxch, j->kurt_nrec, 0};
   put(sched, sizeof(sched));
+  const void* kptrs[3] = {j->d_kurt.p, j->d_kurtout.p, j->d_kurtnz.p};
+  put(kptrs, sizeof(kptrs));
   return k;
 }
 
@@ -945,6 +1026,7 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
     if ((rc = j->ev_end())) return rc;
   }
   j->have_subint = 1;
+  j->kurt_have = j->kurt_on;
   return 0;
 }
 
@@ -1233,7 +1315,7 @@ int fxg_batch_process(fxg_batch* b, const fxg_subint_desc* subs, const fxg_stati
                             hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
-  for (fxg_job* j : b->jobs) j->have_subint = 1;
+  for (fxg_job* j : b->jobs) { j->have_subint = 1; j->kurt_have = j->kurt_on; }
   return 0;
 }
 

@@ -1,6 +1,6 @@
 // fx_kernels.h — kernel argument blocks and launchers of libfxcorr_gpu
-// (implemented in fx_station.hip, fx_xmac.hip, fx_xmac_mfma.hip and
-// fx_pcal.hip).
+// (implemented in fx_station.hip, fx_xmac.hip, fx_xmac_mfma.hip,
+// fx_pcal.hip, fx_mux.hip and fx_kurtosis.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fx_internal.h"
@@ -71,6 +71,22 @@ struct PcalArgs {
   int nblocks;                // blocks per subintegration
   int ngroups;                // block groups per band
 };
+
+struct KurtArgs {
+  const void* spectra;        // [band][pass block][C] of the job's precision
+  int prec;
+  float invscale;             // 1 / the fp16 / bf16 spectrum scale
+  const signed char* cexp;    // fp8: scale exponent per [band][channel]
+  int nbands, C, nout;        // recorded bands, channels, record channels
+  long long band_stride;      // spectrum elements between bands
+  int chunk0, nchunk;         // blocks of this pass
+  int nblocks, nrec;          // blocks per subintegration, records (threads)
+  float* s;                   // [record][s1, s2][band][C]
+};
+// s1/s2 of one pass's blocks; the records' sk into out [record][band][nout]
+// and the nonzero flag of each (record, band)
+hipError_t launch_kurt_accum(const KurtArgs& a, hipStream_t stream);
+hipError_t launch_kurt_final(const KurtArgs& a, float* out, int* nzflag, hipStream_t stream);
 
 struct MuxArgs {
   const unsigned char* src;     // input bytes on the device

@@ -35,7 +35,7 @@ FXG_XMAC_FP8 = 3
 FXG_OK, FXG_EINVAL, FXG_ENOMEM, FXG_EHIP, FXG_EUNSUPPORTED, FXG_ESTATE, FXG_ENODEV = 0, -1, -2, -3, -4, -5, -6
 FXG_INVALID_SUBINT = -99999999
 FXG_FLAGS_PER_INT = 30
-STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h", "pcal"]
+STAGE_NAMES = ["frames", "prologue", "fstage", "xmac", "weights", "scatter", "h2d", "d2h", "pcal", "kurtosis"]
 
 XMAC_NAMES = {"fp32": FXG_XMAC_FP32, "fp16": FXG_XMAC_FP16, "bf16": FXG_XMAC_BF16, "fp8": FXG_XMAC_FP8}
 
@@ -371,6 +371,9 @@ def load_gpu_lib():
                                      P(TuneResult), C.c_int, P(C.c_int)]
     lib.fxg_job_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_calibrate_scales.argtypes = [C.c_void_p]
+    lib.fxg_job_set_kurtosis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    lib.fxg_job_get_kurtosis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(C.c_float), C.c_int,
+                                         P(C.c_int32), P(C.c_int32)]
     lib.fxg_job_graph_captures.argtypes = [C.c_void_p, P(C.c_int64)]
     lib.fxg_batch_create.argtypes = [C.c_void_p, P(C.c_void_p), C.c_int, P(C.c_void_p)]
     lib.fxg_batch_destroy.argtypes = [C.c_void_p]
@@ -445,6 +448,8 @@ def load_oracle():
     lib.orc_process.argtypes = [P(JobDesc), P(SubintDesc), P(StationInput), C.c_int, C.c_int,
                                 C.c_int, C.c_int, C.c_void_p]
     lib.orc_fft.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    lib.orc_kurtosis.argtypes = [P(JobDesc), P(SubintDesc), P(StationInput), C.c_int, C.c_int, C.c_int,
+                                 P(C.c_float), P(C.c_int)]
     lib.orc_delay_interpolator.argtypes = [P(C.c_double), C.c_int, C.c_double, C.c_double,
                                            C.c_int, P(C.c_double)]
     _oracle = lib

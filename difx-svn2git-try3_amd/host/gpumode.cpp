@@ -215,6 +215,17 @@ f32 GpuMode::getWeight(bool crosspol, int band) const {
   return acw_[crosspol ? 1 : 0][band];
 }
 
+bool GpuMode::calculateAndAverageKurtosis(int thread) {
+  kurtchan_ = std::min(core_->kurtchan_, nchan_);
+  kurt_.assign((size_t)nrec_ * kurtchan_, 0.0f);
+  int32_t n = 0, valid = 0;
+  for (int b = 0; b < nrec_; b++) {
+    if (fxg_job_get_kurtosis(core_->job_, thread, ds_, b, &kurt_[(size_t)b * kurtchan_], kurtchan_, &n, &valid) != 0)
+      return false;
+  }
+  return valid != 0;
+}
+
 long long GpuMode::getEstimatedBytes() const {
   return (long long)(sizeof(cf32) * (2 * freqs_.size() * freqs_[0].size() + 2 * ac_[0].size()));
 }
@@ -288,6 +299,14 @@ int GpuCore::processdata(cf32* results) {
   gather(in, sub);
   int rc = fxg_subint_process(job_, &sub, in.data(), 0, (float*)results, 0);
   if (rc) err_ = fxg_last_error();
+  return rc;
+}
+
+int GpuCore::setDumpKurtosis(bool dump, int stadumpchannels, int numprocessthreads) {
+  if (!ok_) return FXG_ESTATE;
+  int rc = fxg_job_set_kurtosis(job_, dump ? 1 : 0, stadumpchannels, numprocessthreads);
+  if (rc) err_ = fxg_last_error();
+  else kurtchan_ = stadumpchannels;
   return rc;
 }
 

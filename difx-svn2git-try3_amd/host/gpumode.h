@@ -84,6 +84,13 @@ class GpuMode {
   const cf32* getAutocorrelation(bool crosspol, int outputband) const;
   f32 getWeight(bool crosspol, int outputband) const;
   bool initialisedOK() const { return initok_; }
+  // Spectral kurtosis of Core process thread `thread` over its blocks of the
+  // last processdata (GpuCore::setDumpKurtosis): fetches every recorded
+  // band's record and returns whether it is valid (mode.cpp:1426-1478);
+  // getKurtosis(band) then points at that band's channels (mode.h:154).
+  bool calculateAndAverageKurtosis(int thread = 0);
+  const f32* getKurtosis(int outputband) const { return &kurt_[(size_t)outputband * kurtchan_]; }
+  int getKurtosisChannels() const { return kurtchan_; }
   long long getEstimatedBytes() const;
   bool writeCrossAutoCorrs() const;
   int datastream() const { return ds_; }
@@ -109,6 +116,9 @@ class GpuMode {
   // autocorrelations[2][recorded band][chan] and weights[2][recorded band]
   std::vector<cf32> ac_[2];
   std::vector<f32> acw_[2];
+  // the fetched kurtosis records [recorded band][kurtchan_]
+  std::vector<f32> kurt_;
+  int kurtchan_ = 0;
 };
 
 // Mk5Mode: the mark5access-decoded formats (VDIF, Mark5B)
@@ -156,6 +166,10 @@ class GpuCore {
   long long getCoreResultAutocorrOffset(int ds) const;
   long long getCoreResultACWeightOffset(int ds) const;
   const fxg_job_desc& desc() const { return desc_; }
+  // Configuration::dumpKurtosis, getSTADumpChannels and Core's number of
+  // process threads (one record each, core.cpp:1054-1056): kurtosis is
+  // accumulated from the next processdata on.  Returns 0 or an FXG_E* code.
+  int setDumpKurtosis(bool dump, int stadumpchannels = 32, int numprocessthreads = 1);
   int scanStartSec() const { return scanstart_; }
   void setScanStartSec(int s) { scanstart_ = s; }
 
@@ -175,6 +189,7 @@ class GpuCore {
   std::vector<fxg_baseline> bls_;
   std::vector<std::unique_ptr<GpuMode>> modes_;
   int scanstart_ = 0;
+  int kurtchan_ = 32;
 };
 
 }  // namespace fxgpu

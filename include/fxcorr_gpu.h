@@ -302,7 +302,8 @@ int fxg_station_process_block(fxg_job* job, const fxg_subint_desc* sub,
 #define FXG_STAGE_H2D       6   /* host -> device copies of raw data           */
 #define FXG_STAGE_D2H       7   /* device -> host copy of results              */
 #define FXG_STAGE_PCAL      8   /* phase-cal accumulation and tone DFT         */
-#define FXG_NSTAGES         9
+#define FXG_STAGE_KURTOSIS  9   /* spectral kurtosis accumulation and records  */
+#define FXG_NSTAGES        10
 int fxg_job_set_profiling(fxg_job* job, int on);
 /* ms[FXG_NSTAGES] accumulated milliseconds, launches[FXG_NSTAGES] counts;
  * reset != 0 clears the accumulators after reading. */
@@ -420,6 +421,30 @@ int fxg_batch_graph_captures(const fxg_batch* batch, int64_t* captures);
  * subintegration the job processed.  The products are exact in the scale
  * (the matrix cores apply its inverse), so the scales change rounding only. */
 int fxg_job_calibrate_scales(fxg_job* job);
+
+/* ---- spectral kurtosis (Configuration::dumpKurtosis) -------------------
+ * Mode::setDumpKurtosis / zeroKurtosis (mode.cpp:1496-1523), the per-block
+ * accumulation s1 += |X|^2, s2 += |X|^4 of every recorded band's spectrum
+ * (mode.cpp:1260-1277) in block order, and calculateAndAverageKurtosis
+ * (:1426-1478) at the end of every subintegration:
+ *     sk = (n/(n-1)) (n s2 / s1^2 - 1) - 1,  n = the thread's blocks,
+ * summed over groups of nchan/maxchannels channels when maxchannels <
+ * nchan.  nthreads records per subintegration, one per Core process thread
+ * with the block ranges of Core::updateconfig (core.cpp:2083-2095).
+ * maxchannels is Configuration::getSTADumpChannels (32 by default).
+ * The spectra are read after the fractional-sample rotation (unit modulus,
+ * so the same magnitudes to rounding) at the job's spectrum precision.
+ * enable = 0 turns the accumulation off; the buffers are device memory of
+ * 2 * nthreads * bands * nchan floats. */
+int fxg_job_set_kurtosis(fxg_job* job, int enable, int maxchannels, int nthreads);
+/* The record of (thread, datastream, recorded band) of the last
+ * subintegration processed (Core::averageAndSendKurtosis, core.cpp:1371-
+ * 1420): *nchan = min(maxchannels, nchan) values into out (maxout floats),
+ * *valid = calculateAndAverageKurtosis's return for the datastream (nonzero
+ * data in its last recorded band).  A band whose s1 is all zero reads zeros.
+ * Synchronises with the job's stream. */
+int fxg_job_get_kurtosis(fxg_job* job, int thread, int datastream, int band, float* out, int maxout,
+                         int32_t* nchan, int32_t* valid);
 
 /* Debug taps valid after fxg_subint_process with keep_taps set. */
 int fxg_job_set_keep_taps(fxg_job* job, int keep);

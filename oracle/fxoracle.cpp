@@ -632,13 +632,14 @@ template <typename T>
 static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
                                const fxg_subint_desc& sub, const fxg_station_input& in,
                                int index, const FFT<T>& fft, StationWork<T>& w,
-                               cpx<T>* spectra /* [nrecbands*C] */) {
+                               cpx<T>* spectra /* [nrecbands*C] */, cpx<T>* prefrac = nullptr) {
   const fxg_datastream& ds = j->datastreams[d];
   const int C = g.C, L = g.L, s = g.s;
   Unpacked up{};
   BlockGeom bg = block_geom(g, sub, in, index, &w.unpacked, &up);
   w.up = up;
   std::memset((void*)spectra, 0, sizeof(cpx<T>) * (size_t)ds.nrecbands * C);
+  if (prefrac) std::memset((void*)prefrac, 0, sizeof(cpx<T>) * (size_t)ds.nrecbands * C);
   if (bg.status != 0) return bg;
   const bool fp64 = sizeof(T) == 8;
   w.rot.resize(L); w.fracA.resize(C); w.fracB.resize(C); w.fftbuf.resize(L);
@@ -762,6 +763,7 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
       cpx<T>* out = spectra + (size_t)jb * C;
       for (int c = 0; c < C; c++) {
         cpx<T> v; v.re = w.fftbuf[off + c].real(); v.im = w.fftbuf[off + c].imag();
+        if (prefrac) prefrac[(size_t)jb * C + c] = v;     /* fftoutputs before the frac rotation */
         cpx<T> o; o.re = v.re * fr[c].re - v.im * fr[c].im; o.im = v.re * fr[c].im + v.im * fr[c].re;
         out[c] = o;
       }
@@ -1361,6 +1363,78 @@ int orc_spectra(const fxg_job_desc* j, const fxg_subint_desc* sub, const fxg_sta
   } else {
     FFT<float> fft(g.L); StationWork<float> w;
     station_block<float>(j, g, d, *sub, st[d], block, fft, w, (cpx<float>*)out);
+  }
+  return 0;
+}
+
+/* Spectral kurtosis records of datastream d for one subintegration
+ * (Configuration::dumpKurtosis): per record t of nthreads, the blocks of
+ * Core process thread t (Core::updateconfig, core.cpp:2089-2095), the
+ * accumulation s1 += |X|^2, s2 += |X|^4 of every recorded band's FP32
+ * spectrum before the fractional-sample rotation (mode.cpp:1260-1277; blocks
+ * that Mode::process zeroes return before it, :671-684), then
+ * calculateAndAverageKurtosis (:1426-1478) with maxchannels =
+ * Configuration::getSTADumpChannels.  out[nthreads][nrecbands][nout] with
+ * nout = min(maxchannels, nchan); valid[t] = the value it returns (nonzero
+ * s1 in the last recorded band).  A band whose s1 is all zero, which the
+ * reference skips (leaving the previous call's values), reads zeros. */
+int orc_kurtosis(const fxg_job_desc* j, const fxg_subint_desc* sub, const fxg_station_input* st,
+                 int d, int nthreads, int maxchannels, float* out, int* valid) {
+  Layout L; int r = make_layout(j, &L); if (r) return r;
+  if (d < 0 || d >= j->ndatastreams) return fail(FXG_EINVAL, "datastream");
+  if (nthreads < 1 || maxchannels < 1) return fail(FXG_EINVAL, "nthreads / maxchannels");
+  DsGeom g; r = make_geom(j, L, d, &g); if (r) return r;
+  const int nb = j->datastreams[d].nrecbands, C = g.C, n = j->blockspersend;
+  const int nout = maxchannels < C ? maxchannels : C;
+  FFT<float> fft(g.L); StationWork<float> w;
+  std::vector<cpx<float>> spec((size_t)nb * C), pre((size_t)nb * C);
+  std::vector<float> s1((size_t)nb * C), s2((size_t)nb * C), sk(C);
+  int startblock = 0, numblocks = 0;
+  for (int t = 0; t < nthreads; t++) {
+    startblock += numblocks;                                       /* core.cpp:2091-2095 */
+    numblocks = n / nthreads + ((t < n % nthreads) ? 1 : 0);
+    std::fill(s1.begin(), s1.end(), 0.0f);                         /* zeroKurtosis, mode.cpp:1514-1522 */
+    std::fill(s2.begin(), s2.end(), 0.0f);
+    for (int blk = startblock; blk < startblock + numblocks; blk++) {
+      BlockGeom bg = station_block<float>(j, g, d, *sub, st[d], blk, fft, w, spec.data(), pre.data());
+      if (bg.status != 0) continue;
+      for (int b = 0; b < nb; b++)
+        for (int c = 0; c < C; c++) {
+          const cpx<float>& x = pre[(size_t)b * C + c];
+          float k = std::sqrt(x.re * x.re + x.im * x.im);          /* vectorMagnitude_cf32 */
+          k = k * k;                                               /* vectorSquare_f32_I */
+          s1[(size_t)b * C + c] += k;
+          k = k * k;
+          s2[(size_t)b * C + c] += k;
+        }
+    }
+    bool nonzero = false;
+    for (int b = 0; b < nb; b++) {
+      float* o = out + ((size_t)t * nb + b) * nout;
+      float* a = s1.data() + (size_t)b * C;
+      float* q = s2.data() + (size_t)b * C;
+      nonzero = false;
+      for (int c = 0; c < C; c++) if (a[c] > 0.0f) { nonzero = true; break; }
+      if (!nonzero) { std::fill(o, o + nout, 0.0f); continue; }
+      for (int c = 0; c < C; c++) a[c] *= a[c];
+      for (int c = 0; c < C; c++) q[c] *= (float)numblocks;
+      for (int c = 0; c < C; c++) sk[c] = q[c] / a[c];
+      for (int c = 0; c < C; c++) sk[c] += (float)-1.0;
+      const float corr = (float)((float)numblocks / (numblocks - 1.0));
+      for (int c = 0; c < C; c++) sk[c] *= corr;
+      for (int c = 0; c < C; c++) sk[c] += (float)-1.0;
+      if (maxchannels < C) {
+        const int kchanavg = C / maxchannels;
+        const float inv = (float)(1.0 / ((float)kchanavg));
+        for (int c = 0; c < C; c++) sk[c] *= inv;
+        for (int jj = 0; jj < maxchannels; jj++) {
+          sk[jj] = sk[jj * kchanavg];
+          for (int k = 1; k < kchanavg; k++) sk[jj] += sk[jj * kchanavg + k];
+        }
+      }
+      std::copy(sk.begin(), sk.begin() + nout, o);
+    }
+    valid[t] = nonzero ? 1 : 0;
   }
   return 0;
 }

@@ -75,8 +75,27 @@ int main(int argc, char** argv) {
   }
   const long long rl = core.getCoreResultLength();
   std::vector<cf32> ra(rl), rb(rl, cf32{0, 0});
+  // spectral kurtosis: 32 channels per record, two Core threads
+  const int kthreads = 2, kchan = 32;
+  if (core.setDumpKurtosis(true, kchan, kthreads)) { fprintf(stderr, "setDumpKurtosis: %s\n", core.error().c_str()); return 4; }
   int rc = core.processdata(ra.data());
   if (rc) { fprintf(stderr, "processdata: %d %s\n", rc, core.error().c_str()); return 4; }
+  {
+    // records [thread][datastream][recorded band][channel], then the valid flags [thread][datastream]
+    std::vector<float> krec;
+    std::vector<float> kvalid;
+    for (int t = 0; t < kthreads; t++)
+      for (int d = 0; d < nds; d++) {
+        GpuMode* m = core.getMode(d);
+        kvalid.push_back(m->calculateAndAverageKurtosis(t) ? 1.0f : 0.0f);
+        for (int b = 0; b < dss[d].nrecbands; b++) {
+          const f32* k = m->getKurtosis(b);
+          krec.insert(krec.end(), k, k + m->getKurtosisChannels());
+        }
+      }
+    krec.insert(krec.end(), kvalid.begin(), kvalid.end());
+    spit(dir + "/kurtosis.bin", krec.data(), sizeof(float) * krec.size());
+  }
 
   // (b) Mode::process per block + Core's baseline loop on the host
   std::vector<float> bw(2 * rl, 0.0f);

@@ -46,6 +46,19 @@ def oracle_spectra(orc, job, sd, ds, block, nbands, nchan, fp64=False):
     return out.reshape(nbands, nchan)
 
 
+def oracle_kurtosis(orc, job, sd, ds, nthreads, maxchannels, nbands, nchan):
+    """(sk[nthreads][nbands][nout], valid[nthreads]) of datastream ds."""
+    sub = sd.subint_desc()
+    st, keep = sd.station_inputs()
+    nout = min(maxchannels, nchan)
+    out = np.zeros(nthreads * nbands * nout, dtype=np.float32)
+    valid = np.zeros(nthreads, dtype=np.int32)
+    check(orc, orc.orc_kurtosis(C.byref(job), C.byref(sub), st, ds, nthreads, maxchannels,
+                                out.ctypes.data_as(C.POINTER(C.c_float)),
+                                valid.ctypes.data_as(C.POINTER(C.c_int))), "orc_kurtosis")
+    return out.reshape(nthreads, nbands, nout), valid
+
+
 class GpuJob:
     """A job on the HIP library; results come back to the host."""
 
@@ -74,6 +87,16 @@ class GpuJob:
         out = np.zeros(nchan, dtype=np.complex64)
         check(self.lib, self.lib.fxg_tap_spectrum(self.h, ds, band, block, out.ctypes.data), "tap_spectrum")
         return out
+
+    def kurtosis(self, thread, ds, band, nout):
+        """(sk[nout], valid) of one record of the last subintegration."""
+        out = np.zeros(nout, dtype=np.float32)
+        n, v = C.c_int32(), C.c_int32()
+        check(self.lib, self.lib.fxg_job_get_kurtosis(self.h, thread, ds, band,
+                                                      out.ctypes.data_as(C.POINTER(C.c_float)), nout,
+                                                      C.byref(n), C.byref(v)), "get_kurtosis")
+        assert n.value == nout
+        return out, v.value
 
     def close(self):
         if self.h:

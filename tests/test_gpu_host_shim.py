@@ -17,7 +17,7 @@ import pytest
 import fxcorr
 from fxcorr import Baseline, Datastream, Freq, JobSpec
 from fxsynth import make_subint
-from fxtestlib import oracle_results, rel_err
+from fxtestlib import oracle_kurtosis, oracle_results, rel_err
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(os.path.dirname(HERE), "difx-svn2git-try3_amd", "_build", "test_host_shim")
@@ -70,3 +70,12 @@ def test_core_and_mode_paths_agree_with_oracle(oracle, tmp_path, case):
     assert core.size == ref.size == blocks.size
     assert rel_err(core, ref) <= 1e-5
     assert rel_err(blocks, ref) <= 1e-5
+    # kurtosis records of two Core threads, 32 channels (GpuCore::setDumpKurtosis)
+    k = np.fromfile(os.path.join(tmp_path, "kurtosis.bin"), dtype=np.float32)
+    nb, nout = spec.nrecbands, min(32, spec.nchan)
+    recs = k[:2 * spec.nstations * nb * nout].reshape(2, spec.nstations, nb, nout)
+    valid = k[2 * spec.nstations * nb * nout:].reshape(2, spec.nstations)
+    for d in range(spec.nstations):
+        kref, kvalid = oracle_kurtosis(oracle, job, sd, d, 2, 32, nb, spec.nchan)
+        np.testing.assert_array_equal(valid[:, d], kvalid)
+        np.testing.assert_allclose(recs[:, d], kref, rtol=0, atol=5e-4)
