@@ -191,6 +191,8 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
     REQUIRE(ds.nrecbands > 0 && ds.nrecbands <= FXG_MAX_BANDS, FXG_EINVAL, "nrecbands");
     REQUIRE(power_of_two(ds.nrecbands), FXG_EUNSUPPORTED, "nrecbands must be a power of two");
     REQUIRE(ds.nzoombands >= 0 && ds.nzoombands <= FXG_MAX_ZOOMBANDS, FXG_EINVAL, "nzoombands");
+    REQUIRE(ds.sampling == FXG_SAMPLING_REAL || ds.sampling == FXG_SAMPLING_COMPLEX, FXG_EINVAL, "sampling");
+    REQUIRE(ds.sampling == FXG_SAMPLING_REAL, FXG_EUNSUPPORTED, "complex sampling");
     if (ds.format == FXG_FORMAT_VDIF) {
       REQUIRE(ds.nbits == 2 || ds.nbits == 8, FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
     } else {

@@ -446,60 +446,71 @@ __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStatio
     if (fraclooffset > kTiny) isfraclooffset = true;
     if (looffset < 0) fraclooffset = -fraclooffset;
   }
-  // pre-F rotator points: order 1 x = a k/L + b (mode.cpp:804-812), order 2
-  // x = (k/L)(b + a s/L) + c with quad points (k/L) 2a s/L and steps
-  // (js/L) b + (js/L)^2 a (mode.cpp:821-837)
-  const double stepxoff1 = (double)(1 * s) / (double)L;
-  for (int k = tid; rot && k < s && S.order != 0; k += NT) {
-    const double subxoff = (double)k / (double)L;
-    double ph = (S.order == 1 ? subxoff * G.a + G.b : subxoff * (G.b + G.a * stepxoff1) + G.c) * lofreq;
-    if (S.fractionalLo) ph += (lofreq - (int)lofreq) * (double)G.integerdelay;
-    if (looff) {
-      ph += ((k * S.sampletime) / 1e6) * (-looffset);
-      ph += -looffset * G.fracwall;
-      if (isfraclooffset) ph += -fraclooffset * G.intwall;
-    }
-    float arg = (float)(-kTwoPi * (ph - (int)ph));
-    float sn, cs; sincosf(arg, &sn, &cs);
-    rsub[k] = make_float2(cs, sn);
-    if (S.order == 2) {
-      const double qph = (subxoff * (2 * G.a * stepxoff1)) * lofreq;
-      arg = (float)(-kTwoPi * (qph - (int)qph));
-      sincosf(arg, &sn, &cs);
-      rquad[k] = make_float2(cs, sn);
-    }
-  }
-  for (int j = tid; rot && j < S.nfrsteps && S.order != 0; j += NT) {
-    const double stepxoff = ((double)(j * s)) / (double)L;
-    double ph = (S.order == 1 ? stepxoff * G.a : stepxoff * G.b + (stepxoff * stepxoff) * G.a) * lofreq;
-    if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
-    float arg = (float)(-kTwoPi * (ph - (int)ph));
-    float sn, cs; sincosf(arg, &sn, &cs);
-    rstep[j] = make_float2(cs, sn);
-  }
+  // The entries of the four tables are dealt out over the threads as one
+  // index range (sub, step, frac sub, frac step), so that no wave computes
+  // more than its share of the sin/cos.
+  const int nsub = (rot && S.order != 0) ? s : 0;
+  const int nstep = (rot && S.order != 0) ? S.nfrsteps : 0;
+  const int nfsub = frac ? s : 0;
+  const int nfstep = frac ? S.nfracsteps : 0;
+  const int e1 = nsub, e2 = e1 + nstep, e3 = e2 + nfsub, e4 = e3 + nfstep;
   const double sign = B.useB ? -1.0 : 1.0;
   const float val = (float)((double)G.fse - B.clock + sign * B.clockdelta / 2);
   // order 0: the fringe phase of the block at its centre delay, added to
   // the sub arguments (mode.cpp:1064-1073, 1111-1119)
   float postf = 0.0f;
-  if (S.order == 0) {
+  if (S.order == 0 && nfsub > 0) {
     double phaserotation = (G.c - G.integerdelay) * lofreq;
     if (S.fractionalLo) phaserotation += G.integerdelay * (lofreq - (int)lofreq);
     phaserotation -= G.wall * looffset;
     postf = (float)(-kTwoPi * (phaserotation - (int)phaserotation));
   }
-  for (int k = tid; frac && k < s; k += NT) {
-    float cf = (float)((kTwoPi * k * S.bw) / C);
-    float arg = cf * val;
-    if (S.order == 0) arg = arg + postf;
-    float sn, cs; sincosf(arg, &sn, &cs);
-    fsub[k] = make_float2(cs, sn);
-  }
-  for (int j = tid; frac && j < S.nfracsteps; j += NT) {
-    double dcf = B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
-    float cf = (float)dcf;
-    float sn, cs; sincosf(cf * val, &sn, &cs);
-    fstep[j] = make_float2(cs, sn);
+  // pre-F rotator points: order 1 x = a k/L + b (mode.cpp:804-812), order 2
+  // x = (k/L)(b + a s/L) + c with quad points (k/L) 2a s/L and steps
+  // (js/L) b + (js/L)^2 a (mode.cpp:821-837)
+  const double stepxoff1 = (double)(1 * s) / (double)L;
+  for (int e = tid; e < e4; e += NT) {
+    if (e < e1) {
+      const int k = e;
+      const double subxoff = (double)k / (double)L;
+      double ph = (S.order == 1 ? subxoff * G.a + G.b : subxoff * (G.b + G.a * stepxoff1) + G.c) * lofreq;
+      if (S.fractionalLo) ph += (lofreq - (int)lofreq) * (double)G.integerdelay;
+      if (looff) {
+        ph += ((k * S.sampletime) / 1e6) * (-looffset);
+        ph += -looffset * G.fracwall;
+        if (isfraclooffset) ph += -fraclooffset * G.intwall;
+      }
+      float arg = (float)(-kTwoPi * (ph - (int)ph));
+      float sn, cs; sincosf(arg, &sn, &cs);
+      rsub[k] = make_float2(cs, sn);
+      if (S.order == 2) {
+        const double qph = (subxoff * (2 * G.a * stepxoff1)) * lofreq;
+        arg = (float)(-kTwoPi * (qph - (int)qph));
+        sincosf(arg, &sn, &cs);
+        rquad[k] = make_float2(cs, sn);
+      }
+    } else if (e < e2) {
+      const int j = e - e1;
+      const double stepxoff = ((double)(j * s)) / (double)L;
+      double ph = (S.order == 1 ? stepxoff * G.a : stepxoff * G.b + (stepxoff * stepxoff) * G.a) * lofreq;
+      if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
+      float arg = (float)(-kTwoPi * (ph - (int)ph));
+      float sn, cs; sincosf(arg, &sn, &cs);
+      rstep[j] = make_float2(cs, sn);
+    } else if (e < e3) {
+      const int k = e - e2;
+      float cf = (float)((kTwoPi * k * S.bw) / C);
+      float arg = cf * val;
+      if (S.order == 0) arg = arg + postf;
+      float sn, cs; sincosf(arg, &sn, &cs);
+      fsub[k] = make_float2(cs, sn);
+    } else {
+      const int j = e - e3;
+      double dcf = B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
+      float cf = (float)dcf;
+      float sn, cs; sincosf(cf * val, &sn, &cs);
+      fstep[j] = make_float2(cs, sn);
+    }
   }
 }
 

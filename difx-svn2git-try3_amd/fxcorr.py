@@ -28,6 +28,8 @@ FXG_FORMAT_VDIF = 0
 FXG_FORMAT_MARK5B = 1
 FXG_BLANKER_FORMAT = 0
 FXG_BLANKER_NONE = 1
+FXG_SAMPLING_REAL = 0
+FXG_SAMPLING_COMPLEX = 1
 FXG_XMAC_FP32 = 0
 FXG_XMAC_FP16 = 1
 FXG_XMAC_BF16 = 2
@@ -63,6 +65,7 @@ class Datastream(C.Structure):
         ("zoomband_choffset", C.c_int32 * FXG_MAX_ZOOMBANDS),
         ("pcal_interval_mhz", C.c_double),
         ("pcal_base_mhz", C.c_double),
+        ("sampling", C.c_int32), ("complex_dsb", C.c_int32),
     ]
 
 
@@ -195,6 +198,11 @@ class JobSpec:
     chans_to_avg: int = 1               # channel averaging of every frequency
     pcal_interval_mhz: float = 0.0      # phase-cal tone interval (0 = no extraction)
     pcal_base_mhz: float = 0.0
+    sampling: str = "real"              # "real", "complex" (single sideband) or "complex_dsb"
+
+    @property
+    def is_complex(self) -> bool:
+        return self.sampling != "real"
 
     @property
     def nrecbands(self) -> int:
@@ -202,15 +210,20 @@ class JobSpec:
 
     @property
     def fftchannels(self) -> int:
-        return 2 * self.nchan
+        return self.nchan if self.is_complex else 2 * self.nchan
 
     @property
     def sampletime_us(self) -> float:
-        return 1.0 / (2.0 * self.bw_mhz)
+        return (1.0 if self.is_complex else 0.5) / self.bw_mhz
+
+    @property
+    def unitbits(self) -> int:
+        """Bits of one sample of one band (both components if complex)."""
+        return 2 * self.nbits if self.is_complex else self.nbits
 
     @property
     def framesamples(self) -> int:
-        return (self.framebytes - self.headerbytes) * 8 // (self.nrecbands * self.nbits)
+        return (self.framebytes - self.headerbytes) * 8 // (self.nrecbands * self.unitbits)
 
     def build(self):
         """Return (JobDesc, keepalive) with ctypes arrays owned by keepalive."""
@@ -247,6 +260,8 @@ class JobSpec:
             d.nzoombands = nz * npol
             d.pcal_interval_mhz = self.pcal_interval_mhz
             d.pcal_base_mhz = self.pcal_base_mhz
+            d.sampling = FXG_SAMPLING_COMPLEX if self.is_complex else FXG_SAMPLING_REAL
+            d.complex_dsb = 1 if self.sampling == "complex_dsb" else 0
             recpol = bytearray(FXG_MAX_BANDS)
             zpol = bytearray(FXG_MAX_ZOOMBANDS)
             for sb in range(nsb):

@@ -87,10 +87,16 @@ def quantise8(x: np.ndarray) -> np.ndarray:
 
 
 def pack_codes(codes: np.ndarray, nbits: int) -> np.ndarray:
-    """codes[T, nchan] -> bytes, sample-time major, channel minor, LSB first."""
+    """codes[T, nchan] -> bytes, sample-time major, channel minor, LSB first
+    (nbits 2 or 8; 4 and 16 for complex 2- and 8-bit units)."""
+    if nbits == 16:
+        return np.ascontiguousarray(codes, dtype=np.uint16).reshape(-1).astype("<u2").view(np.uint8).copy()
     flat = np.ascontiguousarray(codes, dtype=np.uint8).reshape(-1)
     if nbits == 8:
         return flat.copy()
+    if nbits == 4:
+        flat = flat.reshape(-1, 2)
+        return (flat[:, 0] | (flat[:, 1] << 4)).astype(np.uint8)
     if nbits == 2:
         flat = flat.reshape(-1, 4)
         return (flat[:, 0] | (flat[:, 1] << 2) | (flat[:, 2] << 4) | (flat[:, 3] << 6)).astype(np.uint8)
@@ -98,12 +104,12 @@ def pack_codes(codes: np.ndarray, nbits: int) -> np.ndarray:
 
 
 def vdif_header(sec: int, frameno: int, framebytes: int, nchan: int, nbits: int,
-                station: int, invalid: bool = False) -> np.ndarray:
+                station: int, invalid: bool = False, complex_data: bool = False) -> np.ndarray:
     w = np.zeros(8, dtype=np.uint32)
     w[0] = (sec & 0x3FFFFFFF) | ((1 << 31) if invalid else 0)
     w[1] = (frameno & 0xFFFFFF) | (20 << 24)
     w[2] = ((framebytes // 8) & 0xFFFFFF) | ((int(math.log2(nchan)) & 0x1F) << 24)
-    w[3] = (station & 0xFFFF) | (((nbits - 1) & 0x1F) << 26)
+    w[3] = (station & 0xFFFF) | (((nbits - 1) & 0x1F) << 26) | ((1 << 31) if complex_data else 0)
     return w.view(np.uint8)
 
 
@@ -129,7 +135,7 @@ def mark5b_header(frameno: int, invalid: bool = False) -> np.ndarray:
 def frame_stream(spec: JobSpec, codes: np.ndarray, first_frame_sec: int, first_frameno: int,
                  frames_per_sec: int, station: int, invalid_frames: Sequence[int] = ()) -> np.ndarray:
     """Pack codes[T, nchan] (T a multiple of framesamples) into frames."""
-    payload = pack_codes(codes, spec.nbits)
+    payload = pack_codes(codes, spec.unitbits)
     databytes = spec.framebytes - spec.headerbytes
     nframes = payload.size // databytes
     payload = payload[: nframes * databytes].reshape(nframes, databytes)
@@ -142,7 +148,8 @@ def frame_stream(spec: JobSpec, codes: np.ndarray, first_frame_sec: int, first_f
         fno = fn % frames_per_sec
         if spec.fmt == FXG_FORMAT_VDIF:
             out[f, : spec.headerbytes] = vdif_header(sec, fno, spec.framebytes, spec.nrecbands,
-                                                     spec.nbits, station, f in inval)[: spec.headerbytes]
+                                                     spec.nbits, station, f in inval,
+                                                     spec.is_complex)[: spec.headerbytes]
         else:
             out[f, :16] = mark5b_header(fno, f in inval)
     return out.reshape(-1)
@@ -161,6 +168,9 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
     delays_us: model delay per station (the data carry the whole-sample part
     of it; any fractional part is then a model-vs-data offset).  rates add a
     linear delay rate to the model only (exercises the fringe rotator).
+    Complex sampling: the common and station signals are complex white
+    noise, each component quantised like a real sample (real part in the
+    low bits of the unit).
     pcal: phase-cal comb added to every band before quantisation, locked to
     the second boundary: {"amp": a, "spacing_hz": s, "offsets_hz": [per
     subband], "ntones": [per subband], "slope_deg": d}; tone k of a band is
@@ -198,10 +208,17 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
     lo = min(dsamp)
     hi = max(dsamp)
     stations = []
-    codes_all = [np.empty((T, spec.nrecbands), dtype=np.uint8) for _ in range(nst)]
+    cplx = spec.is_complex
+    if cplx and pcal:
+        raise ValueError("phase-cal combs are generated for real sampling only")
+    codes_all = [np.empty((T, spec.nrecbands), dtype=np.uint16 if cplx and spec.nbits == 8 else np.uint8)
+                 for _ in range(nst)]
+    quant = quantise2 if spec.nbits == 2 else quantise8
     datasamples = int(datans / (ts * 1e3) + 0.5)
     for sb in range(nsb):
         g = rng.standard_normal(T + hi - lo)
+        if cplx:
+            g = g + 1j * rng.standard_normal(T + hi - lo)
         r = float(rhos[sb])
         comb = 0.0
         if pcal:
@@ -215,8 +232,13 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
         for s in range(nst):
             gs = g[dsamp[s] - lo: dsamp[s] - lo + T]
             for p in range(npol):
-                x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * rng.standard_normal(T) + comb
-                q = quantise2(x) if spec.nbits == 2 else quantise8(x)
+                if cplx:
+                    n = rng.standard_normal(T) + 1j * rng.standard_normal(T)
+                    x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * n
+                    q = quant(x.real).astype(np.uint16) | (quant(x.imag).astype(np.uint16) << spec.nbits)
+                else:
+                    x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * rng.standard_normal(T) + comb
+                    q = quant(x)
                 codes_all[s][:, sb * npol + p] = q
     nflags = (spec.blockspersend + FXG_FLAGS_PER_INT - 1) // FXG_FLAGS_PER_INT
     for s in range(nst):

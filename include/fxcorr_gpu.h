@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FXG_ABI_VERSION 2
+#define FXG_ABI_VERSION 3
 
 /* ---- limits of the fixed-size descriptors ------------------------------ */
 #define FXG_MAX_FREQS        64   /* frequency table entries per job          */
@@ -55,6 +55,10 @@ extern "C" {
 /* Data formats handled by the Mk5Mode path (configuration.cpp:991-1011). */
 #define FXG_FORMAT_VDIF     0   /* VDIF, 32-byte (or 16-byte legacy) header */
 #define FXG_FORMAT_MARK5B   1   /* Mark5B, 16-byte header, 10000-byte payload */
+
+/* Sampling of a datastream (Configuration::datasampling / complextype). */
+#define FXG_SAMPLING_REAL     0
+#define FXG_SAMPLING_COMPLEX  1
 
 /* Blanker selection (mark5access): the VDIF default blanks frames whose
  * payload starts or ends with the fill pattern (blanker_mark5.c:282-312);
@@ -89,7 +93,7 @@ typedef struct {
 /* ---- datastream (station) table entry (configuration.h datastreamdata) -- */
 typedef struct {
   int32_t format;          /* FXG_FORMAT_*                                      */
-  int32_t nbits;           /* 2 or 8 (VDIF); 2 (Mark5B)                         */
+  int32_t nbits;           /* 2 or 8 (VDIF, per component if complex); 2 (Mark5B) */
   int32_t framebytes;      /* header + payload bytes of one frame               */
   int32_t headerbytes;     /* 32 or 16 (VDIF legacy); 16 (Mark5B)               */
   int32_t blanker;         /* FXG_BLANKER_*                                     */
@@ -117,6 +121,13 @@ typedef struct {
    * mode.cpp:774-788; PCal::getNew, pcal.cpp:139-203).  0 = no extraction. */
   double  pcal_interval_mhz;
   double  pcal_base_mhz;
+  /* sampling (datastreamdata sampling / tcomplex, configuration.cpp:1540-1560):
+   * FXG_SAMPLING_REAL, or FXG_SAMPLING_COMPLEX (VDIF complex samples: the
+   * real part in the low nbits of each 2*nbits unit, the imaginary part in
+   * the high nbits); complex_dsb = 1 for double-sideband complex data (the
+   * LO at the band centre, COMPLEX_DSB). */
+  int32_t sampling;
+  int32_t complex_dsb;
 } fxg_datastream;
 
 /* ---- baseline table entry (configuration.h baselinedata) ---------------- */

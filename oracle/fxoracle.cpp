@@ -83,6 +83,8 @@ static long gcdl(long a, long b) { while (b) { long t = a % b; a = b; b = t; } r
 /* ------------------------------------------------------------------------ */
 struct StreamFmt {
   int format, nbits, nchan, framebytes, headerbytes, databytes, blanker;
+  bool cplx;             /* complex samples: 2*nbits per unit, real part low */
+  int unitbits;          /* nbits, or 2*nbits for complex */
   int framesamples;      /* sample-times per frame */
   int granularity;       /* samplegranularity (format_vdif.c:3845; mark5b.c:2837) */
   int groupbytes;        /* bytes consumed per decoder iteration */
@@ -97,7 +99,15 @@ static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
   f->databytes = ds->framebytes - ds->headerbytes; f->blanker = ds->blanker;
   if (f->nchan <= 0 || (f->nchan & (f->nchan - 1)) != 0)
     return fail(FXG_EUNSUPPORTED, "nrecbands must be a power of two");
-  int bitsper = f->nchan * f->nbits;
+  f->cplx = ds->sampling == FXG_SAMPLING_COMPLEX;
+  if (ds->sampling != FXG_SAMPLING_REAL && !f->cplx) return fail(FXG_EINVAL, "sampling");
+  if (f->cplx && ds->format != FXG_FORMAT_VDIF) return fail(FXG_EUNSUPPORTED, "complex sampling is VDIF only");
+  f->unitbits = f->cplx ? 2 * f->nbits : f->nbits;
+  int bitsper = f->nchan * f->unitbits;
+  /* complex units smaller than a byte (1-channel 2-bit) have a sample
+   * granularity of 2, whose erasure in Mk5Mode::unpack (mk5mode.cpp:121-140)
+   * runs over the complex array's floats as if they were real samples */
+  if (f->cplx && bitsper < 8) return fail(FXG_EUNSUPPORTED, "complex sampling needs >= 8 bits per sample-time");
   if (ds->format == FXG_FORMAT_VDIF) {
     if (f->nbits == 2) lut_vdif2(f->lut);
     else if (f->nbits == 8) lut_vdif8(f->lut);
@@ -113,8 +123,11 @@ static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
   f->framesamples = (int)((int64_t)f->databytes * 8 / bitsper);
   f->groupbytes = bitsper >= 8 ? bitsper / 8 : 1;
   f->groupsamples = bitsper >= 8 ? 1 : 8 / bitsper;
-  f->noadvance_quirk = (ds->format == FXG_FORMAT_VDIF && f->nbits == 8 &&
-                        f->nchan >= 2 && f->nchan <= 4);
+  /* decoders that do not advance past a blank group: real 8-bit with 2..4
+   * channels (format_vdif.c:1837-2020), complex 8-bit with any channel count
+   * (format_vdif.c:2829-3054) */
+  f->noadvance_quirk = ds->format == FXG_FORMAT_VDIF && f->nbits == 8 &&
+                       (f->cplx || (f->nchan >= 2 && f->nchan <= 4));
   return 0;
 }
 
@@ -189,15 +202,17 @@ static void load_frame(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
   }
 }
 
-/* mark5_unpack_with_offset + the format decoder loop.  out[c*nsamp + o].
+/* mark5_unpack_with_offset (mark5_unpack_complex_with_offset) + the format
+ * decoder loop.  out[c*nsamp + o] (complex: float pairs at 2*(c*nsamp + o)).
  * Returns goodsamples (the decoder's return value). */
 static int decode(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
                   int64_t offsetsamples, int nsamp, float* out) {
   int64_t frameidx = offsetsamples / f.framesamples;
   FrameState st; load_frame(f, data, nbytes, frameidx, &st);
-  int i = (int)((offsetsamples % f.framesamples) * f.nchan * f.nbits / 8);
+  int i = (int)((offsetsamples % f.framesamples) * f.nchan * f.unitbits / 8);
   int nblank = 0;
   const uint32_t mask = (1u << f.nbits) - 1u;
+  const int nc = f.cplx ? 2 : 1;
   for (int o = 0; o < nsamp; o += f.groupsamples) {
     bool blank = (i < st.startvalid) || (i >= st.endvalid) || (st.frame == nullptr);
     if (!blank && f.format == FXG_FORMAT_MARK5B && (st.frame[f.headerbytes - 11] & 0x80)) blank = true;
@@ -205,16 +220,20 @@ static int decode(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
       nblank++;
       for (int k = 0; k < f.groupsamples; k++)
         for (int c = 0; c < f.nchan; c++)
-          if (o + k < nsamp) out[(size_t)c * nsamp + o + k] = 0.0f;
+          for (int q = 0; q < nc; q++)
+            if (o + k < nsamp) out[((size_t)c * nsamp + o + k) * nc + q] = 0.0f;
       if (!f.noadvance_quirk) i += f.groupbytes;
     } else {
       const uint8_t* g = st.frame + f.headerbytes + i;
       for (int k = 0; k < f.groupsamples; k++)
-        for (int c = 0; c < f.nchan; c++) {
-          int bit = (k * f.nchan + c) * f.nbits;
-          uint32_t code = (g[bit >> 3] >> (bit & 7)) & mask;
-          if (o + k < nsamp) out[(size_t)c * nsamp + o + k] = f.lut[code];
-        }
+        for (int c = 0; c < f.nchan; c++)
+          for (int q = 0; q < nc; q++) {
+            /* component q of the unit: real in the low nbits, imaginary above
+             * (complex_lut2bit / lut8bit pairs, format_vdif.c:133-140, 2829-2850) */
+            int bit = (k * f.nchan + c) * f.unitbits + q * f.nbits;
+            uint32_t code = (g[bit >> 3] >> (bit & 7)) & mask;
+            if (o + k < nsamp) out[((size_t)c * nsamp + o + k) * nc + q] = f.lut[code];
+          }
       i += f.groupbytes;
     }
     if (i >= f.databytes) { frameidx++; load_frame(f, data, nbytes, frameidx, &st); i = 0; }
@@ -229,7 +248,7 @@ struct Unpacked { int stride; int munged; float weight; };
 static Unpacked mk5_unpack(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
                            int64_t sampleoffset, int unpacksamples, std::vector<float>& work) {
   int samplestounpack = unpacksamples + (f.granularity > 1 ? f.granularity : 0);
-  work.assign((size_t)f.nchan * samplestounpack, 0.0f);
+  work.assign((size_t)f.nchan * samplestounpack * (f.cplx ? 2 : 1), 0.0f);
   int64_t unpackstart = sampleoffset - (sampleoffset % f.granularity);
   float goodsamples = (float)decode(f, data, nbytes, unpackstart, samplestounpack, work.data());
   int munged = 0;
@@ -306,6 +325,7 @@ template <typename T> static inline cpx<T> cmul(cpx<T> a, cpx<T> v) {
 /* ------------------------------------------------------------------------ */
 struct DsGeom {
   StreamFmt fmt;
+  bool cplx, dsb;                 /* complex sampling; double sideband */
   int C, L, s, numfrstrides, nfracsteps;
   double bw, sampletime;          /* MHz, µs */
   int order;                      /* fringe rotation order 0, 1, 2 */
@@ -515,16 +535,24 @@ static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g)
   const fxg_datastream& ds = j->datastreams[d];
   int r = make_fmt(&ds, &g->fmt); if (r) return r;
   const fxg_freq& f0 = j->freqs[ds.recfreq_index[0]];
-  g->C = f0.nchan; g->L = 2 * g->C; g->bw = f0.bw_mhz;
+  /* complex sampling: fftchannels = nchan, sample time 1/bw (mode.cpp:82-87) */
+  g->cplx = g->fmt.cplx; g->dsb = g->cplx && ds.complex_dsb != 0;
+  g->C = f0.nchan; g->L = g->cplx ? g->C : 2 * g->C; g->bw = f0.bw_mhz;
   g->s = lay.arraystridelen[d];
   if (g->L % g->s) return fail(FXG_EINVAL, "array stride must divide fftchannels");
   if (g->C % g->s) return fail(FXG_EINVAL, "array stride must divide nchan");
-  g->numfrstrides = g->L / g->s; g->nfracsteps = g->numfrstrides / 2;
+  g->numfrstrides = g->L / g->s; g->nfracsteps = g->C / g->s;
   g->sampletime = 1.0 / (2.0 * g->bw);
+  if (g->cplx) g->sampletime *= 2.0;
   g->order = j->fringe_order;
+  if (g->cplx && g->order == 0)
+    return fail(FXG_EUNSUPPORTED, "post-F fringe rotation is not supported for complex data (mode.cpp:1155-1158)");
+  if (g->cplx && ds.pcal_interval_mhz > 0) return fail(FXG_EUNSUPPORTED, "phase cal of complex data");
   g->fractionalLoFreq = false; g->deltapoloffsets = false;
   for (int i = 0; i < ds.nrecfreqs; i++) {
-    double loval = j->freqs[ds.recfreq_index[i]].freq_mhz;
+    const fxg_freq& fi = j->freqs[ds.recfreq_index[i]];
+    double loval = fi.freq_mhz;
+    if (g->dsb) loval += fi.lsb ? -fi.bw_mhz / 2.0 : fi.bw_mhz / 2.0;     /* mode.cpp:95-105 */
     if (std::fabs(loval - (int)(loval + 0.5)) > TINY) g->fractionalLoFreq = true;
     if (ds.recfreq_clockoffsetdelta_us[i] != 0.0) g->deltapoloffsets = true;
     if (ds.recfreq_phaseoffset_deg[i] != 0.0) return fail(FXG_EUNSUPPORTED, "pol phase offsets");
@@ -532,7 +560,7 @@ static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g)
   if ((g->L & (g->L - 1)) != 0) return fail(FXG_EUNSUPPORTED, "fftchannels must be a power of two");
   /* Mk5Mode: blockclock = 2*bw, so samplesperblock = 1 (mode.cpp:129-141) */
   g->samplesperblock = 1;
-  int numsamplebits = ds.nbits;
+  int numsamplebits = g->fmt.unitbits;
   g->bytesperblocknum = (ds.nrecbands * g->samplesperblock * numsamplebits) / 8;
   if (g->bytesperblocknum == 0) {
     g->bytesperblocknum = 1;
@@ -662,7 +690,11 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
 
   for (int i = 0; i < ds.nrecfreqs; i++) {
     const fxg_freq& fq = j->freqs[ds.recfreq_index[i]];
-    const double lofreq = fq.freq_mhz;
+    /* the LO of the fringe rotator (mode.cpp:886-908): the band edge; for
+     * double-sideband complex data the band centre; negative for complex LSB */
+    double lofreq = fq.freq_mhz;
+    if (g.dsb) lofreq += fq.lsb ? -fq.bw_mhz / 2.0 : fq.bw_mhz / 2.0;
+    if (g.cplx && fq.lsb) lofreq = -lofreq;
     const double looffset = ds.recfreq_looffset_hz[i];
     bool looff = false, isfraclooffset = false; double fraclooffset = 0.0;
     if (looffset > 0.0 || looffset < 0.0) {                                               /* mode.cpp:888-896 */
@@ -738,7 +770,9 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
         T sn, cs; sincos_t(arg, &sn, &cs); fr[k].re = cs; fr[k].im = sn;
       }
       for (int jj = 0; jj < g.nfracsteps; jj++) {
-        double dcf = fq.lsb ? (-TWO_PI * ((g.nfracsteps - jj) * s) * g.bw) / C
+        /* step channel frequencies (mode.cpp:328-334, 864-880): DSB, LSB, USB */
+        double dcf = g.dsb ? (TWO_PI * jj * s * g.bw) / C - TWO_PI * g.bw / 2.0
+                   : fq.lsb ? (-TWO_PI * ((g.nfracsteps - jj) * s) * g.bw) / C
                             : (TWO_PI * jj * s * g.bw) / C;
         T cf = fp64 ? (T)dcf : (T)(float)dcf;
         T arg = cf * val; T sn, cs; sincos_t(arg, &sn, &cs); stepf[jj].re = cs; stepf[jj].im = sn;
@@ -752,17 +786,39 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
     /* bands of this frequency (mode.cpp:1148-1310) */
     for (int jb = 0; jb < ds.nrecbands; jb++) {
       if (ds.recband_freq[jb] != i) continue;
-      const float* x = w.unpacked.data() + (size_t)jb * up.stride + up.munged;
-      for (int n = 0; n < L; n++) {
-        T xv = (T)x[n];
-        w.fftbuf[n] = std::complex<T>(xv * w.rot[n].re, xv * w.rot[n].im);
+      if (g.cplx) {
+        /* rotator times the complex samples (vectorMul_cf32, mode.cpp:1184-1189) */
+        const float* x = w.unpacked.data() + 2 * ((size_t)jb * up.stride + up.munged);
+        for (int n = 0; n < L; n++) {
+          T xr = (T)x[2 * n], xi = (T)x[2 * n + 1];
+          w.fftbuf[n] = std::complex<T>(w.rot[n].re * xr - w.rot[n].im * xi, w.rot[n].re * xi + w.rot[n].im * xr);
+        }
+      } else {
+        const float* x = w.unpacked.data() + (size_t)jb * up.stride + up.munged;
+        for (int n = 0; n < L; n++) {
+          T xv = (T)x[n];
+          w.fftbuf[n] = std::complex<T>(xv * w.rot[n].re, xv * w.rot[n].im);
+        }
       }
       fft.run(w.fftbuf.data());
       const int off = fq.lsb ? C : 0;
       const cpx<T>* fr = (!g.deltapoloffsets || ds.recband_pol[jb] == 'R') ? w.fracA.data() : w.fracB.data();
       cpx<T>* out = spectra + (size_t)jb * C;
       for (int c = 0; c < C; c++) {
-        cpx<T> v; v.re = w.fftbuf[off + c].real(); v.im = w.fftbuf[off + c].imag();
+        cpx<T> v;
+        if (g.cplx) {
+          /* channel c of the band (mode.cpp:1208-1241): USB as transformed,
+           * DSB with the halves swapped; LSB conjugated in reversed order
+           * (vectorConjFlip_cf32), DC kept in channel 0 without conjugation
+           * for single sideband */
+          int m = c; bool cj = false;
+          if (!fq.lsb) m = g.dsb ? (c + C / 2) % C : c;
+          else if (g.dsb) { m = ((C / 2 - c) % C + C) % C; cj = true; }
+          else if (c > 0) { m = C - c; cj = true; }
+          v.re = w.fftbuf[m].real(); v.im = cj ? -w.fftbuf[m].imag() : w.fftbuf[m].imag();
+        } else {
+          v.re = w.fftbuf[off + c].real(); v.im = w.fftbuf[off + c].imag();
+        }
         if (prefrac) prefrac[(size_t)jb * C + c] = v;     /* fftoutputs before the frac rotation */
         cpx<T> o; o.re = v.re * fr[c].re - v.im * fr[c].im; o.im = v.re * fr[c].im + v.im * fr[c].re;
         out[c] = o;
