@@ -345,6 +345,7 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
   int rc = build_plan(&jd, &p, &g_err);
   if (rc) return rc;
   DevStation S = p.st[0];
+  if (S.cplx) return set_err(FXG_EUNSUPPORTED, "fxg_unpack of complex samples (fxg_tap_spectrum gives their spectra)");
   HIPCHK(hipSetDevice(c->device));
   DBuf d_st, d_ss, d_ftab;
   if ((rc = d_st.ensure(sizeof(DevStation))) || (rc = d_ss.ensure(sizeof(DevStationSub)))) return rc;
@@ -425,6 +426,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     std::string why;
     if (!P.uniform) why = "all datastreams of a job must share nchan";
     else if (!(P.L >= 128 && P.L <= 65536 && (P.L & (P.L - 1)) == 0)) why = "FFT length must be a power of two in [128, 65536]";
+    else if (P.st[0].cplx && P.L > kFStageMaxL) why = "complex sampling: at most 16384 channels";
     for (const DevStation& st : P.st) {
       const int tabs = fxg_rotator_table_entries(st.L);
       const int rotpts = (st.order == 2 ? 2 : 1) * st.s + st.L / st.s, fracpts = st.s + st.C / st.s;
@@ -751,7 +753,7 @@ static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stati
     maxframes = std::max(maxframes, q.nframes);
     {
       const DevStation& S = P.st[d];
-      const int W = S.nbits == 2 && S.nchan % 16 == 0 ? S.nchan / 16 : 0;
+      const int W = S.nbits == 2 && !S.cplx && S.nchan % 16 == 0 ? S.nchan / 16 : 0;
       const bool ok = (W == 1 || W == 2 || W == 4) &&
                       (((uintptr_t)q.data | (uintptr_t)S.framebytes | (uintptr_t)S.headerbytes |
                         (uintptr_t)S.databytes) % (4 * W)) == 0;
@@ -873,11 +875,11 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
     if ((rc = j->ev_begin(FXG_STAGE_FSTAGE))) return rc;
     if (j->nb2) {
       fa.bandlist = j->d_bandlist.as<int>();
-      HIPCHK(launch_fstage(P.L, 2, fa, j->nb2, cn, s));
+      HIPCHK(launch_fstage(P.L, 2, P.st[0].cplx, fa, j->nb2, cn, s));
     }
     if (j->nb8) {
       fa.bandlist = j->d_bandlist.as<int>() + j->nb2;
-      HIPCHK(launch_fstage(P.L, 8, fa, j->nb8, cn, s));
+      HIPCHK(launch_fstage(P.L, 8, P.st[0].cplx, fa, j->nb8, cn, s));
     }
     if ((rc = j->ev_end())) return rc;
     if (j->kurt_on) {
@@ -1335,7 +1337,7 @@ int fxg_station_process_block(fxg_job* j, const fxg_subint_desc* sub, const fxg_
   fill_fstage_args(j, &fa);
   fa.chunk0 = block;
   fa.bandlist = j->d_iota.as<int>() + P.st[ds].band0;
-  HIPCHK(launch_fstage(P.L, P.st[ds].nbits, fa, P.st[ds].nchan, 1, j->stream));
+  HIPCHK(launch_fstage(P.L, P.st[ds].nbits, P.st[ds].cplx, fa, P.st[ds].nchan, 1, j->stream));
   HIPCHK(hipStreamSynchronize(j->stream));
   j->ev_collect();
   j->have_subint = 1;

@@ -24,6 +24,8 @@ struct DevStation {
   int slog;             // log2(s) when s is a power of two, else -1
   int bpf;              // bytes per band per frame in the corner-turned streams
   int order;            // fringe rotation order 0 (post-F), 1 (linear), 2 (quadratic)
+  int cplx;             // complex samples (real part in the low nbits of each unit)
+  int unitbits;         // bits of one band's sample: nbits, or 2 nbits if complex
   double sampletime, bw;
   float lut[4];         // 2-bit level table (VDIF or Mark5B order)
 };
@@ -32,8 +34,8 @@ struct DevStation {
 struct DevBand {
   int station, band, localfreq, lsb;
   int useB;             // apply the "B" frac-sample rotator (delta pol offsets)
-  int pad;
-  double lofreq, clock, clockdelta, looffset;
+  int dsb;              // complex double sideband: halves swapped, frac steps offset by -BW/2
+  double lofreq, clock, clockdelta, looffset;   // lofreq: the fringe rotator's LO
   long long specoff;    // element offset of this band in the spectra buffer
 };
 

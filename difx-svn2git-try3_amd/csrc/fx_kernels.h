@@ -108,7 +108,8 @@ hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsi
                              hipStream_t stream);
 hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
                            const int2* ftab, int nblocks, DevBlock* out, int nst, hipStream_t stream);
-hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
+// cplx: complex samples (L = nchan, every output of the transform kept)
+hipError_t launch_fstage(int L, int nbits, int cplx, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
 hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const int2* ftab, const float* lut8,
                          long long first, int nsamp, int nchan, long long validend, float* out,
                          hipStream_t stream);

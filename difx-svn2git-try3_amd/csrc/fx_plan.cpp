@@ -192,7 +192,18 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
     REQUIRE(power_of_two(ds.nrecbands), FXG_EUNSUPPORTED, "nrecbands must be a power of two");
     REQUIRE(ds.nzoombands >= 0 && ds.nzoombands <= FXG_MAX_ZOOMBANDS, FXG_EINVAL, "nzoombands");
     REQUIRE(ds.sampling == FXG_SAMPLING_REAL || ds.sampling == FXG_SAMPLING_COMPLEX, FXG_EINVAL, "sampling");
-    REQUIRE(ds.sampling == FXG_SAMPLING_REAL, FXG_EUNSUPPORTED, "complex sampling");
+    REQUIRE(ds.sampling == d->datastreams[0].sampling, FXG_EUNSUPPORTED,
+            "all datastreams of a job must share real or complex sampling (one FFT length per job)");
+    if (ds.sampling == FXG_SAMPLING_COMPLEX) {
+      REQUIRE(ds.format == FXG_FORMAT_VDIF, FXG_EUNSUPPORTED, "complex sampling is VDIF only");
+      // a unit below a byte (1-band 2-bit) has a sample granularity of 2, whose
+      // erasure in Mk5Mode::unpack (mk5mode.cpp:121-140) runs over the complex
+      // array's floats as real samples
+      REQUIRE(ds.nrecbands * 2 * ds.nbits >= 8, FXG_EUNSUPPORTED, "complex sampling needs >= 8 bits per sample-time");
+      REQUIRE(d->fringe_order != 0, FXG_EUNSUPPORTED,
+              "post-F fringe rotation is not supported for complex data (mode.cpp:1155-1158)");
+      REQUIRE(!(ds.pcal_interval_mhz > 0), FXG_EUNSUPPORTED, "phase cal of complex data");
+    }
     if (ds.format == FXG_FORMAT_VDIF) {
       REQUIRE(ds.nbits == 2 || ds.nbits == 8, FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
     } else {
@@ -346,7 +357,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
 
   // ---- per-datastream device constants; one FFT length per job
   P.C = d->freqs[d->datastreams[0].recfreq_index[0]].nchan;
-  P.L = 2 * P.C;
+  P.L = d->datastreams[0].sampling == FXG_SAMPLING_COMPLEX ? P.C : 2 * P.C;
   int band = 0;
   for (int s = 0; s < d->ndatastreams; s++) {
     const fxg_datastream& ds = d->datastreams[s];
@@ -361,13 +372,19 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     st.format = ds.format; st.nbits = ds.nbits; st.nchan = ds.nrecbands;
     st.framebytes = ds.framebytes; st.headerbytes = ds.headerbytes;
     st.databytes = ds.framebytes - ds.headerbytes; st.blanker = ds.blanker;
-    int bitsper = ds.nrecbands * ds.nbits;
+    st.cplx = ds.sampling == FXG_SAMPLING_COMPLEX ? 1 : 0;
+    st.unitbits = st.cplx ? 2 * ds.nbits : ds.nbits;
+    int bitsper = ds.nrecbands * st.unitbits;
     st.framesamples = (int)((long long)st.databytes * 8 / bitsper);
     st.granularity = bitsper >= 8 ? 1 : 8 / bitsper;
     st.groupbytes = bitsper >= 8 ? bitsper / 8 : 1;
     st.groupsamples = bitsper >= 8 ? 1 : 8 / bitsper;
-    st.quirk8 = (ds.format == FXG_FORMAT_VDIF && ds.nbits == 8 && ds.nrecbands >= 2 && ds.nrecbands <= 4);
-    st.C = f0.nchan; st.L = 2 * f0.nchan; st.s = P.arraystride[s];
+    // decoders that stop at a blank group: real 8-bit with 2..4 bands
+    // (format_vdif.c:1837-2020), complex 8-bit with any (:2829-3054)
+    st.quirk8 = ds.format == FXG_FORMAT_VDIF && ds.nbits == 8 &&
+                (st.cplx || (ds.nrecbands >= 2 && ds.nrecbands <= 4));
+    // complex: fftchannels = nchan, sample time 1/bw (mode.cpp:82-87)
+    st.C = f0.nchan; st.L = st.cplx ? f0.nchan : 2 * f0.nchan; st.s = P.arraystride[s];
     REQUIRE(st.C % st.s == 0, FXG_EINVAL, "array stride must divide nchan");
     st.nfrsteps = st.L / st.s; st.nfracsteps = st.C / st.s;
     st.slog = -1;
@@ -380,9 +397,13 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     if (st.bpbnum == 0) { st.bpbnum = 1; st.bpbden = 8 / bitsper; }
     st.bw = f0.bw_mhz;
     st.sampletime = 1.0 / (2.0 * st.bw);
+    if (st.cplx) st.sampletime *= 2.0;
+    const bool dsb = st.cplx && ds.complex_dsb != 0;
     st.fractionalLo = 0; st.deltapol = 0;
     for (int i = 0; i < ds.nrecfreqs; i++) {
-      double lo = d->freqs[ds.recfreq_index[i]].freq_mhz;
+      const fxg_freq& fi = d->freqs[ds.recfreq_index[i]];
+      double lo = fi.freq_mhz;
+      if (dsb) lo += fi.lsb ? -fi.bw_mhz / 2.0 : fi.bw_mhz / 2.0;      // mode.cpp:95-105
       if (std::fabs(lo - (int)(lo + 0.5)) > kTiny) st.fractionalLo = 1;
       if (ds.recfreq_clockoffsetdelta_us[i] != 0.0) st.deltapol = 1;
     }
@@ -396,7 +417,12 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       const fxg_freq& fq = d->freqs[ds.recfreq_index[lf]];
       b.station = s; b.band = k; b.localfreq = lf; b.lsb = fq.lsb ? 1 : 0;
       b.useB = (st.deltapol && ds.recband_pol[k] != 'R') ? 1 : 0;
+      b.dsb = dsb ? 1 : 0;
+      // the fringe rotator's LO (mode.cpp:886-908): the band edge; the band
+      // centre for double-sideband complex data; negative for complex LSB
       b.lofreq = fq.freq_mhz;
+      if (dsb) b.lofreq += fq.lsb ? -fq.bw_mhz / 2.0 : fq.bw_mhz / 2.0;
+      if (st.cplx && fq.lsb) b.lofreq = -b.lofreq;
       b.clock = ds.recfreq_clockoffset_us[lf];
       b.clockdelta = ds.recfreq_clockoffsetdelta_us[lf];
       b.looffset = ds.recfreq_looffset_hz[lf];

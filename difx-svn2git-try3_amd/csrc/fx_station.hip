@@ -338,8 +338,9 @@ __device__ __forceinline__ int pidx(int a) { return a + (a >> 4); }
 // ---------------------------------------------------------------------------
 // k_cornerturn: one workgroup per (frame, datastream).  De-interleaves the
 // frame payload (sample-time major, channel minor) into one contiguous
-// sample stream per band, nbits per sample, LSB first, so that the F-stage
-// of one band reads L*nbits/8 contiguous bytes per FFT block.
+// sample stream per band, unitbits per sample (nbits, or 2 nbits for complex
+// samples), LSB first, so that the F-stage of one band reads L*unitbits/8
+// contiguous bytes per FFT block.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cornerturn(const DevStation* __restrict__ st,
                                                     const DevStationSub* __restrict__ ss,
@@ -359,7 +360,17 @@ __global__ __launch_bounds__(256) void k_cornerturn(const DevStation* __restrict
     for (int i = threadIdx.x; i < S.databytes; i += blockDim.x) pay[i] = src[i];
   }
   __syncthreads();
-  const int nbits = S.nbits, nchan = S.nchan, bpf = S.bpf;
+  const int nbits = S.unitbits, nchan = S.nchan, bpf = S.bpf;
+  if (nbits >= 8) {
+    // units of whole bytes (complex 8-bit: two bytes per unit)
+    const int bpu = nbits / 8;
+    for (int ob = threadIdx.x; ob < nchan * bpf; ob += blockDim.x) {
+      const int c = ob / bpf, i = ob - c * bpf;
+      const int u = i / bpu;
+      ct[Q.ctbase + (long long)c * Q.ctstride + f * bpf + i] = pay[(u * nchan + c) * bpu + (i - u * bpu)];
+    }
+    return;
+  }
   const int per = 8 / nbits;                   // samples per output byte
   const unsigned mask = (1u << nbits) - 1u;
   for (int ob = threadIdx.x; ob < nchan * bpf; ob += blockDim.x) {
@@ -435,7 +446,7 @@ __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStatio
                                                int tid, int NT, bool rot, bool frac, float2* rsub,
                                                float2* rstep, float2* fsub, float2* fstep, float2* rquad) {
 #pragma clang fp contract(off)
-  const int C = L / 2;
+  const int C = S.C;                 // recorded channels: L/2 (real) or L (complex)
   const int s = S.s;
   const double lofreq = B.lofreq, looffset = B.looffset;
   const bool looff = looffset > 0.0 || looffset < 0.0;
@@ -506,7 +517,9 @@ __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStatio
       fsub[k] = make_float2(cs, sn);
     } else {
       const int j = e - e3;
-      double dcf = B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
+      // step channel frequencies (mode.cpp:328-334, 864-880): DSB, LSB, USB
+      double dcf = B.dsb ? (kTwoPi * j * s * S.bw) / C - kTwoPi * S.bw / 2.0
+                 : B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
       float cf = (float)dcf;
       float sn, cs; sincosf(cf * val, &sn, &cs);
       fstep[j] = make_float2(cs, sn);
@@ -825,6 +838,209 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// k_fstage_c<L, NBITS>: the F stage of complex-sampled data (Mode with
+// usecomplex, mode.cpp:1184-1241): L = nchan complex samples per block, each
+// a unit of 2 NBITS bits (real part low), times the fringe rotator, an
+// L-point transform with every output kept, channel c of the band taken from
+// bin m (USB: m = c; DSB: the halves swapped, m = c + L/2 mod L; LSB:
+// conjugated in reverse order, m = L - c with DC kept unconjugated; LSB DSB:
+// m = L/2 - c mod L, conjugated), then the fractional-sample rotation.
+// ---------------------------------------------------------------------------
+template <int L, int NBITS>
+__global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
+  using Sh = FftShape<L>;
+  constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
+  constexpr int UB = 2 * NBITS;                 // bits per complex sample
+  constexpr int RAWW = (L * UB) / 32 + 2;
+  __shared__ float2 buf[Sh::PAD];
+  __shared__ float2 tabs[Sh::TABS];
+  __shared__ float2 twlo[64], twhi[L / 64];
+  constexpr bool kAliasRaw = NBITS == 8 && L >= 16384;
+  __shared__ unsigned raw_own[kAliasRaw ? 1 : RAWW];
+  unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : raw_own;
+  __shared__ float lut8s[NBITS == 8 ? 256 : 4];
+
+  const int tid = threadIdx.x;
+  const int cb = blockIdx.x;
+  const int blk = A.chunk0 + cb;
+  const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
+  const DevBand B = A.bands[gband];
+  const DevStation S = A.st[B.station];
+  const DevStationSub Q = A.ss[B.station];
+  const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
+  constexpr int C = L;
+  const long long outbase = B.specoff + (long long)cb * C;
+
+  if (G.status != 0) {
+    zero_spec(A.spectra, A.prec, outbase, C, tid, NT);
+    return;
+  }
+  const long long t0 = G.nearestsample;
+  const bool fast = G.allvalid != 0;
+  const long long bit0 = t0 * UB;
+  if (fast) {
+    const long long w0 = bit0 >> 5;
+    const unsigned* src = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
+    const long long nwords = Q.ctstride >> 2;
+    for (int i = tid; i < RAWW; i += NT) raw[i] = (w0 + i < nwords) ? src[w0 + i] : 0u;
+  }
+  float2* rsub = tabs;
+  float2* rstep = tabs + S.s;
+  float2* fsub = rstep + S.nfrsteps;
+  float2* fstep = fsub + S.s;
+  float2* rquad = fstep + S.nfracsteps;
+  for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
+  for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
+  rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
+  if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
+  else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
+  __syncthreads();
+
+  const int s = S.s, slog = S.slog;
+  const int bitoff = (int)(bit0 & 31);
+  constexpr bool kStride16 = NTF % 16 == 0;
+  constexpr int RSTR = NTF + NTF / 16;
+  constexpr int WSTEP = (UB * NTF) / 32;        // UB * NTF is a multiple of 32 for L >= 128
+  const bool rotfast = slog >= 0 && (NTF & (s - 1)) == 0;
+
+  // ---- pass 1: decode, rotator times sample (vectorMul_cf32), radix 16
+  float2 v[16];
+  if (tid < NTF) {
+    const int j = tid;
+    if (fast && rotfast) {
+      const int bp = bitoff + UB * j;
+      const unsigned* wp = raw + (bp >> 5);
+      const int sh = bp & 31;                   // a unit never straddles a word
+      const int k = j & (s - 1), q0 = j >> slog, sstep = NTF >> slog;
+      float2 piece = rsub[k];
+      float2 qd = make_float2(1.0f, 0.0f);
+      if (S.order == 2) {
+        qd = rquad[k];
+        for (int i = 0; i < q0; i++) piece = cmulf(piece, qd);
+      }
+      const float2* stp = rstep + q0;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << UB) - 1u);
+        const float2 x = make_float2(lut8s[code & ((1u << NBITS) - 1u)], lut8s[code >> NBITS]);
+        const float2 rot = cmulf(piece, *stp);  // sub[k] step[j] (order 2: piece_j[k] step[j])
+        stp += sstep;
+        v[r] = cmulf(rot, x);
+        if (S.order == 2 && r < 15)
+          for (int i = 0; i < sstep; i++) piece = cmulf(piece, qd);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int n = j + r * NTF;
+        float2 x;
+        if (fast) {
+          const int bp = bitoff + n * UB;
+          const unsigned code = (raw[bp >> 5] >> (bp & 31)) & ((1u << UB) - 1u);
+          x = make_float2(lut8s[code & ((1u << NBITS) - 1u)], lut8s[code >> NBITS]);
+        } else {
+          x = sample_value_c(S, Q, A.ftab, lut8s, t0 + n, G.validend, B.band);
+        }
+        const int kk = slog >= 0 ? (n & (s - 1)) : n % s;
+        const int jj = slog >= 0 ? (n >> slog) : n / s;
+        float2 rot;
+        if (S.order == 1) {
+          rot = n < s ? rsub[n] : cmulf(rsub[kk], rstep[jj]);
+        } else {
+          const float2 qd = rquad[kk];
+          float2 piece = rsub[kk];
+          for (int i = 0; i < jj; i++) piece = cmulf(piece, qd);
+          rot = cmulf(piece, rstep[jj]);
+        }
+        v[r] = cmulf(rot, x);
+      }
+    }
+    DFT<16>::run(v);
+  }
+  if (kAliasRaw) __syncthreads();
+  if (tid < NTF) {
+    float2* o = buf + 17 * tid;
+#pragma unroll
+    for (int r = 0; r < 16; r++) o[r] = v[r];
+  }
+  __syncthreads();
+
+  // ---- middle radix-16 passes
+#pragma unroll
+  for (int p = 0; p < Sh::NMID; p++) {
+    const int Ns = 16 << (4 * p);
+    float2 v[16];
+    const int j = tid;
+    if (tid < NTF) {
+      const int k = j & (Ns - 1);
+      float2 w[16];
+      twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
+      const float2* ip = buf + pidx(j);
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float2 x = kStride16 ? ip[r * RSTR] : buf[pidx(j + r * NTF)];
+        v[r] = r == 0 ? x : cmulf(x, w[r]);
+      }
+      DFT<16>::run(v);
+    }
+    __syncthreads();
+    if (tid < NTF) {
+      const int k = j & (Ns - 1);
+      const int idxD = (j / Ns) * Ns * 16 + k;
+      float2* op = buf + pidx(idxD);
+#pragma unroll
+      for (int r = 0; r < 16; r++) op[r * (Ns + Ns / 16)] = v[r];
+    }
+    __syncthreads();
+  }
+
+  // ---- last pass (radix RL): every output, to its channel
+  if (tid < NTF) {
+    const int j = tid;
+#pragma unroll
+    for (int u = 0; u < 16 / RL; u++) {
+      const int k = j + u * NTF;
+      float2 v[RL], w[RL];
+      if (RL == 16) twiddles16(twlo, twhi, k, w);
+      else twiddlesR<RL>(twlo, twhi, k, w);
+#pragma unroll
+      for (int r = 0; r < RL; r++) {
+        const float2 x = buf[pidx(k + NSL * r)];
+        v[r] = r == 0 ? x : cmulf(x, w[r]);
+      }
+      DFT<RL>::run(v);
+#pragma unroll
+      for (int q = 0; q < RL; q++) {
+        const int m = k + NSL * q;               // transform bin
+        int c = m;
+        bool cj = false;
+        if (!B.lsb) {
+          if (B.dsb) c = (m + C / 2) & (C - 1);
+        } else if (B.dsb) {
+          c = (C / 2 - m) & (C - 1);
+          cj = true;
+        } else if (m > 0) {
+          c = C - m;
+          cj = true;
+        }
+        const float2 x = cj ? make_float2(v[q].x, -v[q].y) : v[q];
+        const float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
+        const float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
+        if (A.prec == FXG_XMAC_FP32) {
+          ((float2*)A.spectra)[outbase + c] = y;
+        } else if (A.prec == FXG_XMAC_FP16) {
+          ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
+        } else if (A.prec == FXG_XMAC_FP8) {
+          ((unsigned short*)A.spectra)[outbase + c] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
+        } else {
+          ((__hip_bfloat162*)A.spectra)[outbase + c] = __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FFT lengths above 16384: L = 16 M and
 //     X[k2 + M k1] = sum_n1 W_16^(n1 k1) W_L^(n1 k2) Y_n1[k2],
 // Y_n1 the M-point FFT of x[n1 + 16 n2], n2 < M.  k_bigfft_a: one workgroup
@@ -1066,9 +1282,21 @@ hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const 
   return hipGetLastError();
 }
 
-hipError_t launch_fstage(int L, int nbits, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream) {
+hipError_t launch_fstage(int L, int nbits, int cplx, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream) {
   dim3 grid(nblocks, nbands);
   if (nbits != 2 && nbits != 8) return hipErrorInvalidValue;
+  if (cplx) {
+    switch (L) {
+#define CASE(Lv) case Lv: \
+      if (nbits == 2) hipLaunchKernelGGL((k_fstage_c<Lv, 2>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
+      else hipLaunchKernelGGL((k_fstage_c<Lv, 8>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
+      break;
+      CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192) CASE(16384)
+#undef CASE
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (L) {
 #define CASE(Lv) case Lv: \
       if (nbits == 2) hipLaunchKernelGGL((k_fstage<Lv, 2>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \

@@ -1308,8 +1308,10 @@ int orc_unpack(const fxg_datastream* ds, const uint8_t* data, int64_t nbytes,
   StreamFmt f; int r = make_fmt(ds, &f); if (r) return r;
   std::vector<float> work;
   Unpacked u = mk5_unpack(f, data, nbytes, sampleoffset, nsamp, work);
+  const int nc = f.cplx ? 2 : 1;     /* complex: (re, im) float pairs */
   for (int c = 0; c < f.nchan; c++)
-    std::memcpy(out + (size_t)c * nsamp, work.data() + (size_t)c * u.stride + u.munged, sizeof(float) * nsamp);
+    std::memcpy(out + (size_t)c * nsamp * nc, work.data() + ((size_t)c * u.stride + u.munged) * nc,
+                sizeof(float) * nsamp * nc);
   *weight = u.weight;
   return 0;
 }

@@ -1,0 +1,82 @@
+"""Complex-sampled VDIF (single and double sideband, USB and LSB, 2- and
+8-bit) through the GPU against the oracle: the FP32 path within 1e-5 of the
+FP32 oracle (spectra and visibilities), reduced precision against the FP64
+oracle, with a bad frame (a complex 8-bit decoder stops at it) and a
+flagged block."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fxcorr import JobSpec
+from fxsynth import make_subint
+from fxtestlib import GpuJob, oracle_results, oracle_spectra, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+
+
+def cspec(lsb, dsb, nbits=2, **kw):
+    base = dict(nstations=4, nsubbands=2, npol=2, bw_mhz=16.0, nchan=512, blockspersend=12, nbits=nbits,
+                lsb=[lsb, lsb], sampling="complex_dsb" if dsb else "complex", base_freq_mhz=8400.25,
+                clockoffset_us=[0.011, -0.02, 0.0, 0.004])
+    base.update(kw)
+    return JobSpec(**base)
+
+
+def cdata(spec, seed=5):
+    rng = np.random.default_rng(seed)
+    return make_subint(spec, seed=seed, rho=0.05, delays_us=rng.uniform(-20, 20, spec.nstations),
+                       rates_us_per_s=rng.uniform(-1, 1, spec.nstations),
+                       invalid_frames={1: [2]}, invalid_blocks={2: [3]})
+
+
+MODES = [(0, 0), (1, 0), (0, 1), (1, 1)]
+
+
+@pytest.mark.parametrize("nbits", [2, 8])
+@pytest.mark.parametrize("lsb,dsb", MODES)
+def test_complex_visibilities_fp32(gpu, oracle, lsb, dsb, nbits):
+    lib, ctx = gpu
+    spec = cspec(lsb, dsb, nbits)
+    job, keep = spec.build()
+    sd = cdata(spec)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    ref = oracle_results(oracle, job, sd, fp64=False, nthreads=8)
+    assert rel_err(got, ref) < FP32_TOL, rel_err(got, ref)
+    # spectra of one block of station 1 (whose frame 2 is bad) as Mode::getFreqs
+    for blk in (0, 5):
+        sp = oracle_spectra(oracle, job, sd, 1, blk, spec.nrecbands, spec.nchan)
+        for b in range(spec.nrecbands):
+            gs = g.spectrum(1, b, blk, spec.nchan)
+            assert rel_err(gs, sp[b]) < FP32_TOL, (blk, b, rel_err(gs, sp[b]))
+    g.close()
+
+
+@pytest.mark.parametrize("nchan", [128, 1024, 4096])
+def test_complex_fft_lengths_and_order2(gpu, oracle, nchan):
+    lib, ctx = gpu
+    spec = cspec(0, 1, 2, nchan=nchan, fringe_order=2, nstations=3, blockspersend=6)
+    job, keep = spec.build()
+    sd = cdata(spec, seed=8)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd, fp64=False, nthreads=8)
+    assert rel_err(got, ref) < FP32_TOL, rel_err(got, ref)
+    g.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp16", 1e-3), ("bf16", 8e-3)])
+@pytest.mark.parametrize("lsb,dsb", [(0, 0), (1, 1)])
+def test_complex_mixed_vs_fp64(gpu, oracle, prec, tol, lsb, dsb):
+    lib, ctx = gpu
+    spec = cspec(lsb, dsb, 2, xmac=prec)
+    job, keep = spec.build()
+    sd = cdata(spec, seed=9)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd, fp64=True, nthreads=8)
+    assert rel_err(got, ref) < tol, rel_err(got, ref)
+    g.close()
