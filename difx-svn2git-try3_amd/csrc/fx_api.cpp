@@ -87,6 +87,8 @@ struct fxg_job {
   DBuf d_pcal, d_pcalaux, d_pcaltone, d_pcalpart;   // phase-cal tables and partial sums
   DBuf d_vpart;                   // block-split XMAC partial matrices (zeroed once)
   DBuf d_cexp, d_eoff;            // fp8: scale exponents [gband][channel], per-input offsets
+  long long gstride = 16;         // spectra layout (cidx): 16 row-major, 16 x pass blocks grouped
+  int bstride = 0;                // elements between the spectra of consecutive blocks of a band
   // spectral kurtosis: sums [record][s1, s2][gband][C], records [record][gband][nout], flags
   int kurt_on = 0, kurt_maxch = 32, kurt_nrec = 1, kurt_nout = 0, kurt_have = 0;
   DBuf d_kurt, d_kurtout, d_kurtnz;
@@ -454,9 +456,21 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   j->scale = (prec == FXG_XMAC_FP32 || prec == FXG_XMAC_FP8)
                  ? 1.0f : std::ldexp(1.0f, -(int)std::lround(0.5 * std::log2((double)P.L)));
 
-  // spectra offsets: [global band][chunk block][C]
+  // spectra offsets: band g's chunk of spectra at g * chunk * C; within it
+  // channel-group-major [C/16][chunk][16] when every cross-multiply of the
+  // job runs on the matrix cores in 16-channel groups that start on a group
+  // boundary (each workgroup then streams whole lines), else row-major
   const long long chunk = P.chunk_blocks;
   for (size_t g = 0; g < P.bands.size(); g++) P.bands[g].specoff = (long long)g * chunk * P.C;
+  {
+    bool grouped = prec != FXG_XMAC_FP32 && P.C % 16 == 0;
+    for (const XmacFreq& x : P.xf) {
+      if (!xmac_mfma_supported((int)x.in_station.size(), x.nchan)) grouped = false;
+      for (long long o : x.in_off) if (o % 16) grouped = false;     // zoom channel offsets
+    }
+    j->gstride = grouped ? 16 * chunk : 16;
+    j->bstride = grouped ? 16 : P.C;
+  }
   std::vector<long long> inoff;
   std::vector<int> eoff;            // fp8: [gband][channel] index of each input's channel 0
   for (XmacFreq& x : P.xf) {
@@ -464,7 +478,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     for (size_t i = 0; i < x.in_station.size(); i++) {
       int gband = P.st[x.in_station[i]].band0 + x.in_band[i];
       eoff.push_back(gband * P.C + (int)x.in_off[i]);
-      x.in_off[i] = P.bands[gband].specoff + x.in_off[i];
+      x.in_off[i] = P.bands[gband].specoff + cidx((int)x.in_off[i], j->gstride);
       inoff.push_back(x.in_off[i]);
     }
   }
@@ -830,6 +844,8 @@ static void fill_fstage_args(fxg_job* j, FStageArgs* fa) {
   fa->rtabn = j->plan.rtabn;
   fa->zcb0 = 0;
   fa->cexp = j->d_cexp.p ? j->d_cexp.as<signed char>() : nullptr;
+  fa->gstride = j->gstride;
+  fa->bstride = j->bstride;
 }
 
 // identity of a subintegration's inputs: the prepared state is reused while it matches
@@ -865,6 +881,7 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
     ka.cexp = j->d_cexp.p ? j->d_cexp.as<signed char>() : nullptr;
     ka.nbands = (int)P.bands.size(); ka.C = P.C; ka.nout = j->kurt_nout;
     ka.band_stride = (long long)P.chunk_blocks * P.C;
+    ka.gstride = j->gstride; ka.bstride = j->bstride;
     ka.nblocks = nblocks; ka.nrec = j->kurt_nrec; ka.s = j->d_kurt.as<float>();
     // Mode::zeroKurtosis at the start of every subintegration (core.cpp:712-714)
     HIPCHK(launch_zero(ka.s, 2LL * ka.nrec * ka.nbands * ka.C, s));
@@ -893,7 +910,8 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       XmacArgs xa{};
       xa.spec = j->d_spec.p;
       xa.in_off = j->d_inoff.as<long long>() + j->inoff_base[x];
-      xa.block_stride = P.C;
+      xa.block_stride = j->bstride;
+      xa.gstride = j->gstride;
       xa.nin = (int)xf.in_station.size();
       xa.nchan = xf.nchan;
       xa.nblocks = cn;
@@ -1389,13 +1407,22 @@ int fxg_tap_spectrum(fxg_job* j, int ds, int band, int block, float* out) {
     return set_err(FXG_ESTATE, "block is not in the resident chunk (the last chunk of the subintegration)");
   HIPCHK(hipSetDevice(j->ctx->device));
   const DevBand& B = P.bands[P.st[ds].band0 + band];
-  long long off = B.specoff + (long long)(block - j->tap_chunk0) * P.C;
+  long long off = B.specoff + (long long)(block - j->tap_chunk0) * j->bstride;
+  // the C channels of the spectrum in channel order (gathered from the
+  // 16-channel groups of the grouped layout)
+  auto fetch = [&](void* dst, size_t es) -> int {
+    const char* src = (const char*)j->d_spec.p + off * (long long)es;
+    if (j->gstride == 16) HIPCHK(hipMemcpy(dst, src, es * (size_t)P.C, hipMemcpyDeviceToHost));
+    else HIPCHK(hipMemcpy2D(dst, 16 * es, src, (size_t)j->gstride * es, 16 * es, (size_t)P.C / 16, hipMemcpyDeviceToHost));
+    return 0;
+  };
+  int rc;
   if (j->desc.xmac_precision == FXG_XMAC_FP32) {
-    HIPCHK(hipMemcpy(out, j->d_spec.as<float2>() + off, sizeof(float2) * P.C, hipMemcpyDeviceToHost));
+    if ((rc = fetch(out, sizeof(float2)))) return rc;
   } else if (j->desc.xmac_precision == FXG_XMAC_FP8) {
     std::vector<uint8_t> h(2 * (size_t)P.C);
     std::vector<signed char> ex((size_t)P.C);
-    HIPCHK(hipMemcpy(h.data(), j->d_spec.as<uint16_t>() + off, 2 * (size_t)P.C, hipMemcpyDeviceToHost));
+    if ((rc = fetch(h.data(), 2))) return rc;
     HIPCHK(hipMemcpy(ex.data(), j->d_cexp.as<signed char>() + (size_t)(P.st[ds].band0 + band) * P.C, (size_t)P.C,
                      hipMemcpyDeviceToHost));
     for (size_t i = 0; i < h.size(); i++) {
@@ -1407,7 +1434,7 @@ int fxg_tap_spectrum(fxg_job* j, int ds, int band, int block, float* out) {
     }
   } else {
     std::vector<uint16_t> h(2 * (size_t)P.C);
-    HIPCHK(hipMemcpy(h.data(), j->d_spec.as<uint32_t>() + off, 4 * (size_t)P.C, hipMemcpyDeviceToHost));
+    if ((rc = fetch(h.data(), 4))) return rc;
     for (size_t i = 0; i < h.size(); i++) {
       float v;
       if (j->desc.xmac_precision == FXG_XMAC_BF16) {

@@ -14,6 +14,15 @@ constexpr int fxg_rotator_table_entries(int L) { return L >= 16384 ? 768 : 512; 
 // 65536) take the two-kernel path
 constexpr int kFStageMaxL = 16384;
 
+// Spectra layout.  Channel c of the spectrum of one (band, pass block) sits
+// at that spectrum's base + cidx(c, gstride), the bases of consecutive blocks
+// of a band bstride elements apart:
+//   gstride = 16, bstride = C: [band][block][channel] (row-major);
+//   gstride = 16 x pass blocks, bstride = 16: [band][channel / 16][block][16]
+//   (channel-group-major: the 16 channels one matrix-core workgroup owns are
+//   contiguous across blocks, so its loads are whole lines).
+__host__ __device__ inline long long cidx(int c, long long gstride) { return (long long)(c >> 4) * gstride + (c & 15); }
+
 struct FStageArgs {
   const DevStation* st;
   const DevBand* bands;
@@ -36,12 +45,15 @@ struct FStageArgs {
   int rtabn;                  // table entries per (band, block)
   int zcb0;                   // first chunk block of this sub-chunk
   const signed char* cexp;    // fp8: power-of-two scale exponent per [global band][channel]
+  long long gstride;          // spectra layout (cidx)
+  int bstride;
 };
 
 struct XmacArgs {
   const void* spec;
-  const long long* in_off;    // [nin] element offsets of the inputs' block 0
+  const long long* in_off;    // [nin] element offsets of the inputs' block 0, channel 0
   long long block_stride;     // elements between consecutive blocks of one input
+  long long gstride;          // channel c of an input's block at + cidx(c, gstride)
   int nin, nchan, nblocks;
   int CT, TB, ntiles, G;      // channel tile, blocks per stage, tiles, 8-groups
   float2* V;                  // [nin][nin][nchan]
@@ -79,6 +91,8 @@ struct KurtArgs {
   const signed char* cexp;    // fp8: scale exponent per [band][channel]
   int nbands, C, nout;        // recorded bands, channels, record channels
   long long band_stride;      // spectrum elements between bands
+  long long gstride;          // spectra layout (cidx)
+  int bstride;
   int chunk0, nchunk;         // blocks of this pass
   int nblocks, nrec;          // blocks per subintegration, records (threads)
   float* s;                   // [record][s1, s2][band][C]

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_kurt_accum(KurtArgs A) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   const int g = blockIdx.y;
   if (c >= A.C || g >= A.nbands) return;
-  const long long base = (long long)g * A.band_stride + c;
+  const long long base = (long long)g * A.band_stride + cidx(c, A.gstride);
   const int end = A.chunk0 + A.nchunk;
   int b = A.chunk0;
   while (b < end) {
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_kurt_accum(KurtArgs A) {
     float* p2 = p1 + (long long)A.nbands * A.C;
     float s1 = *p1, s2 = *p2;
     for (; b < tend; b++) {
-      const float2 x = kurt_load(A, base + (long long)(b - A.chunk0) * A.C, g, c);
+      const float2 x = kurt_load(A, base + (long long)(b - A.chunk0) * A.bstride, g, c);
       const float m = sqrtf(x.x * x.x + x.y * x.y);
       const float m2 = m * m;
       s1 = s1 + m2;

@@ -539,22 +539,24 @@ struct Fp8Out {
   const signed char* e;       // the band's channel exponents
 };
 
-__device__ __forceinline__ void put_spec(float2* o, int c, float2 y) { o[c] = y; }
-__device__ __forceinline__ void put_spec(__half2* o, int c, float2 y) { o[c] = __floats2half2_rn(y.x, y.y); }
-__device__ __forceinline__ void put_spec(__hip_bfloat162* o, int c, float2 y) { o[c] = __float22bfloat162_rn(y); }
-__device__ __forceinline__ void put_spec(const Fp8Out& o, int c, float2 y) { o.p[c] = fp8_pair(y, o.e[c]); }
+// channel c's value y stored at element i of the spectrum (i = cidx(c))
+__device__ __forceinline__ void put_spec(float2* o, long long i, int c, float2 y) { o[i] = y; }
+__device__ __forceinline__ void put_spec(__half2* o, long long i, int c, float2 y) { o[i] = __floats2half2_rn(y.x, y.y); }
+__device__ __forceinline__ void put_spec(__hip_bfloat162* o, long long i, int c, float2 y) { o[i] = __float22bfloat162_rn(y); }
+__device__ __forceinline__ void put_spec(const Fp8Out& o, long long i, int c, float2 y) { o.p[i] = fp8_pair(y, o.e[c]); }
 
 // zero spectrum of one (band, block): C values of the job's spectrum type
-__device__ __forceinline__ void zero_spec(void* spectra, int prec, long long outbase, int C, int tid, int nt) {
+__device__ __forceinline__ void zero_spec(void* spectra, int prec, long long outbase, int C, long long gs, int tid,
+                                          int nt) {
   if (prec == FXG_XMAC_FP32) {
     float2* o = (float2*)spectra + outbase;
-    for (int c = tid; c < C; c += nt) o[c] = make_float2(0.f, 0.f);
+    for (int c = tid; c < C; c += nt) o[cidx(c, gs)] = make_float2(0.f, 0.f);
   } else if (prec == FXG_XMAC_FP8) {
     unsigned short* o = (unsigned short*)spectra + outbase;
-    for (int c = tid; c < C; c += nt) o[c] = 0;
+    for (int c = tid; c < C; c += nt) o[cidx(c, gs)] = 0;
   } else {
     unsigned* o = (unsigned*)spectra + outbase;
-    for (int c = tid; c < C; c += nt) o[c] = 0u;
+    for (int c = tid; c < C; c += nt) o[cidx(c, gs)] = 0u;
   }
 }
 
@@ -566,8 +568,10 @@ __device__ __forceinline__ void zero_spec(void* spectra, int prec, long long out
 template <int L, typename TO>
 __device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* twlo, const float2* twhi,
                                              const float2* fsub, const float2* fstep, int s, int slog,
-                                             float scale, bool lsb, TO out, int j) {
+                                             float scale, bool lsb, TO out, long long gs, int j) {
   constexpr int NTF = L / 16, NSL = L / 2, RSTR = NTF + NTF / 16;
+  // channel j + u NTF is element cj + u ustep of the spectrum (NTF is a multiple of 16)
+  const long long cj = cidx(j, gs), ustep = (NTF / 16) * gs;
   const float2 wj = tw_of(twlo, twhi, j);
   float2 f0 = fsub[j & (s - 1)];
   f0 = make_float2(f0.x * scale, f0.y * scale);
@@ -589,7 +593,7 @@ __device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* tw
     const float2 x = lsb ? sub2(x0, t) : add2(x0, t);
     const float2 fr = cmulf(f0, *fsp);
     fsp += fss;
-    put_spec(out, j + u * NTF, make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x));
+    put_spec(out, cj + u * ustep, j + u * NTF, make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x));
   }
 }
 
@@ -617,10 +621,10 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   const DevStationSub Q = A.ss[B.station];
   const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
   const int C = L / 2;
-  const long long outbase = B.specoff + (long long)cb * C;
+  const long long outbase = B.specoff + (long long)cb * A.bstride;
 
   if (G.status != 0) {
-    zero_spec(A.spectra, A.prec, outbase, C, tid, NT);
+    zero_spec(A.spectra, A.prec, outbase, C, A.gstride, tid, NT);
     return;
   }
 
@@ -788,15 +792,15 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   //      (USB [0, C), real LSB [C, 2C)), then the fractional-sample rotation
   if (RL == 2 && tid < NTF && rotfast) {
     if (A.prec == FXG_XMAC_FP32)
-      last_pass_r2<L, float2*>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0, (float2*)A.spectra + outbase, tid);
+      last_pass_r2<L, float2*>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0, (float2*)A.spectra + outbase, A.gstride, tid);
     else if (A.prec == FXG_XMAC_FP16)
-      last_pass_r2<L, __half2*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0, (__half2*)A.spectra + outbase, tid);
+      last_pass_r2<L, __half2*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0, (__half2*)A.spectra + outbase, A.gstride, tid);
     else if (A.prec == FXG_XMAC_FP8)
       last_pass_r2<L, Fp8Out>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0,
-                              Fp8Out{(unsigned short*)A.spectra + outbase, A.cexp + (long long)gband * C}, tid);
+                              Fp8Out{(unsigned short*)A.spectra + outbase, A.cexp + (long long)gband * C}, A.gstride, tid);
     else
       last_pass_r2<L, __hip_bfloat162*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0,
-                                        (__hip_bfloat162*)A.spectra + outbase, tid);
+                                        (__hip_bfloat162*)A.spectra + outbase, A.gstride, tid);
   } else if (tid < NTF) {
     const int j = tid;
     const int mh = B.lsb ? RL / 2 : 0;
@@ -823,13 +827,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
         const float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
         const float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
         if (A.prec == FXG_XMAC_FP32) {
-          ((float2*)A.spectra)[outbase + c] = y;
+          ((float2*)A.spectra)[outbase + cidx(c, A.gstride)] = y;
         } else if (A.prec == FXG_XMAC_FP16) {
-          ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
+          ((__half2*)A.spectra)[outbase + cidx(c, A.gstride)] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
         } else if (A.prec == FXG_XMAC_FP8) {
-          ((unsigned short*)A.spectra)[outbase + c] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
+          ((unsigned short*)A.spectra)[outbase + cidx(c, A.gstride)] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
         } else {
-          ((__hip_bfloat162*)A.spectra)[outbase + c] =
+          ((__hip_bfloat162*)A.spectra)[outbase + cidx(c, A.gstride)] =
               __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
         }
       }
@@ -869,10 +873,10 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   const DevStationSub Q = A.ss[B.station];
   const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
   constexpr int C = L;
-  const long long outbase = B.specoff + (long long)cb * C;
+  const long long outbase = B.specoff + (long long)cb * A.bstride;
 
   if (G.status != 0) {
-    zero_spec(A.spectra, A.prec, outbase, C, tid, NT);
+    zero_spec(A.spectra, A.prec, outbase, C, A.gstride, tid, NT);
     return;
   }
   const long long t0 = G.nearestsample;
@@ -1027,13 +1031,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
         const float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
         const float2 y = make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x);
         if (A.prec == FXG_XMAC_FP32) {
-          ((float2*)A.spectra)[outbase + c] = y;
+          ((float2*)A.spectra)[outbase + cidx(c, A.gstride)] = y;
         } else if (A.prec == FXG_XMAC_FP16) {
-          ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
+          ((__half2*)A.spectra)[outbase + cidx(c, A.gstride)] = __floats2half2_rn(y.x * A.scale, y.y * A.scale);
         } else if (A.prec == FXG_XMAC_FP8) {
-          ((unsigned short*)A.spectra)[outbase + c] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
+          ((unsigned short*)A.spectra)[outbase + cidx(c, A.gstride)] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
         } else {
-          ((__hip_bfloat162*)A.spectra)[outbase + c] = __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
+          ((__hip_bfloat162*)A.spectra)[outbase + cidx(c, A.gstride)] = __float22bfloat162_rn(make_float2(y.x * A.scale, y.y * A.scale));
         }
       }
     }
@@ -1213,12 +1217,12 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   const DevStation S = A.st[B.station];
   const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
   const int k2 = blockIdx.z * 256 + tid;
-  const long long outbase = B.specoff + (long long)cb * C;
+  const long long outbase = B.specoff + (long long)cb * A.bstride;
   if (G.status != 0) {
     for (int q = 0; q < 8; q++) {
-      if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + k2 + M * q] = make_float2(0.f, 0.f);
-      else if (A.prec == FXG_XMAC_FP8) ((unsigned short*)A.spectra)[outbase + k2 + M * q] = 0;
-      else ((unsigned*)A.spectra)[outbase + k2 + M * q] = 0u;
+      if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + cidx(k2 + M * q, A.gstride)] = make_float2(0.f, 0.f);
+      else if (A.prec == FXG_XMAC_FP8) ((unsigned short*)A.spectra)[outbase + cidx(k2 + M * q, A.gstride)] = 0;
+      else ((unsigned*)A.spectra)[outbase + cidx(k2 + M * q, A.gstride)] = 0u;
     }
     return;
   }
@@ -1243,11 +1247,11 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
     const float2 x = v[mh + q];
     const float2 fr = slog >= 0 ? cmulf(fsub[c & (s - 1)], fstep[c >> slog]) : cmulf(fsub[c % s], fstep[c / s]);
     const float2 y = make_float2((x.x * fr.x - x.y * fr.y) * scale, (x.x * fr.y + x.y * fr.x) * scale);
-    if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + c] = y;
-    else if (A.prec == FXG_XMAC_FP16) ((__half2*)A.spectra)[outbase + c] = __floats2half2_rn(y.x, y.y);
+    if (A.prec == FXG_XMAC_FP32) ((float2*)A.spectra)[outbase + cidx(c, A.gstride)] = y;
+    else if (A.prec == FXG_XMAC_FP16) ((__half2*)A.spectra)[outbase + cidx(c, A.gstride)] = __floats2half2_rn(y.x, y.y);
     else if (A.prec == FXG_XMAC_FP8)
-      ((unsigned short*)A.spectra)[outbase + c] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
-    else ((__hip_bfloat162*)A.spectra)[outbase + c] = __float22bfloat162_rn(y);
+      ((unsigned short*)A.spectra)[outbase + cidx(c, A.gstride)] = fp8_pair(y, A.cexp[(long long)gband * C + c]);
+    else ((__hip_bfloat162*)A.spectra)[outbase + cidx(c, A.gstride)] = __float22bfloat162_rn(y);
   }
 }
 

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
       int tt = e / (CT * NP);
       float2 v = make_float2(0.f, 0.f);
       if (a < A.nin && t0 + tt < nb && c0 + c < A.nchan)
-        v = load_spec(spec, A.in_off[a] + (long long)(sb0 + t0 + tt) * A.block_stride + c0 + c);
+        v = load_spec(spec, A.in_off[a] + (long long)(sb0 + t0 + tt) * A.block_stride + cidx(c0 + c, A.gstride));
       sh[e] = v;
     }
     __syncthreads();

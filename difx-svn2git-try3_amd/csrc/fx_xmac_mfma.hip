@@ -80,7 +80,7 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
       const int a = it / (QN * TT);
       uint4 v = make_uint4(0u, 0u, 0u, 0u);
       if (a < A.nin && t0 + t < nb) {
-        const long long e = A.in_off[a] + (long long)(sb0 + t0 + t) * A.block_stride + c0 + 4 * q;
+        const long long e = A.in_off[a] + (long long)(sb0 + t0 + t) * A.block_stride + cidx(c0, A.gstride) + 4 * q;
         if ((e & 3) == 0) {
           v = *(const uint4*)(spec + e);
         } else {
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
 #pragma unroll
   for (int u = 0; u < LPW; u++) {
     const int a = wu * LPW + u;
-    ioff[u] = a < A.nin ? A.in_off[a] + c0 : -1;
+    ioff[u] = a < A.nin ? A.in_off[a] + cidx(c0, A.gstride) : -1;
   }
 
   auto issue = [&](int stage, int buf) {
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_fp8(XmacArgs A) {
       const int a = it / (QN * (TT / 2));
       uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
       if (a < A.nin) {
-        const long long e0 = A.in_off[a] + (long long)(sb0 + t0 + 2 * p) * A.block_stride + c0 + 8 * q;
+        const long long e0 = A.in_off[a] + (long long)(sb0 + t0 + 2 * p) * A.block_stride + cidx(c0 + 8 * q, A.gstride);
         if (t0 + 2 * p < nb) v0 = load8pairs(spec, e0);
         if (t0 + 2 * p + 1 < nb) v1 = load8pairs(spec, e0 + A.block_stride);
       }
