@@ -613,7 +613,10 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   __shared__ float lut8s[NBITS == 8 ? 256 : 4];  // level table indexed by the sample code
 
   const int tid = threadIdx.x;
-  const int cb = blockIdx.x;                      // block within chunk
+  // block within the chunk: workgroups are dealt to the 8 XCDs round-robin,
+  // so consecutive blocks go to one XCD (its L2 merges the halves of the
+  // lines that neighbouring blocks write in the grouped spectra layout)
+  const int cb = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
   const int blk = A.chunk0 + cb;
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
@@ -865,7 +868,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   __shared__ float lut8s[NBITS == 8 ? 256 : 4];
 
   const int tid = threadIdx.x;
-  const int cb = blockIdx.x;
+  const int cb = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
   const int blk = A.chunk0 + cb;
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
