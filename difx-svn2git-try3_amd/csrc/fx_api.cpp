@@ -463,7 +463,9 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   const long long chunk = P.chunk_blocks;
   for (size_t g = 0; g < P.bands.size(); g++) P.bands[g].specoff = (long long)g * chunk * P.C;
   {
-    bool grouped = prec != FXG_XMAC_FP32 && P.C % 16 == 0;
+    // (FFTs above 16384 points write their channels from a separate pass
+    // whose scattered 64-byte pieces cost more than the grouped reads save)
+    bool grouped = prec != FXG_XMAC_FP32 && P.C % 16 == 0 && P.L <= kFStageMaxL;
     for (const XmacFreq& x : P.xf) {
       if (!xmac_mfma_supported((int)x.in_station.size(), x.nchan)) grouped = false;
       for (long long o : x.in_off) if (o % 16) grouped = false;     // zoom channel offsets

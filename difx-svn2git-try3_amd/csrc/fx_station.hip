@@ -1212,7 +1212,8 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   constexpr int L = 16 * M, C = L / 2;
   __shared__ float2 tabs[fxg_rotator_table_entries(L)];
   const int tid = threadIdx.x;
-  const int cbl = blockIdx.x;                    // block within the sub-chunk
+  // block within the sub-chunk, consecutive blocks on one XCD (as k_fstage)
+  const int cbl = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
   const int cb = A.zcb0 + cbl;                   // block within the chunk
   const int blk = A.chunk0 + cb;
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
