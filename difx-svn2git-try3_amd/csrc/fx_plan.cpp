@@ -63,6 +63,17 @@ int choff_of_band(const fxg_datastream& ds, int k) {
 
 bool power_of_two(long v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// f and g cover the same sky band with opposite sidebands (the USB x LSB
+// products of configuration.cpp:2926-2937): equal lower edges and bandwidths,
+// and, for the channels to line up, equal channel counts and averaging
+bool sideband_pair(const fxg_job_desc* d, int f, int g) {
+  const fxg_freq& a = d->freqs[f];
+  const fxg_freq& b = d->freqs[g];
+  if (f == g || (a.lsb != 0) == (b.lsb != 0)) return false;
+  const double la = a.freq_mhz - (a.lsb ? a.bw_mhz : 0.0), lb = b.freq_mhz - (b.lsb ? b.bw_mhz : 0.0);
+  return la == lb && a.bw_mhz == b.bw_mhz && a.nchan == b.nchan && a.chans_to_avg == b.chans_to_avg;
+}
+
 // ---- phase cal ------------------------------------------------------------
 // gcd of two values rounded to integers (mathutil.cpp:8-30)
 long long gcd_ll(long long a, long long b) {
@@ -243,8 +254,9 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
         int b1 = bl.band1[f][p], b2 = bl.band2[f][p];
         REQUIRE(b1 >= 0 && b1 < a.nrecbands + a.nzoombands && b2 >= 0 && b2 < c.nrecbands + c.nzoombands,
                 FXG_EINVAL, "baseline band index");
-        REQUIRE(freq_of_band(a, b1) == fi && freq_of_band(c, b2) == fi, FXG_EINVAL,
-                "baseline band does not carry the baseline frequency");
+        const int f1 = freq_of_band(a, b1), f2 = freq_of_band(c, b2);
+        REQUIRE((f1 == fi || sideband_pair(d, f1, fi)) && (f2 == fi || sideband_pair(d, f2, fi)), FXG_EINVAL,
+                "baseline band carries neither the baseline frequency nor its opposite-sideband equivalent");
       }
     }
   }
@@ -512,10 +524,23 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     if (!need[f]) continue;
     XmacFreq x;
     x.freq = f; x.nchan = d->freqs[f].nchan;
+    // bands of the opposite sideband that baselines of f correlate against it
+    std::vector<std::vector<char>> cross(d->ndatastreams);
+    for (int s = 0; s < d->ndatastreams; s++)
+      cross[s].assign(d->datastreams[s].nrecbands + d->datastreams[s].nzoombands, 0);
+    for (int b = 0; b < d->nbaselines; b++) {
+      const int lf = P.bllocal[b][f];
+      if (lf < 0) continue;
+      const fxg_baseline& bl = d->baselines[b];
+      for (int pp = 0; pp < bl.npolproducts[lf]; pp++) {
+        cross[bl.ds1][bl.band1[lf][pp]] = 1;
+        cross[bl.ds2][bl.band2[lf][pp]] = 1;
+      }
+    }
     for (int s = 0; s < d->ndatastreams; s++) {
       const fxg_datastream& ds = d->datastreams[s];
       for (int k = 0; k < ds.nrecbands + ds.nzoombands; k++) {
-        if (freq_of_band(ds, k) != f) continue;
+        if (freq_of_band(ds, k) != f && !cross[s][k]) continue;
         x.in_station.push_back(s);
         x.in_band.push_back(parent_of_band(ds, k));
         x.in_total.push_back(k);

@@ -394,7 +394,18 @@ static int validate(const fxg_job_desc* j) {
       for (int p = 0; p < bl.npolproducts[f]; p++) {
         int b1 = bl.band1[f][p], b2 = bl.band2[f][p];
         if (b1 < 0 || b1 >= d1.nrecbands + d1.nzoombands || b2 < 0 || b2 >= d2.nrecbands + d2.nzoombands) return fail(FXG_EINVAL, "baseline band index");
-        if (ds_band_freqindex(d1, b1) != bl.freq_index[f] || ds_band_freqindex(d2, b2) != bl.freq_index[f]) return fail(FXG_EINVAL, "baseline band frequency mismatch");
+        /* a band carries the baseline frequency or covers the same sky band
+         * in the opposite sideband (the USB x LSB products of
+         * configuration.cpp:2926-2937), with equal channel counts */
+        auto carries = [&](int g) {
+          const int fi = bl.freq_index[f];
+          if (g == fi) return true;
+          const fxg_freq& a = j->freqs[g]; const fxg_freq& c = j->freqs[fi];
+          if ((a.lsb != 0) == (c.lsb != 0)) return false;
+          const double la = a.freq_mhz - (a.lsb ? a.bw_mhz : 0.0), lc = c.freq_mhz - (c.lsb ? c.bw_mhz : 0.0);
+          return la == lc && a.bw_mhz == c.bw_mhz && a.nchan == c.nchan && a.chans_to_avg == c.chans_to_avg;
+        };
+        if (!carries(ds_band_freqindex(d1, b1)) || !carries(ds_band_freqindex(d2, b2))) return fail(FXG_EINVAL, "baseline band frequency mismatch");
       }
     }
   }

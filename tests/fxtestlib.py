@@ -59,6 +59,39 @@ def oracle_kurtosis(orc, job, sd, ds, nthreads, maxchannels, nbands, nchan):
     return out.reshape(nthreads, nbands, nout), valid
 
 
+def mixed_sideband_job(spec, lsb_stations):
+    """spec's job (one subband, real sampling) with the stations in
+    lsb_stations recording the same sky band as lower sideband: a second
+    freq table entry, band edge + bandwidth, LSB, that their recorded band
+    carries; the baselines keep the USB entry (Configuration's rewrite of
+    LSB freqs correlated against USB, configuration.cpp:1143-1186).
+    Returns (job, keepalive)."""
+    job, keep = spec.build()
+    f0 = job.freqs[0]
+    freqs = (fxcorr.Freq * 2)()
+    C.memmove(C.addressof(freqs[0]), C.addressof(f0), C.sizeof(fxcorr.Freq))
+    freqs[1].freq_mhz = f0.freq_mhz + f0.bw_mhz
+    freqs[1].bw_mhz = f0.bw_mhz
+    freqs[1].lsb = 1
+    freqs[1].nchan = f0.nchan
+    freqs[1].chans_to_avg = f0.chans_to_avg
+    job.nfreqs = 2
+    job.freqs = C.cast(freqs, C.POINTER(fxcorr.Freq))
+    for s in lsb_stations:
+        job.datastreams[s].recfreq_index[0] = 1
+    return job, (keep, freqs)
+
+
+def usb_to_lsb_raw(spec, raw):
+    """The frames of a 1-band 2-bit USB stream as the same sky band seen in
+    LSB from an LO at the upper band edge: x_lsb[n] = (-1)^n x_usb[n], i.e.
+    the odd samples' codes k -> 3 - k (bits 2-3 and 6-7 of every byte)."""
+    assert spec.nrecbands == 1 and spec.nbits == 2 and spec.sampling == "real"
+    fr = raw.reshape(-1, spec.framebytes).copy()
+    fr[:, spec.headerbytes:] ^= 0xCC
+    return fr.reshape(-1)
+
+
 class GpuJob:
     """A job on the HIP library; results come back to the host."""
 

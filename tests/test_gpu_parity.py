@@ -458,3 +458,23 @@ def test_visibilities_fp8_vs_fp64(gpu, oracle, mk):
         assert rel_err(got[:xend], ref[:xend]) < FP8_TOL, rel_err(got[:xend], ref[:xend])
         assert rel_err(got[a0:a1], ref[a0:a1]) < FP8_TOL / 2, rel_err(got[a0:a1], ref[a0:a1])
     g.close()
+
+
+@pytest.mark.parametrize("xmac,tol,fp64", [("fp32", FP32_TOL, False), ("fp16", 1e-3, True)])
+def test_usb_against_lsb(gpu, oracle, xmac, tol, fp64):
+    """USB x LSB baseline products (configuration.cpp:2926-2937): stations 1
+    and 3 record the sky band as lower sideband, the others as upper; the
+    cross-sideband products ride in the USB frequency's cross-multiply."""
+    from fxtestlib import mixed_sideband_job, usb_to_lsb_raw
+    lib, ctx = gpu
+    spec = JobSpec(nstations=4, nsubbands=1, npol=1, bw_mhz=16.0, nchan=512, blockspersend=10, pols="R",
+                   xmac=xmac)
+    job, keep = mixed_sideband_job(spec, [1, 3])
+    sd = make_data(spec, rho=0.1)
+    for s in (1, 3):
+        sd.stations[s].raw = usb_to_lsb_raw(spec, sd.stations[s].raw)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd)
+    ref = oracle_results(oracle, job, sd, fp64=fp64, nthreads=8)
+    assert rel_err(got, ref) < tol, rel_err(got, ref)
+    g.close()

@@ -26,7 +26,8 @@ import pytest
 
 from fxcorr import JobSpec, check
 from fxsynth import make_subint
-from fxtestlib import oracle_block_info, oracle_results, oracle_spectra, plan_info, rel_err
+from fxtestlib import (mixed_sideband_job, oracle_block_info, oracle_results, oracle_spectra, plan_info, rel_err,
+                       usb_to_lsb_raw)
 
 
 def unpacked(orc, spec, job, sd, ds, ns):
@@ -223,3 +224,28 @@ def test_fp32_matches_fp64(oracle):
     r32 = oracle_results(oracle, job, sd, fp64=False)
     r64 = oracle_results(oracle, job, sd, fp64=True)
     assert rel_err(r32, r64) < 1e-6
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_usb_against_lsb_phase_equals_sky_frequency_times_delay(oracle, order):
+    """A USB station against an LSB station recording the same sky band (the
+    USB x LSB baseline products of configuration.cpp:2926-2937): the LSB
+    voltages are the USB ones mirrored about the band centre, x[n] (-1)^n,
+    and the delay models differ by delta; the cross spectrum's phase is
+    2 pi (sky frequency) delta in every channel, channel c of both bands
+    being the same sky frequency."""
+    spec = JobSpec(nstations=2, nsubbands=1, npol=1, bw_mhz=16.0, nchan=256, blockspersend=8,
+                   base_freq_mhz=8400.0, fringe_order=order, pols="R")
+    job, keep = mixed_sideband_job(spec, [1])
+    ts = spec.sampletime_us
+    delta = 0.3 * ts
+    sd = make_subint(spec, seed=10, delays_us=[0.0, 0.0])
+    sd.stations[1].raw = usb_to_lsb_raw(spec, sd.stations[0].raw)
+    sd.stations[1].interpolator = np.array([0.0, 0.0, delta])
+    res = oracle_results(oracle, job, sd, fp64=True)
+    Cn, bw = spec.nchan, spec.bw_mhz
+    v = res[0:Cn]
+    sky = spec.base_freq_mhz + np.arange(Cn) * bw / Cn
+    got = v / np.abs(v)
+    want = np.exp(2j * np.pi * sky * delta)
+    assert np.max(np.abs(got - want)) < 1e-9, np.max(np.abs(got - want))
