@@ -37,6 +37,7 @@ struct DevBand {
   int dsb;              // complex double sideband: halves swapped, frac steps offset by -BW/2
   double lofreq, clock, clockdelta, looffset;   // lofreq: the fringe rotator's LO
   long long specoff;    // element offset of this band in the spectra buffer
+  float polre, polim;   // polarisation phase correction of the band's spectrum (1, 0: none)
 };
 
 // ---- per subintegration, per datastream ----------------------------------

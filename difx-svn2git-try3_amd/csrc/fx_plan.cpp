@@ -226,7 +226,6 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
     REQUIRE((ds.framebytes - ds.headerbytes) % 8 == 0, FXG_EINVAL, "payload must be a multiple of 8 bytes");
     for (int i = 0; i < ds.nrecfreqs; i++) {
       REQUIRE(ds.recfreq_index[i] >= 0 && ds.recfreq_index[i] < d->nfreqs, FXG_EINVAL, "recfreq_index");
-      REQUIRE(ds.recfreq_phaseoffset_deg[i] == 0.0, FXG_EUNSUPPORTED, "pol phase offsets are not built");
     }
     for (int b = 0; b < ds.nrecbands; b++)
       REQUIRE(ds.recband_freq[b] >= 0 && ds.recband_freq[b] < ds.nrecfreqs, FXG_EINVAL, "recband_freq");
@@ -438,6 +437,23 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       b.clock = ds.recfreq_clockoffset_us[lf];
       b.clockdelta = ds.recfreq_clockoffsetdelta_us[lf];
       b.looffset = ds.recfreq_looffset_hz[lf];
+      // polarisation phase offset (mode.cpp:435-460, 1341-1353): with two or
+      // more bands of the frequency, the second of the first two if the first
+      // is R, else the first, is turned by phasecorrA = e^{i offset/2}
+      b.polre = 1.0f; b.polim = 0.0f;
+      {
+        int i0 = -1, i1 = -1;
+        for (int q = 0; q < ds.nrecbands; q++)
+          if (ds.recband_freq[q] == lf) { if (i0 < 0) i0 = q; else if (i1 < 0) i1 = q; }
+        bool anyoff = false;
+        for (int i = 0; i < ds.nrecfreqs; i++) anyoff |= ds.recfreq_phaseoffset_deg[i] != 0.0;
+        const int lcp = i1 < 0 ? -1 : (ds.recband_pol[i0] == 'R' ? i1 : i0);
+        if (anyoff && k == lcp) {
+          const double degphase = -ds.recfreq_phaseoffset_deg[lf] / 2;
+          b.polre = (float)std::cos(degphase * M_PI / 180.0);
+          b.polim = (float)-std::sin(degphase * M_PI / 180.0);
+        }
+      }
       P.bands.push_back(b);
       band++;
     }

@@ -514,7 +514,9 @@ __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStatio
       float arg = cf * val;
       if (S.order == 0) arg = arg + postf;
       float sn, cs; sincosf(arg, &sn, &cs);
-      fsub[k] = make_float2(cs, sn);
+      // the band's polarisation phase correction rides on every channel's
+      // fractional-sample rotator (a constant factor)
+      fsub[k] = cmulf(make_float2(cs, sn), make_float2(B.polre, B.polim));
     } else {
       const int j = e - e3;
       // step channel frequencies (mode.cpp:328-334, 864-880): DSB, LSB, USB

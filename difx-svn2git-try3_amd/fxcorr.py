@@ -189,6 +189,7 @@ class JobSpec:
     lsb: Sequence[int] = ()
     clockoffset_us: Sequence[float] = ()
     looffset_hz: Sequence[float] = ()
+    phaseoffset_deg: Sequence = ()      # per station: one value, or one per subband
     zoom: Sequence[tuple] = ()          # (subband, channel offset, nchan) per zoom freq
     arraystridelen: int = 0
     xmacstridelen: int = 0
@@ -268,6 +269,9 @@ class JobSpec:
                 d.recfreq_index[sb] = sb
                 d.recfreq_clockoffset_us[sb] = self.clockoffset_us[s] if s < len(self.clockoffset_us) else 0.0
                 d.recfreq_looffset_hz[sb] = self.looffset_hz[s] if s < len(self.looffset_hz) else 0.0
+                if s < len(self.phaseoffset_deg):
+                    po = self.phaseoffset_deg[s]
+                    d.recfreq_phaseoffset_deg[sb] = po[sb] if isinstance(po, (list, tuple)) else po
                 for p in range(npol):
                     d.recband_freq[sb * npol + p] = sb
                     recpol[sb * npol + p] = ord(self.pols[p])

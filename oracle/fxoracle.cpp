@@ -330,6 +330,7 @@ struct DsGeom {
   double bw, sampletime;          /* MHz, µs */
   int order;                      /* fringe rotation order 0, 1, 2 */
   bool fractionalLoFreq, deltapoloffsets;
+  std::vector<double> polre, polim;  /* per recorded band: phasecorrA of the band, or (1, 0) */
   int bytesperblocknum, bytesperblockden, samplesperblock;
 };
 
@@ -566,7 +567,23 @@ static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g)
     if (g->dsb) loval += fi.lsb ? -fi.bw_mhz / 2.0 : fi.bw_mhz / 2.0;     /* mode.cpp:95-105 */
     if (std::fabs(loval - (int)(loval + 0.5)) > TINY) g->fractionalLoFreq = true;
     if (ds.recfreq_clockoffsetdelta_us[i] != 0.0) g->deltapoloffsets = true;
-    if (ds.recfreq_phaseoffset_deg[i] != 0.0) return fail(FXG_EUNSUPPORTED, "pol phase offsets");
+  }
+  /* polarisation phase offsets (mode.cpp:337-346, 435-460, 1341-1353): when
+   * any recorded frequency has one, each frequency with two or more bands
+   * turns its LCP band (the second of its first two bands if the first is R,
+   * else the first) by phasecorrA = cos(d) - i sin(d), d = -offset/2 degrees */
+  g->polre.assign(ds.nrecbands, 1.0); g->polim.assign(ds.nrecbands, 0.0);
+  bool phasepoloffset = false;
+  for (int i = 0; i < ds.nrecfreqs; i++) phasepoloffset |= ds.recfreq_phaseoffset_deg[i] != 0.0;
+  for (int i = 0; phasepoloffset && i < ds.nrecfreqs; i++) {
+    int idx[2] = {-1, -1}, count = 0;
+    for (int b = 0; b < ds.nrecbands; b++)
+      if (ds.recband_freq[b] == i) { if (count < 2) idx[count] = b; count++; }
+    if (count < 2) continue;
+    const int lcp = ds.recband_pol[idx[0]] == 'R' ? idx[1] : idx[0];
+    const double degphase = -ds.recfreq_phaseoffset_deg[i] / 2;
+    g->polre[lcp] = std::cos(degphase * M_PI / 180.0);
+    g->polim[lcp] = -std::sin(degphase * M_PI / 180.0);
   }
   if ((g->L & (g->L - 1)) != 0) return fail(FXG_EUNSUPPORTED, "fftchannels must be a power of two");
   /* Mk5Mode: blockclock = 2*bw, so samplesperblock = 1 (mode.cpp:129-141) */
@@ -832,6 +849,12 @@ static BlockGeom station_block(const fxg_job_desc* j, const DsGeom& g, int d,
         }
         if (prefrac) prefrac[(size_t)jb * C + c] = v;     /* fftoutputs before the frac rotation */
         cpx<T> o; o.re = v.re * fr[c].re - v.im * fr[c].im; o.im = v.re * fr[c].im + v.im * fr[c].re;
+        if (g.polre[jb] != 1.0 || g.polim[jb] != 0.0) {   /* vectorMulC_cf32_I(phasecorrA) */
+          const T pr = (T)g.polre[jb], pi = (T)g.polim[jb];
+          const T re = o.re * pr - o.im * pi;
+          o.im = o.re * pi + o.im * pr;
+          o.re = re;
+        }
         out[c] = o;
       }
     }

@@ -478,3 +478,46 @@ def test_usb_against_lsb(gpu, oracle, xmac, tol, fp64):
     ref = oracle_results(oracle, job, sd, fp64=fp64, nthreads=8)
     assert rel_err(got, ref) < tol, rel_err(got, ref)
     g.close()
+
+
+POL_OFFSETS = [[30.0, -70.0], [110.0, 0.0], 45.0, [-15.0, 200.0]]
+
+
+@pytest.mark.parametrize("case", ["real-fp32", "real-big-fp32", "lsb-order0-fp32", "complex-fp32", "zoom-fp16",
+                                  "real-bf16"])
+def test_pol_phase_offsets(gpu, oracle, case):
+    """Recorded-frequency polarisation phase offsets (mode.cpp:435-460,
+    1341-1353): each frequency's LCP band turned by e^{i theta/2}, the turn
+    riding on the fractional-sample rotators; FP32 against the FP32 oracle
+    (visibilities, autocorrelations and one block's spectra), reduced
+    precision against the FP64 oracle; a zoom band inherits its parent's
+    turn."""
+    lib, ctx = gpu
+    kw = dict(nstations=4, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=10,
+              phaseoffset_deg=POL_OFFSETS)
+    if case == "real-big-fp32":
+        kw.update(nchan=32768, blockspersend=3, nstations=3, phaseoffset_deg=POL_OFFSETS[:3])
+    if case == "lsb-order0-fp32":
+        kw.update(lsb=[1, 0], fringe_order=0, pols="LR")
+    if case == "complex-fp32":
+        kw.update(sampling="complex_dsb", lsb=[0, 1], pols="XY")
+    if case == "zoom-fp16":
+        kw.update(xmac="fp16", zoom=[(1, 32, 64)])
+    if case == "real-bf16":
+        kw.update(xmac="bf16")
+    spec = JobSpec(**kw)
+    job, keep = spec.build()
+    sd = make_data(spec, rho=0.1, invalid_blocks={1: [2]})
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    fp64 = not case.endswith("fp32")
+    ref = oracle_results(oracle, job, sd, fp64=fp64, nthreads=8)
+    tol = FP32_TOL if not fp64 else MIXED_TOL[spec.xmac]
+    assert rel_err(got, ref) < tol, rel_err(got, ref)
+    if not fp64:
+        blk = spec.blockspersend - 1
+        for ds in range(spec.nstations):
+            refs = oracle_spectra(oracle, job, sd, ds, blk, spec.nrecbands, spec.nchan)
+            for band in range(spec.nrecbands):
+                assert rel_err(g.spectrum(ds, band, blk, spec.nchan), refs[band]) < FP32_TOL, (ds, band)
+    g.close()

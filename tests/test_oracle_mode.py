@@ -249,3 +249,63 @@ def test_usb_against_lsb_phase_equals_sky_frequency_times_delay(oracle, order):
     got = v / np.abs(v)
     want = np.exp(2j * np.pi * sky * delta)
     assert np.max(np.abs(got - want)) < 1e-9, np.max(np.abs(got - want))
+
+
+def _polturn(spec, s, band):
+    """e^{i theta/2} of the band turned by station s's phase offset theta
+    (the second of a frequency's first two bands if the first is R, else the
+    first), 1 for the other band."""
+    npol = spec.npol
+    sb, p = divmod(band, npol)
+    turned = 1 if spec.pols[0] == "R" else 0
+    po = spec.phaseoffset_deg[s]
+    th = po[sb] if isinstance(po, (list, tuple)) else po
+    return np.exp(0.5j * np.deg2rad(th)) if p == turned else 1.0
+
+
+@pytest.mark.parametrize("pols", ["RL", "LR", "XY"])
+def test_pol_phase_offset_turns_one_band_per_frequency(oracle, pols):
+    """Recorded-frequency polarisation phase offsets (mode.cpp:337-346,
+    435-460, 1341-1353): with any offset set, each frequency's LCP band (the
+    second of its first two bands when the first is R, otherwise the first)
+    is turned by e^{i theta/2} after the fractional-sample rotation.  Against
+    the same data without offsets, product p x q* of stations a, b changes by
+    turn(a, p) conj(turn(b, q)); self autocorrelations do not change and the
+    cross-pol ones change by turn(R) conj(turn(L)) and its conjugate.  The
+    offsets differ per station and subband, one of them zero."""
+    base = dict(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=128, blockspersend=6,
+                base_freq_mhz=8400.0, pols=pols, clockoffset_us=[0.01, -0.02, 0.0])
+    spec0 = JobSpec(**base)
+    spec1 = JobSpec(**base, phaseoffset_deg=[[30.0, -70.0], [110.0, 0.0], 45.0])
+    job0, k0 = spec0.build()
+    job1, k1 = spec1.build()
+    sd = make_subint(spec0, seed=21, rho=0.2, delays_us=[0.3, -1.1, 0.7])
+    r0 = oracle_results(oracle, job0, sd, fp64=True)
+    r1 = oracle_results(oracle, job1, sd, fp64=True)
+    Cn = spec0.nchan
+    off = C.c_int64()
+    pairs = [(i, j) for i in range(3) for j in range(i + 1, 3)]
+    prods = [(0, 0), (0, 1), (1, 0), (1, 1)]
+    checked = 0
+    for b, (sa, sb_) in enumerate(pairs):
+        for f in range(spec0.nsubbands):
+            check(oracle, oracle.orc_plan_baseline_offset(C.byref(job1), f, b, C.byref(off)), "bl")
+            for k, (p, q) in enumerate(prods):
+                s0 = slice(off.value + k * Cn, off.value + (k + 1) * Cn)
+                want = r0[s0] * _polturn(spec1, sa, 2 * f + p) * np.conj(_polturn(spec1, sb_, 2 * f + q))
+                assert np.max(np.abs(r1[s0] - want)) <= 1e-9 * np.max(np.abs(r0[s0])), (b, f, p, q)
+                checked += 1
+    assert checked == 3 * 2 * 4
+    for s in range(3):
+        check(oracle, oracle.orc_plan_autocorr_offset(C.byref(job1), s, C.byref(off)), "ac")
+        a = off.value
+        nb = spec0.nrecbands
+        self0, self1 = r0[a:a + nb * Cn], r1[a:a + nb * Cn]
+        assert np.max(np.abs(self1 - self0)) <= 1e-9 * np.max(np.abs(self0))
+        for band in range(nb):
+            other = band ^ 1
+            x = slice(a + (nb + band) * Cn, a + (nb + band + 1) * Cn)
+            want = r0[x] * _polturn(spec1, s, band) * np.conj(_polturn(spec1, s, other))
+            assert np.max(np.abs(r1[x] - want)) <= 1e-9 * np.max(np.abs(r0[x])), (s, band)
+    # the turn is not the identity anywhere it should act
+    assert abs(_polturn(spec1, 2, 1 if pols[0] == "R" else 0) - 1.0) > 0.3
