@@ -1081,6 +1081,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   using Sh = FftShape<M>;
   constexpr int L = 16 * M;
   constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
+  static_assert(NTF % 16 == 0, "M/16 threads, a multiple of 16");
   __shared__ float2 buf[Sh::PAD];
   __shared__ float2 tabs[fxg_rotator_table_entries(L)];
   __shared__ float2 twlo[64], twhi[L / 64];
@@ -1123,6 +1124,29 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   if (tid < NTF) {
     const int j = tid;
     float2 v[16];
+    // samples n1 + 16 (j + r M/16) lie WSTEP whole stream words apart at one
+    // bit offset, and (16 M/16 being a multiple of the stride s) share one
+    // sub-rotator value, their step values sstep apart
+    constexpr int WSTEP = (16 * NBITS * NTF) / 32;
+    const int n0 = n1 + 16 * j;
+    const long long bq = bit0 + (long long)NBITS * n0;
+    const long long wq = bq >> 5;
+    const int sh = (int)(bq & 31);
+    const bool wordfast = fast && S.order == 1 && slog >= 0 && ((16 * NTF) & (s - 1)) == 0 &&
+                          wq + 15LL * WSTEP < nwords;
+    if (wordfast) {
+      const float2 r0 = rsub[n0 & (s - 1)];
+      const float2* stp = rstep + (n0 >> slog);
+      const int sstep = (16 * NTF) >> slog;
+      const unsigned* wp = stream + wq;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float x = lut8s[(wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u)];
+        const float2 rot = cmulf(r0, *stp);     // sub[k] * step[j], mode.cpp:975-977
+        stp += sstep;
+        v[r] = make_float2(x * rot.x, x * rot.y);
+      }
+    } else {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       const int n = n1 + 16 * (j + r * NTF);       // sample of the block
@@ -1149,6 +1173,7 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
       }
       v[r] = S.order == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
     }
+    }
     DFT<16>::run(v);
     float2* o = buf + 17 * j;
 #pragma unroll
@@ -1166,9 +1191,11 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
       const int k = j & (Ns - 1);
       float2 w[16];
       twiddles16(twlo, twhi, 16 * k * (M / (Ns * 16)), w);
+      constexpr int RSTR = NTF + NTF / 16;     // pidx(j + r NTF) = pidx(j) + r RSTR (NTF a multiple of 16)
+      const float2* ip = buf + pidx(j);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
-        const float2 x = buf[pidx(j + r * NTF)];
+        const float2 x = ip[r * RSTR];
         v[r] = r == 0 ? x : cmulf(x, w[r]);
       }
       DFT<16>::run(v);
