@@ -681,10 +681,13 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
   chunk = std::min(chunk, d->blockspersend);
   P.chunk_blocks = chunk;
   P.info.chunk_blocks = chunk;
-  // FFTs above 16384 points: a scratch of about 2 GiB takes zcap blocks of
-  // the chunk at a time, with their rotator tables
+  // FFTs above 16384 points: a scratch of about 160 MiB takes zcap blocks of
+  // the chunk at a time, with their rotator tables; at that size the second
+  // kernel reads back what the first wrote from the 256 MB Infinity Cache
+  // rather than from HBM (measured: the 65536-point job of the uneven mix
+  // 54.0 -> 51.5 ms per subint against a 2 GiB scratch)
   if (P.L > 16384) {
-    P.zcap = (int)std::max(1LL, std::min<long long>(chunk, (2LL << 30) / ((long long)P.nbands_total * P.L * 8)));
+    P.zcap = (int)std::max(1LL, std::min<long long>(chunk, (160LL << 20) / ((long long)P.nbands_total * P.L * 8)));
     for (const DevStation& st : P.st) P.rtabn = std::max(P.rtabn, 3 * st.s + st.nfrsteps + st.nfracsteps);
   }
   // partial matrices for XMAC launches with few channel groups (block split):
