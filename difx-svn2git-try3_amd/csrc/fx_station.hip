@@ -567,10 +567,10 @@ __device__ __forceinline__ void zero_spec(void* spectra, int prec, long long out
 // butterfly; channel c = k has c mod s = j mod s (one fsub value per thread,
 // the power-of-two spectrum scale folded into it exactly); then the
 // fractional-sample rotation fsub[c mod s] * fstep[c / s] (mode.cpp:1228-1245).
-template <int L, typename TO>
+template <int L, bool LSB, typename TO>
 __device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* twlo, const float2* twhi,
                                              const float2* fsub, const float2* fstep, int s, int slog,
-                                             float scale, bool lsb, TO out, long long gs, int j) {
+                                             float scale, TO out, long long gs, int j) {
   constexpr int NTF = L / 16, NSL = L / 2, RSTR = NTF + NTF / 16;
   // channel j + u NTF is element cj + u ustep of the spectrum (NTF is a multiple of 16)
   const long long cj = cidx(j, gs), ustep = (NTF / 16) * gs;
@@ -592,7 +592,7 @@ __device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* tw
     if (u == 5) t = cmulf(t, make_float2(-s1, -c1));
     if (u == 6) t = cmulf(t, make_float2(-h, -h));
     if (u == 7) t = cmulf(t, make_float2(-c1, -s1));
-    const float2 x = lsb ? sub2(x0, t) : add2(x0, t);
+    const float2 x = LSB ? sub2(x0, t) : add2(x0, t);
     const float2 fr = cmulf(f0, *fsp);
     fsp += fss;
     put_spec(out, cj + u * ustep, j + u * NTF, make_float2(x.x * fr.x - x.y * fr.y, x.x * fr.y + x.y * fr.x));
@@ -604,15 +604,25 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   using Sh = FftShape<L>;
   constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
   constexpr int RAWW = (L * NBITS) / 32 + 2;    // staged stream words of one block
-  __shared__ float2 buf[Sh::PAD];
-  __shared__ float2 tabs[Sh::TABS];             // rotator sub/step tables, packed
-  __shared__ float2 twlo[64], twhi[L / 64];     // e^{-2 pi i m/L}, e^{-2 pi i 64h/L}
   // 8-bit samples at the largest L: the staged stream words share the tail of
   // the FFT buffer (pass 1 reads them all before its first write, below)
   constexpr bool kAliasRaw = NBITS == 8 && L >= 16384;
-  __shared__ unsigned raw_own[kAliasRaw ? 1 : RAWW];
-  unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : raw_own;
-  __shared__ float lut8s[NBITS == 8 ? 256 : 4];  // level table indexed by the sample code
+  // One LDS block with the small tables first, so that they lie inside the
+  // 64 KiB an LDS instruction's immediate offset reaches.
+  struct Smem {
+    float2 tabs[Sh::TABS];                      // rotator sub/step tables, packed
+    float2 twlo[64], twhi[L / 64];              // e^{-2 pi i m/L}, e^{-2 pi i 64h/L}
+    float lut8s[NBITS == 8 ? 256 : 4];          // level table indexed by the sample code
+    unsigned raw_own[kAliasRaw ? 2 : RAWW + (RAWW & 1)];
+    float2 buf[Sh::PAD];
+  };
+  __shared__ Smem sm;
+  float2 (&buf)[Sh::PAD] = sm.buf;
+  float2 (&tabs)[Sh::TABS] = sm.tabs;
+  float2 (&twlo)[64] = sm.twlo;
+  float2 (&twhi)[L / 64] = sm.twhi;
+  float (&lut8s)[NBITS == 8 ? 256 : 4] = sm.lut8s;
+  unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : sm.raw_own;
 
   const int tid = threadIdx.x;
   // block within the chunk: workgroups are dealt to the 8 XCDs round-robin,
@@ -796,16 +806,23 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   // ---- last pass (radix RL, Ns = L/RL): only the bins of the kept sideband
   //      (USB [0, C), real LSB [C, 2C)), then the fractional-sample rotation
   if (RL == 2 && tid < NTF && rotfast) {
-    if (A.prec == FXG_XMAC_FP32)
-      last_pass_r2<L, float2*>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0, (float2*)A.spectra + outbase, A.gstride, tid);
-    else if (A.prec == FXG_XMAC_FP16)
-      last_pass_r2<L, __half2*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0, (__half2*)A.spectra + outbase, A.gstride, tid);
-    else if (A.prec == FXG_XMAC_FP8)
-      last_pass_r2<L, Fp8Out>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, B.lsb != 0,
-                              Fp8Out{(unsigned short*)A.spectra + outbase, A.cexp + (long long)gband * C}, A.gstride, tid);
-    else
-      last_pass_r2<L, __hip_bfloat162*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale, B.lsb != 0,
-                                        (__hip_bfloat162*)A.spectra + outbase, A.gstride, tid);
+    // the sideband is a branch around the whole pass, not a select per output
+#define FX_LAST_R2(LSB)                                                                                           \
+    if (A.prec == FXG_XMAC_FP32)                                                                                  \
+      last_pass_r2<L, LSB, float2*>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, (float2*)A.spectra + outbase,    \
+                                    A.gstride, tid);                                                              \
+    else if (A.prec == FXG_XMAC_FP16)                                                                             \
+      last_pass_r2<L, LSB, __half2*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale,                              \
+                                     (__half2*)A.spectra + outbase, A.gstride, tid);                              \
+    else if (A.prec == FXG_XMAC_FP8)                                                                              \
+      last_pass_r2<L, LSB, Fp8Out>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f,                                   \
+                                   Fp8Out{(unsigned short*)A.spectra + outbase, A.cexp + (long long)gband * C},   \
+                                   A.gstride, tid);                                                               \
+    else                                                                                                          \
+      last_pass_r2<L, LSB, __hip_bfloat162*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale,                      \
+                                             (__hip_bfloat162*)A.spectra + outbase, A.gstride, tid);
+    if (B.lsb) { FX_LAST_R2(true) } else { FX_LAST_R2(false) }
+#undef FX_LAST_R2
   } else if (tid < NTF) {
     const int j = tid;
     const int mh = B.lsb ? RL / 2 : 0;
