@@ -406,16 +406,23 @@ __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restric
   const long long total = Q.nframes * spf;
   const long long base = (long long)blockIdx.x * TPW;
   if (base >= total) return;                            // the whole workgroup
+  // frame f and sample-time r within it of this thread's sample-time, kept
+  // up to date as it steps by 256 (one division per workgroup)
+  const long long fw = base / spf;
+  long long f = fw;
+  long long r = base - fw * spf + threadIdx.x;
+  while (r >= spf) { r -= spf; f++; }
   for (int i = threadIdx.x; i < TPW; i += 256) {
     const long long tau = base + i;
     VW v;
     if (tau < total) {
-      const long long f = tau / spf;
-      v = ((const VW*)(Q.data + f * S.framebytes + S.headerbytes))[tau - f * spf];
+      v = ((const VW*)(Q.data + f * S.framebytes + S.headerbytes))[r];
     } else {
       memset(&v, 0, sizeof v);
     }
     buf[i + (i >> 4)] = v;
+    r += 256;
+    while (r >= spf) { r -= spf; f++; }
   }
   __syncthreads();
   const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
