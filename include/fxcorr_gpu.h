@@ -104,6 +104,10 @@ typedef struct {
   int32_t recfreq_index[FXG_MAX_RECFREQS];       /* freq table index          */
   double  recfreq_clockoffset_us[FXG_MAX_RECFREQS];
   double  recfreq_clockoffsetdelta_us[FXG_MAX_RECFREQS];
+  /* polarisation phase offset: when any is set, each frequency with two or
+   * more bands turns one of its first two (the second if the first is 'R',
+   * else the first) by exp(i offset/2) after the fractional-sample
+   * rotation (mode.cpp:435-460, 1341-1353) */
   double  recfreq_phaseoffset_deg[FXG_MAX_RECFREQS];
   double  recfreq_looffset_hz[FXG_MAX_RECFREQS];
   /* per recorded band: local freq index and polarisation ('R','L','X','Y') */
