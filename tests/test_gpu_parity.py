@@ -521,3 +521,30 @@ def test_pol_phase_offsets(gpu, oracle, case):
             for band in range(spec.nrecbands):
                 assert rel_err(g.spectrum(ds, band, blk, spec.nchan), refs[band]) < FP32_TOL, (ds, band)
     g.close()
+
+
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("nchan,nbits", [(32768, 2), (16384, 8)])
+def test_largest_fft_fringe_orders(gpu, oracle, order, nchan, nbits):
+    """The two-kernel 65536- and 32768-point transforms with post-F (order 0)
+    and quadratic (order 2) fringe rotation: the general sample path of the
+    first kernel (its word-aligned fast path is first order only), FP32
+    against the FP32 oracle, with a curved delay model."""
+    lib, ctx = gpu
+    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=64.0, nchan=nchan, blockspersend=4, nbits=nbits,
+                   fringe_order=order, looffset_hz=[7.5])
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={2: [1]})
+    for i, st in enumerate(sd.stations):
+        st.interpolator = st.interpolator + np.array([2e-5 * (i - 1), 0.0, 0.0])
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    ref = oracle_results(oracle, job, sd)
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    assert rel_err(got[:xend], ref[:xend]) < FP32_TOL
+    assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL
+    refs = oracle_spectra(oracle, job, sd, 1, 3, spec.nrecbands, spec.nchan)
+    for band in range(spec.nrecbands):
+        assert rel_err(g.spectrum(1, band, 3, spec.nchan), refs[band]) < FP32_TOL
+    g.close()
