@@ -33,6 +33,7 @@ sys.path.insert(0, os.path.join(REPO, "difx-svn2git-try3_amd"))
 METRIC = "input Gsamples/s, mixed-precision vs plain FP32 at ≤1e-3 vis err; p99 latency"
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3}
+FP32_VECTOR_PEAK_TFLOPS = 256 * 64 * 2 * 2.4e9 / 1e12     # 78.6: one FMA per lane per clock, unpacked
 
 
 def profiled_traffic(kernel: str):
@@ -558,6 +559,11 @@ def main():
                 "flops_per_launch": 5.0 * L * np.log2(L) * spec.nstations * spec.nrecbands * per_launch_blocks,
                 "avg_launch_ms": avg_s * 1e3}
         roof["fp32_tflops"] = roof["flops_per_launch"] / avg_s / 1e12
+        # the F stage is bound by FP32 vector issue, not HBM: its nominal FFT
+        # flops (5 L log2 L per block) against the unpacked FP32 vector peak
+        # (256 CUs x 64 lanes x 2 flops x 2.4 GHz; packed FP32 is disabled)
+        roof["fp32_vector_peak_tflops"] = FP32_VECTOR_PEAK_TFLOPS
+        roof["fp32_vector_frac"] = roof["fp32_tflops"] / FP32_VECTOR_PEAK_TFLOPS
     else:
         # algorithmic flops: Hermitian pairs incl. autos, 8 flops per complex MAC
         per_launch_blocks = info.chunk_blocks
