@@ -1150,15 +1150,15 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
     float2 v[16];
     // samples n1 + 16 (j + r M/16) lie WSTEP whole stream words apart at one
     // bit offset, and (16 M/16 being a multiple of the stride s) share one
-    // sub-rotator value, their step values sstep apart
+    // sub-rotator index, their step indices sstep apart
     constexpr int WSTEP = (16 * NBITS * NTF) / 32;
     const int n0 = n1 + 16 * j;
     const long long bq = bit0 + (long long)NBITS * n0;
     const long long wq = bq >> 5;
     const int sh = (int)(bq & 31);
-    const bool wordfast = fast && S.order == 1 && slog >= 0 && ((16 * NTF) & (s - 1)) == 0 &&
+    const bool wordfast = fast && (S.order == 1 || S.order == 2) && slog >= 0 && ((16 * NTF) & (s - 1)) == 0 &&
                           wq + 15LL * WSTEP < nwords;
-    if (wordfast) {
+    if (wordfast && S.order == 1) {
       const float2 r0 = rsub[n0 & (s - 1)];
       const float2* stp = rstep + (n0 >> slog);
       const int sstep = (16 * NTF) >> slog;
@@ -1169,6 +1169,25 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
         const float2 rot = cmulf(r0, *stp);     // sub[k] * step[j], mode.cpp:975-977
         stp += sstep;
         v[r] = make_float2(x * rot.x, x * rot.y);
+      }
+    } else if (wordfast) {
+      // piecewise quadratic rotator: piece_q[k] = sub[k] quad[k]^q, one
+      // product chain per thread carried from point to point (the same
+      // products, in the same order, as rebuilding it for each point)
+      const int k = n0 & (s - 1), q0 = n0 >> slog, sstep = (16 * NTF) >> slog;
+      const float2 qd = rquad[k];
+      float2 piece = rsub[k];
+      for (int i = 0; i < q0; i++) piece = cmulf(piece, qd);
+      const float2* stp = rstep + q0;
+      const unsigned* wp = stream + wq;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const float x = lut8s[(wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u)];
+        const float2 rot = cmulf(piece, *stp);
+        stp += sstep;
+        v[r] = make_float2(x * rot.x, x * rot.y);
+        if (r < 15)
+          for (int i = 0; i < sstep; i++) piece = cmulf(piece, qd);
       }
     } else {
 #pragma unroll
