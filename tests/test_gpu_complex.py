@@ -55,7 +55,7 @@ def test_complex_visibilities_fp32(gpu, oracle, lsb, dsb, nbits):
     g.close()
 
 
-@pytest.mark.parametrize("nchan", [128, 1024, 4096])
+@pytest.mark.parametrize("nchan", [128, 1024, 4096, 8192, 16384])
 def test_complex_fft_lengths_and_order2(gpu, oracle, nchan):
     lib, ctx = gpu
     spec = cspec(0, 1, 2, nchan=nchan, fringe_order=2, nstations=3, blockspersend=6)
