@@ -80,3 +80,21 @@ def test_complex_mixed_vs_fp64(gpu, oracle, prec, tol, lsb, dsb):
     ref = oracle_results(oracle, job, sd, fp64=True, nthreads=8)
     assert rel_err(got, ref) < tol, rel_err(got, ref)
     g.close()
+
+
+@pytest.mark.parametrize("nchan", [8192, 16384])
+def test_complex_8bit_largest(gpu, oracle, nchan):
+    """8-bit complex samples at the largest complex transforms; at 16384
+    channels the staged stream words share the tail of the FFT buffer."""
+    lib, ctx = gpu
+    spec = cspec(1, 0, 8, nchan=nchan, nstations=3, blockspersend=5)
+    job, keep = spec.build()
+    sd = cdata(spec, seed=12)
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    ref = oracle_results(oracle, job, sd, fp64=False, nthreads=8)
+    assert rel_err(got, ref) < FP32_TOL, rel_err(got, ref)
+    sp = oracle_spectra(oracle, job, sd, 2, 4, spec.nrecbands, spec.nchan)
+    for b in range(spec.nrecbands):
+        assert rel_err(g.spectrum(2, b, 4, spec.nchan), sp[b]) < FP32_TOL, b
+    g.close()
