@@ -548,3 +548,26 @@ def test_largest_fft_fringe_orders(gpu, oracle, order, nchan, nbits):
     for band in range(spec.nrecbands):
         assert rel_err(g.spectrum(1, band, 3, spec.nchan), refs[band]) < FP32_TOL
     g.close()
+
+
+@pytest.mark.parametrize("nchan,nbits", [(32768, 2), (16384, 8)])
+def test_largest_fft_fp8(gpu, oracle, nchan, nbits):
+    """fp8 e4m3 spectra written by the two-kernel transforms (per-(band,
+    channel) exponents applied in the second kernel), against the FP64
+    oracle, before and after calibrating the scales."""
+    lib, ctx = gpu
+    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=64.0, nchan=nchan, blockspersend=6, nbits=nbits,
+                   xmac="fp8")
+    job, keep = spec.build()
+    sd = make_data(spec, rho=0.1)
+    g = GpuJob(lib, ctx, job)
+    ref = oracle_results(oracle, job, sd, fp64=True, nthreads=8)
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    first = g.process(sd)
+    check(lib, lib.fxg_job_calibrate_scales(g.h), "calibrate")
+    second = g.process(sd)
+    for got in (first, second):
+        assert rel_err(got[:xend], ref[:xend]) < FP8_TOL, rel_err(got[:xend], ref[:xend])
+        assert rel_err(got[a0:a1], ref[a0:a1]) < FP8_TOL / 2, rel_err(got[a0:a1], ref[a0:a1])
+    g.close()
