@@ -75,3 +75,24 @@ def test_kurtosis_graph_replays_and_off(gpu, oracle):
     assert lib.fxg_job_get_kurtosis(g.h, 0, 0, 0, out.ctypes.data_as(C.POINTER(C.c_float)), 32,
                                     C.byref(n), C.byref(v)) != 0
     g.close()
+
+
+@pytest.mark.parametrize("xmac", ["fp32", "fp16"])
+def test_kurtosis_largest_fft(gpu, oracle, xmac):
+    """Kurtosis of the spectra the two-kernel 65536-point transform writes
+    (row-major layout, scratch sub-chunks), two records over six blocks."""
+    lib, ctx = gpu
+    spec = spec_of(xmac, nstations=2, nsubbands=1, bw_mhz=64.0, nchan=32768, blockspersend=6, max_chunk_blocks=0)
+    job, keep = spec.build()
+    sd = make_subint(spec, seed=6, rho=0.05)
+    g = GpuJob(lib, ctx, job)
+    check(lib, lib.fxg_job_set_kurtosis(g.h, 1, 64, 2), "set_kurtosis")
+    g.process(sd)
+    for ds in range(spec.nstations):
+        ref, rvalid = oracle_kurtosis(oracle, job, sd, ds, 2, 64, spec.nrecbands, spec.nchan)
+        for t in range(2):
+            for b in range(spec.nrecbands):
+                got, valid = g.kurtosis(t, ds, b, 64)
+                assert valid == rvalid[t], (ds, t)
+                np.testing.assert_allclose(got, ref[t, b], rtol=0, atol=TOL[xmac], err_msg=f"ds {ds} t {t} b {b}")
+    g.close()
