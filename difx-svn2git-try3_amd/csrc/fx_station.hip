@@ -885,13 +885,23 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
   constexpr int UB = 2 * NBITS;                 // bits per complex sample
   constexpr int RAWW = (L * UB) / 32 + 2;
-  __shared__ float2 buf[Sh::PAD];
-  __shared__ float2 tabs[Sh::TABS];
-  __shared__ float2 twlo[64], twhi[L / 64];
   constexpr bool kAliasRaw = NBITS == 8 && L >= 16384;
-  __shared__ unsigned raw_own[kAliasRaw ? 1 : RAWW];
-  unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : raw_own;
-  __shared__ float lut8s[NBITS == 8 ? 256 : 4];
+  // one LDS block with the small tables below the FFT buffer (immediate
+  // offsets reach them)
+  struct Smem {
+    float2 tabs[Sh::TABS];
+    float2 twlo[64], twhi[L / 64];
+    float lut8s[NBITS == 8 ? 256 : 4];
+    unsigned raw_own[kAliasRaw ? 2 : RAWW + (RAWW & 1)];
+    float2 buf[Sh::PAD];
+  };
+  __shared__ Smem sm;
+  float2 (&buf)[Sh::PAD] = sm.buf;
+  float2 (&tabs)[Sh::TABS] = sm.tabs;
+  float2 (&twlo)[64] = sm.twlo;
+  float2 (&twhi)[L / 64] = sm.twhi;
+  float (&lut8s)[NBITS == 8 ? 256 : 4] = sm.lut8s;
+  unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : sm.raw_own;
 
   const int tid = threadIdx.x;
   const int cb = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
