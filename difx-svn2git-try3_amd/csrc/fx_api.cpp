@@ -95,8 +95,27 @@ struct fxg_job {
   std::vector<long long> inoff_base;  // per XMAC freq: offset into d_inoff
   // tap bookkeeping: which chunk is resident in d_spec
   int tap_chunk0 = -1, tap_chunkn = 0;
-  // the subintegration whose frames / corner turn / geometry are resident
+  // the subintegration whose frames / corner turn / geometry are resident,
+  // with the digest of every frame as it was prepared and the block table
+  // (the per-block entry point re-prepares when a block's frames changed)
   std::vector<unsigned char> prepared_key;
+  std::vector<unsigned long long> prepared_digest;
+  std::vector<long long> prepared_fbase;
+  std::vector<DevBlock> prepared_blocks;
+  DBuf d_digest;
+  // pinned host staging of the per-subint tables (station table, valid
+  // flags): a ring, so that a copy still queued behind earlier work keeps
+  // its source until the copy has run
+  struct HostStage {
+    DevStationSub* ss = nullptr;
+    int* flags = nullptr;
+    size_t ss_cap = 0, flags_cap = 0;
+    hipEvent_t done = nullptr;
+    bool queued = false;
+  };
+  static constexpr int kStages = 4;
+  HostStage stage[kStages];
+  int stage_next = 0;
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
   DBuf d_zero;                    // 64 zero bytes: the rows past the data for streaming loads
   int xmac_streaming = 1;         // direct-to-LDS MFMA XMAC where the layout allows
@@ -404,6 +423,13 @@ static void job_free(fxg_job* j) {
                   &j->d_eoff, &j->d_kurt, &j->d_kurtout, &j->d_kurtnz};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
+  j->d_digest.release();
+  for (auto& hs : j->stage) {
+    if (hs.done) (void)hipEventSynchronize(hs.done), (void)hipEventDestroy(hs.done);
+    if (hs.ss) (void)hipHostFree(hs.ss);
+    if (hs.flags) (void)hipHostFree(hs.flags);
+    hs = fxg_job::HostStage{};
+  }
   for (hipEvent_t e : j->evpool) hipEventDestroy(e);
   j->evpool.clear();
   for (auto& p : j->pending) hipEventDestroy(p.second);
@@ -791,8 +817,30 @@ static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stati
       (rc = j->d_ftab.ensure(sizeof(int2) * (size_t)std::max<long long>(1, fbase))) ||
       (rc = j->d_ct.ensure((size_t)std::max<long long>(4, ctbytes))))
     return rc;
-  HIPCHK(hipMemcpyAsync(j->d_ss.p, ss.data(), sizeof(DevStationSub) * ss.size(), hipMemcpyHostToDevice, s));
-  if (!flags.empty()) HIPCHK(hipMemcpyAsync(j->d_flags.p, flags.data(), sizeof(int) * flags.size(), hipMemcpyHostToDevice, s));
+  // the tables go through the next pinned staging slot; its previous copy
+  // must have run before the slot is refilled
+  fxg_job::HostStage& hs = j->stage[j->stage_next];
+  j->stage_next = (j->stage_next + 1) % fxg_job::kStages;
+  if (!hs.done) HIPCHK(hipEventCreateWithFlags(&hs.done, hipEventDisableTiming));
+  if (hs.queued) { HIPCHK(hipEventSynchronize(hs.done)); hs.queued = false; }
+  if (hs.ss_cap < ss.size()) {
+    if (hs.ss) { HIPCHK(hipHostFree(hs.ss)); hs.ss = nullptr; hs.ss_cap = 0; }
+    if (hipHostMalloc((void**)&hs.ss, sizeof(DevStationSub) * ss.size()) != hipSuccess)
+      return set_err(FXG_ENOMEM, "pinned staging of the station table");
+    hs.ss_cap = ss.size();
+  }
+  if (hs.flags_cap < flags.size()) {
+    if (hs.flags) { HIPCHK(hipHostFree(hs.flags)); hs.flags = nullptr; hs.flags_cap = 0; }
+    if (hipHostMalloc((void**)&hs.flags, sizeof(int) * flags.size()) != hipSuccess)
+      return set_err(FXG_ENOMEM, "pinned staging of the valid flags");
+    hs.flags_cap = flags.size();
+  }
+  std::memcpy(hs.ss, ss.data(), sizeof(DevStationSub) * ss.size());
+  if (!flags.empty()) std::memcpy(hs.flags, flags.data(), sizeof(int) * flags.size());
+  HIPCHK(hipMemcpyAsync(j->d_ss.p, hs.ss, sizeof(DevStationSub) * ss.size(), hipMemcpyHostToDevice, s));
+  if (!flags.empty()) HIPCHK(hipMemcpyAsync(j->d_flags.p, hs.flags, sizeof(int) * flags.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(hs.done, s));
+  hs.queued = true;
   sh->maxframes = maxframes;
   sh->maxsamples = maxsamples;
   sh->maxdatabytes = maxdatabytes;
@@ -1341,6 +1389,46 @@ int fxg_batch_process(fxg_batch* b, const fxg_subint_desc* subs, const fxg_stati
   return 0;
 }
 
+// digests of frames [f0, f0 + nf) of one datastream's input as the caller
+// holds it now (host bytes, or device bytes through k_frame_digest)
+static int frame_digests(fxg_job* j, const fxg_station_input& in, int data_on_device, int framebytes,
+                         long long f0, long long nf, unsigned long long* out) {
+  if (nf <= 0) return 0;
+  if (!data_on_device) {
+    for (long long f = 0; f < nf; f++) out[f] = host_frame_digest(in.data + (f0 + f) * framebytes, framebytes);
+    return 0;
+  }
+  int rc;
+  if ((rc = j->d_digest.ensure(sizeof(unsigned long long) * (size_t)nf))) return rc;
+  HIPCHK(launch_frame_digest(in.data, f0, nf, framebytes, j->d_digest.as<unsigned long long>(), j->stream));
+  HIPCHK(hipMemcpyAsync(out, j->d_digest.p, sizeof(unsigned long long) * (size_t)nf, hipMemcpyDeviceToHost,
+                        j->stream));
+  HIPCHK(hipStreamSynchronize(j->stream));
+  return 0;
+}
+
+// Whether the frames that block `block` of datastream ds reads (one frame of
+// margin either side of its sample window) still hold the bytes they held
+// when the subintegration was prepared.
+static int block_frames_unchanged(fxg_job* j, const fxg_station_input* st, int data_on_device, int ds, int block,
+                                  bool* same) {
+  const Plan& P = j->plan;
+  const DevStation& S = P.st[ds];
+  const DevBlock& g = j->prepared_blocks[(size_t)ds * j->desc.blockspersend + block];
+  const long long nfr = st[ds].nbytes / S.framebytes;
+  const long long t0 = g.nearestsample;
+  const long long fa = std::max(0LL, (t0 >= 0 ? t0 / S.framesamples : -1) - 1);
+  const long long fb = std::min(nfr - 1, (t0 + S.L) / std::max(1, S.framesamples) + 1);
+  *same = true;
+  if (fb < fa) return 0;
+  std::vector<unsigned long long> now((size_t)(fb - fa + 1));
+  int rc;
+  if ((rc = frame_digests(j, st[ds], data_on_device, S.framebytes, fa, fb - fa + 1, now.data()))) return rc;
+  for (long long f = fa; f <= fb; f++)
+    if (now[(size_t)(f - fa)] != j->prepared_digest[(size_t)(j->prepared_fbase[ds] + f)]) { *same = false; break; }
+  return 0;
+}
+
 int fxg_station_process_block(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
                               int data_on_device, int ds, int block, float* spectra, fxg_block_info* info) {
   if (!j || !sub || !st || !spectra) return set_err(FXG_EINVAL, "null argument");
@@ -1349,8 +1437,26 @@ int fxg_station_process_block(fxg_job* j, const fxg_subint_desc* sub, const fxg_
   HIPCHK(hipSetDevice(j->ctx->device));
   int rc;
   std::vector<unsigned char> key = subint_key(j, sub, st, data_on_device);
-  if (key != j->prepared_key) {
-    if ((rc = prepare_subint(j, sub, st, data_on_device))) { j->prepared_key.clear(); return rc; }
+  bool same = key == j->prepared_key;
+  // the same description can hold new bytes (a refilled buffer): compare
+  // the digests of the frames this block reads with their prepared ones
+  if (same && (rc = block_frames_unchanged(j, st, data_on_device, ds, block, &same))) return rc;
+  if (!same) {
+    j->prepared_key.clear();
+    if ((rc = prepare_subint(j, sub, st, data_on_device))) return rc;
+    const size_t nblk = (size_t)P.nds * j->desc.blockspersend;
+    j->prepared_blocks.resize(nblk);
+    HIPCHK(hipMemcpyAsync(j->prepared_blocks.data(), j->d_blocks.p, sizeof(DevBlock) * nblk, hipMemcpyDeviceToHost,
+                          j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    j->prepared_fbase.assign(P.nds, 0);
+    long long nf = 0;
+    for (int d = 0; d < P.nds; d++) { j->prepared_fbase[d] = nf; nf += st[d].nbytes / P.st[d].framebytes; }
+    j->prepared_digest.assign((size_t)nf, 0ull);
+    for (int d = 0; d < P.nds; d++)
+      if ((rc = frame_digests(j, st[d], data_on_device, P.st[d].framebytes, 0, st[d].nbytes / P.st[d].framebytes,
+                              j->prepared_digest.data() + j->prepared_fbase[d])))
+        return rc;
     j->prepared_key = key;
   }
   FStageArgs fa;

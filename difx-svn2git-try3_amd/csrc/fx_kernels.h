@@ -115,6 +115,11 @@ struct MuxArgs {
 };
 hipError_t launch_vdifmux(const MuxArgs& a, hipStream_t stream);
 
+// digests of frames [f0, f0 + nf) of a frame stream (device bytes) into out[nf],
+// and the same digest of n host bytes
+hipError_t launch_frame_digest(const unsigned char* data, long long f0, long long nf, int framebytes,
+                               unsigned long long* out, hipStream_t stream);
+unsigned long long host_frame_digest(const unsigned char* p, int n);
 hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
                          long long maxframes, hipStream_t stream);
 hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsigned char* ct, int nst,

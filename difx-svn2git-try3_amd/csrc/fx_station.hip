@@ -1357,6 +1357,41 @@ template __global__ void k_rotab<65536>(FStageArgs);
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// k_frame_digest: a 64-bit digest of each frame's bytes (FNV-1a over the
+// bytes taken four at a time, little-endian), one thread per frame, so that
+// a caller of the per-block entry point can tell whether the frames a block
+// reads changed since they were prepared.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline unsigned long long frame_digest_bytes(const unsigned char* p, int n) {
+  unsigned long long h = 0xcbf29ce484222325ull;
+  int i = 0;
+  for (; i + 4 <= n; i += 4) {
+    const unsigned w = (unsigned)p[i] | ((unsigned)p[i + 1] << 8) | ((unsigned)p[i + 2] << 16) | ((unsigned)p[i + 3] << 24);
+    h = (h ^ w) * 0x100000001b3ull;
+  }
+  for (; i < n; i++) h = (h ^ p[i]) * 0x100000001b3ull;
+  return h;
+}
+
+__global__ __launch_bounds__(256) void k_frame_digest(const unsigned char* __restrict__ data, long long f0,
+                                                      long long nf, int framebytes,
+                                                      unsigned long long* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  out[i] = frame_digest_bytes(data + (f0 + i) * (long long)framebytes, framebytes);
+}
+
+hipError_t launch_frame_digest(const unsigned char* data, long long f0, long long nf, int framebytes,
+                               unsigned long long* out, hipStream_t stream) {
+  if (nf <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_frame_digest, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, stream, data, f0, nf,
+                     framebytes, out);
+  return hipGetLastError();
+}
+
+unsigned long long host_frame_digest(const unsigned char* p, int n) { return frame_digest_bytes(p, n); }
+
 hipError_t launch_frames(const DevStation* st, const DevStationSub* ss, int2* ftab, int nst,
                          long long maxframes, hipStream_t stream) {
   if (maxframes <= 0) return hipSuccess;

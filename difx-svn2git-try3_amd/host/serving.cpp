@@ -29,6 +29,7 @@ struct Request {
   Clock::time_point arrival, done;
   int rc = 0;
   bool finished = false;
+  bool waited = false;                       // a thread is in fxs_wait for it
 };
 
 struct Config {
@@ -39,6 +40,7 @@ struct Config {
   int max_slots = 1;
   int64_t slot_bytes = 0;
   int nslots = 0, busy = 0;
+  int creating = 0;                          // slots reserved, job being created
   std::deque<Request*> queue;
 };
 
@@ -87,25 +89,30 @@ void slot_loop(fxs_server* s, Config* c, fxg_job* job) {
   }
 }
 
-// a new slot for c when its requests outnumber its idle slots and the
-// budget allows (caller holds s->m)
-int grow(fxs_server* s, Config* c) {
-  const int idle = c->nslots - c->busy;
-  if ((int)c->queue.size() <= idle || c->nslots >= c->max_slots) return 0;
+// Whether c should get another slot: its requests outnumber its idle and
+// coming slots and the budget allows.  A yes reserves the slot's footprint;
+// the caller creates the job without the lock and then publishes it (or
+// gives the reservation back).  Caller holds s->m.
+bool reserve_slot(fxs_server* s, Config* c) {
+  const int idle = c->nslots + c->creating - c->busy;
+  if ((int)c->queue.size() <= idle || c->nslots + c->creating >= c->max_slots) return false;
   if (s->used + c->slot_bytes > s->budget) {
     s->st.refused++;
-    return 0;
+    return false;
   }
-  fxg_job* job = nullptr;
-  int rc = fxg_job_create(s->ctx, &c->desc, &job);
-  if (rc) return rc;
-  s->jobs.push_back(job);
   s->used += c->slot_bytes;
+  s->st.used_bytes = s->used;
+  c->creating++;
+  return true;
+}
+
+// the reserved slot as a running worker (caller holds s->m)
+void publish_slot(fxs_server* s, Config* c, fxg_job* job) {
+  c->creating--;
+  s->jobs.push_back(job);
   c->nslots++;
   s->st.nslots++;
-  s->st.used_bytes = s->used;
   s->workers.emplace_back(slot_loop, s, c, job);
-  return 0;
 }
 
 }  // namespace
@@ -143,7 +150,7 @@ int fxs_destroy(fxs_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  for (std::thread& t : s->workers) t.join();
+  for (std::thread& t : s->workers) t.join();   // (workers drain their queues before they stop)
   for (fxg_job* j : s->jobs) fxg_job_destroy(j);
   for (auto& kv : s->reqs) delete kv.second;
   fxg_ctx_destroy(s->ctx);
@@ -197,19 +204,59 @@ int fxs_submit(fxs_server* s, int config_id, const fxg_subint_desc* sub, const f
   c->queue.push_back(r);
   s->st.submitted++;
   s->st.queued++;
-  int rc = grow(s, c);
-  if (c->nslots == 0) {                  // nothing can ever run it
+  const bool reserved = reserve_slot(s, c);
+  if (!reserved && c->nslots + c->creating == 0) {   // nothing can ever run it
     c->queue.pop_back();
     s->reqs.erase(r->id);
     s->st.submitted--;
     s->st.queued--;
     delete r;
-    return rc ? rc : FXG_ENOMEM;
+    return FXG_ENOMEM;
   }
-  *request = r->id;                      // queued; a refused extra slot only delays it
+  if (!reserved) {
+    *request = r->id;                    // queued; a refused extra slot only delays it
+    lk.unlock();
+    s->cv_work.notify_all();
+    return 0;
+  }
+  // the new slot's job (device allocations, table uploads) is created
+  // without the server lock, so that other slots keep completing
   lk.unlock();
-  s->cv_work.notify_all();
-  return 0;
+  fxg_job* job = nullptr;
+  const int rc = fxg_job_create(s->ctx, &c->desc, &job);
+  lk.lock();
+  if (!rc) {
+    publish_slot(s, c, job);
+    *request = r->id;
+    lk.unlock();
+    s->cv_work.notify_all();
+    return 0;
+  }
+  c->creating--;
+  s->used -= c->slot_bytes;
+  s->st.used_bytes = s->used;
+  if (c->nslots + c->creating > 0) {     // another slot will take it
+    *request = r->id;
+    lk.unlock();
+    s->cv_work.notify_all();
+    return 0;
+  }
+  // no slot of this configuration exists: every request queued for it
+  // fails with the creation error (this one is withdrawn)
+  for (Request* q : c->queue) {
+    s->st.queued--;
+    if (q == r) continue;
+    q->rc = rc;
+    q->done = Clock::now();
+    q->finished = true;
+  }
+  c->queue.clear();
+  s->reqs.erase(r->id);
+  s->st.submitted--;
+  delete r;
+  lk.unlock();
+  s->cv_done.notify_all();
+  return rc;
 }
 
 int fxs_wait(fxs_server* s, int64_t request, int64_t* latency_ns) {
@@ -218,6 +265,8 @@ int fxs_wait(fxs_server* s, int64_t request, int64_t* latency_ns) {
   auto it = s->reqs.find(request);
   if (it == s->reqs.end()) return FXG_EINVAL;
   Request* r = it->second;
+  if (r->waited) return FXG_ESTATE;     // one waiter per request: it frees the request
+  r->waited = true;
   s->cv_done.wait(lk, [&] { return r->finished; });
   if (latency_ns)
     *latency_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(r->done - r->arrival).count();

@@ -300,7 +300,10 @@ int fxg_subint_wait(fxg_job* job, int64_t ticket);
  * host pointer, band-major) and the block's geometry and data weight.  The
  * frame validity, corner turn and FP64 geometry of the whole subintegration
  * are computed on the first call and reused while sub/stations describe the
- * same subintegration (same pointers, lengths, times and delay models). */
+ * same subintegration (same pointers, lengths, times, delay models and valid
+ * flags) and the frames the block reads hold the bytes they held then (a
+ * 64-bit digest of each frame is compared; a refilled buffer is prepared
+ * again). */
 int fxg_station_process_block(fxg_job* job, const fxg_subint_desc* sub,
                               const fxg_station_input* stations, int data_on_device,
                               int datastream, int block, float* spectra,
