@@ -55,6 +55,40 @@ struct DBuf {
 
 }  // namespace
 
+// Device scratch of fxg_unpack_noalloc: [0, 256) the station constants,
+// [256, 512) the station's subint entry, [512, kScratchBytes) the frame
+// table (int2 per frame).  Each device holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// gives for that device (scratch_ptr), resolved once per device when a
+// context is created on it.
+constexpr long long kScratchBytes = 4LL << 20;
+constexpr long long kScratchFrames = (kScratchBytes - 512) / (long long)sizeof(int2);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device's address of scratch (device current); 0 or an FXG_E* code
+int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return set_err(FXG_ENODEV, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_err(FXG_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return 0;
+}
+
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 struct fxg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;   // for standalone calls
@@ -288,6 +322,8 @@ int fxg_ctx_create(int device, fxg_ctx** out) {
   if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
     return set_err(FXG_ENODEV, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
   HIPCHK(hipSetDevice(device));
+  int src = resolve_scratch(device);
+  if (src) return src;
   fxg_ctx* c = new fxg_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -350,11 +386,11 @@ int fxg_memcpy_d2h(fxg_ctx* c, void* dst, const void* src, int64_t bytes) {
 // ---------------------------------------------------------------------------
 // standalone unpack: frames -> floats for one datastream
 // ---------------------------------------------------------------------------
-int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_t nbytes,
-               int64_t sampleoffset, int32_t nsamp, float* out, float* weight) {
-  if (!c || !ds || !data || !out || nsamp <= 0 || sampleoffset < 0 || nbytes < 0)
-    return set_err(FXG_EINVAL, "bad args");
-  // a one-station, one-freq job description gives the decoder constants
+}  // extern "C"
+
+// The decoder constants of a standalone unpack: a one-station, one-freq job
+// description of the datastream (fxg_unpack and fxg_unpack_noalloc).
+static int unpack_station(const fxg_datastream* ds, DevStation* S) {
   fxg_freq fq{};
   fq.freq_mhz = 1000.0; fq.bw_mhz = 16.0; fq.nchan = 64; fq.chans_to_avg = 1;
   fxg_datastream d = *ds;
@@ -365,8 +401,57 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
   Plan p;
   int rc = build_plan(&jd, &p, &g_err);
   if (rc) return rc;
-  DevStation S = p.st[0];
-  if (S.cplx) return set_err(FXG_EUNSUPPORTED, "fxg_unpack of complex samples (fxg_tap_spectrum gives their spectra)");
+  *S = p.st[0];
+  if (S->cplx) return set_err(FXG_EUNSUPPORTED, "fxg_unpack of complex samples (fxg_tap_spectrum gives their spectra)");
+  return 0;
+}
+
+// Mk5Mode::unpack's window [ns, ns + nsamp) over the frame table: the first
+// sample that decodes as zero for 8-bit streams that stop at a blank frame
+// (validend), and the count of good samples.  frame(f) gives frame f's valid
+// payload byte range (f < nframes).
+template <typename FrameFn>
+static void unpack_window(const DevStation& S, long long nframes, long long ns, int nsamp, FrameFn frame,
+                          long long* validend_out, long long* good_out) {
+  const long long tend = ns + nsamp;
+  const long long unpackstart = ns - ns % S.granularity;
+  long long validend = 0x7fffffffffffffffLL;
+  if (S.quirk8) {
+    for (long long f = unpackstart / S.framesamples; f <= (tend - 1) / S.framesamples; f++) {
+      bool fv = false;
+      if (f < nframes) { const int2 z = frame(f); fv = z.x < z.y; }
+      if (!fv) { validend = std::max(unpackstart, f * (long long)S.framesamples); break; }
+    }
+  }
+  long long good = 0;
+  for (long long t = ns; t < std::min(tend, validend);) {
+    long long f = t / S.framesamples;
+    long long fe = std::min(std::min(tend, validend), (f + 1) * (long long)S.framesamples);
+    if (f < nframes) {
+      const int2 z = frame(f);
+      if (z.x < z.y) {
+        long long base = f * (long long)S.framesamples;
+        long long g0 = ((long long)z.x + S.groupbytes - 1) / S.groupbytes;
+        long long g1 = ((long long)z.y + S.groupbytes - 1) / S.groupbytes;
+        long long a = std::max(t - base, g0 * S.groupsamples), b = std::min(fe - base, g1 * S.groupsamples);
+        if (b > a) good += b - a;
+      }
+    }
+    t = fe;
+  }
+  *validend_out = validend;
+  *good_out = good;
+}
+
+extern "C" {
+
+int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_t nbytes,
+               int64_t sampleoffset, int32_t nsamp, float* out, float* weight) {
+  if (!c || !ds || !data || !out || nsamp <= 0 || sampleoffset < 0 || nbytes < 0)
+    return set_err(FXG_EINVAL, "bad args");
+  DevStation S;
+  int rc = unpack_station(ds, &S);
+  if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
   DBuf d_st, d_ss, d_ftab;
   if ((rc = d_st.ensure(sizeof(DevStation))) || (rc = d_ss.ensure(sizeof(DevStationSub)))) return rc;
@@ -380,33 +465,58 @@ int fxg_unpack(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_
   // the window's validity is computed on the host from the frame table
   std::vector<int2> ftab((size_t)std::max<long long>(1, Q.nframes));
   if (Q.nframes) HIPCHK(hipMemcpy(ftab.data(), d_ftab.p, sizeof(int2) * Q.nframes, hipMemcpyDeviceToHost));
-  long long ns = sampleoffset, tend = sampleoffset + nsamp;
-  long long unpackstart = ns - ns % S.granularity;
-  long long validend = 0x7fffffffffffffffLL;
-  if (S.quirk8) {
-    for (long long f = unpackstart / S.framesamples; f <= (tend - 1) / S.framesamples; f++) {
-      bool fv = f < Q.nframes && ftab[f].x < ftab[f].y;
-      if (!fv) { validend = std::max(unpackstart, f * (long long)S.framesamples); break; }
-    }
-  }
-  long long good = 0;
-  for (long long t = ns; t < std::min(tend, validend);) {
-    long long f = t / S.framesamples;
-    long long fe = std::min(std::min(tend, validend), (f + 1) * (long long)S.framesamples);
-    if (f < Q.nframes) {
-      int2 z = ftab[f];
-      if (z.x < z.y) {
-        long long base = f * (long long)S.framesamples;
-        long long g0 = ((long long)z.x + S.groupbytes - 1) / S.groupbytes;
-        long long g1 = ((long long)z.y + S.groupbytes - 1) / S.groupbytes;
-        long long a = std::max(t - base, g0 * S.groupsamples), b = std::min(fe - base, g1 * S.groupsamples);
-        if (b > a) good += b - a;
-      }
-    }
-    t = fe;
-  }
+  long long validend = 0, good = 0;
+  unpack_window(S, Q.nframes, sampleoffset, nsamp, [&](long long f) { return ftab[(size_t)f]; }, &validend, &good);
   HIPCHK(launch_unpack(d_st.as<DevStation>(), d_ss.as<DevStationSub>(), d_ftab.as<int2>(), c->lut8.as<float>(),
-                       ns, nsamp, S.nchan, validend, out, c->stream));
+                       sampleoffset, nsamp, S.nchan, validend, out, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (weight) *weight = (float)good / (float)nsamp;
+  return 0;
+}
+
+int fxg_unpack_noalloc(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data, int64_t nbytes,
+                       int64_t sampleoffset, int32_t nsamp, float* out, float* weight) {
+  if (!c || !ds || !data || !out || nsamp <= 0 || sampleoffset < 0 || nbytes < 0)
+    return set_err(FXG_EINVAL, "bad args");
+  DevStation S;
+  int rc = unpack_station(ds, &S);
+  if (rc) return rc;
+  DevStationSub Q{};
+  Q.data = data; Q.nbytes = nbytes; Q.nframes = nbytes / S.framebytes; Q.fbase = 0;
+  if (Q.nframes > kScratchFrames)
+    return set_err(FXG_EINVAL, "fxg_unpack_noalloc: " + std::to_string(Q.nframes) + " frames exceed the scratch's " +
+                                   std::to_string(kScratchFrames) + " (use fxg_unpack)");
+  HIPCHK(hipSetDevice(c->device));
+  unsigned char* base = (unsigned char*)scratch_ptr(c->device);
+  if (!base) return set_err(FXG_ESTATE, "scratch address of the device not resolved");
+  DevStation* d_st = (DevStation*)base;
+  DevStationSub* d_ss = (DevStationSub*)(base + 256);
+  int2* d_ftab = (int2*)(base + 512);
+  static_assert(sizeof(DevStation) <= 256 && sizeof(DevStationSub) <= 256, "scratch header layout");
+  // (S and Q outlive the copies: the stream is synchronised below)
+  HIPCHK(hipMemcpyAsync(d_st, &S, sizeof(S), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ss, &Q, sizeof(Q), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_frames(d_st, d_ss, d_ftab, 1, Q.nframes, c->stream));
+  // the frame table entries the window reads, fetched in pieces into a
+  // fixed host buffer (k_frames wrote every entry below Q.nframes)
+  int2 piece[512];
+  long long piece0 = -1;
+  int err = 0;
+  auto frame = [&](long long f) -> int2 {
+    if (err) return make_int2(kBig, 0);
+    if (piece0 < 0 || f < piece0 || f >= piece0 + 512) {
+      piece0 = f;
+      const long long n = std::min<long long>(512, Q.nframes - f);
+      if (hipMemcpyAsync(piece, d_ftab + f, sizeof(int2) * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess) { err = 1; return make_int2(kBig, 0); }
+    }
+    return piece[f - piece0];
+  };
+  long long validend = 0, good = 0;
+  unpack_window(S, Q.nframes, sampleoffset, nsamp, frame, &validend, &good);
+  if (err) return set_err(FXG_EHIP, "frame table read-back failed");
+  HIPCHK(launch_unpack(d_st, d_ss, d_ftab, c->lut8.as<float>(), sampleoffset, nsamp, S.nchan, validend, out,
+                       c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (weight) *weight = (float)good / (float)nsamp;
   return 0;

@@ -363,6 +363,8 @@ def load_gpu_lib():
     lib.fxg_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
     lib.fxg_unpack.argtypes = [C.c_void_p, P(Datastream), C.c_void_p, C.c_int64, C.c_int64,
                                C.c_int32, C.c_void_p, P(C.c_float)]
+    lib.fxg_unpack_noalloc.argtypes = [C.c_void_p, P(Datastream), C.c_void_p, C.c_int64, C.c_int64,
+                               C.c_int32, C.c_void_p, P(C.c_float)]
     lib.fxg_job_create.argtypes = [C.c_void_p, P(JobDesc), P(C.c_void_p)]
     lib.fxg_job_destroy.argtypes = [C.c_void_p]
     lib.fxg_job_plan_info.argtypes = [C.c_void_p, P(PlanInfo)]

@@ -265,6 +265,18 @@ int fxg_memcpy_d2h(fxg_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int fxg_unpack(fxg_ctx* ctx, const fxg_datastream* ds, const uint8_t* data,
                int64_t nbytes, int64_t sampleoffset, int32_t nsamp,
                float* out, float* weight);
+/* fxg_unpack without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as fxg_unpack; its frame table and station tables live in a
+ * static 4 MiB device scratch held by each device until the process ends, so
+ * the stream may hold at most (4 MiB - 512) / 8 = 524224 whole frames
+ * (FXG_EINVAL beyond).  Every call on one device shares that scratch: calls
+ * that overlap on one device (several threads, or several contexts on the
+ * same device) race and must be serialised by the caller. */
+int fxg_unpack_noalloc(fxg_ctx* ctx, const fxg_datastream* ds, const uint8_t* data,
+                       int64_t nbytes, int64_t sampleoffset, int32_t nsamp,
+                       float* out, float* weight);
 
 /* ---- jobs ----------------------------------------------------------------- */
 int fxg_job_create(fxg_ctx* ctx, const fxg_job_desc* desc, fxg_job** job);
