@@ -36,6 +36,23 @@ PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0, "fp32": 157.3}
 FP32_VECTOR_PEAK_TFLOPS = 256 * 64 * 2 * 2.4e9 / 1e12     # 78.6: one FMA per lane per clock, unpacked
 
 
+# CPU threads of the baseline: the host share of one GPU on the benchmark
+# machines (16), or fewer where the host has fewer cores
+CPU_THREADS = max(1, min(16, os.cpu_count() or 1))
+FP32_PEAK_TFLOPS = 157.3                # MI355X published FP32 vector peak (packed issue)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def profiled_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of the 32-station workload
@@ -202,6 +219,105 @@ def vis_errors(lib, ctx, orc, seed, blocks, threads, torch):
     return out
 
 
+def roofline(spec, info, stage, launch, precision):
+    """Roofline of the step's dominant kernel from the profiled step's HIP
+    event times: algorithmic bytes and flops per launch against HBM and the
+    FP32 vector (F stage) or matrix-core (XMAC) peak; "bound" names the
+    resource with the larger fraction."""
+    C_, L = spec.nchan, spec.fftchannels
+    nin = spec.nstations * spec.npol
+    specbytes = 8 if precision == "fp32" else 4
+    dom = max(("fstage", "xmac"), key=lambda k: stage.get(k, {"ms": 0})["ms"])
+    nblk = info.chunk_blocks
+    samples = spec.nstations * spec.nrecbands * nblk * L
+    if dom == "fstage":
+        avg_s = stage["fstage"]["ms"] / stage["fstage"]["launches"] / 1e3
+        kname = f"fxg::k_fstage<{L}, 2>"
+        # 2-bit input read once + spectra written once
+        nbytes = samples * 0.25 + spec.nstations * spec.nrecbands * nblk * C_ * specbytes
+        # FFT 5 log2 L, fringe rotation 6, fractional-sample rotation 3 flops per sample
+        flops = samples * (5.0 * np.log2(L) + 9.0)
+        hbm_frac = nbytes / avg_s / 1e9 / HBM_PEAK_GBS
+        valu_tf = flops / avg_s / 1e12
+        roof = {"kernel": kname, "avg_launch_ms": avg_s * 1e3, "traffic": profiled_traffic(kname),
+                "algorithmic_bytes_per_launch": nbytes, "flops_per_launch": flops,
+                "hbm_gbs": nbytes / avg_s / 1e9, "hbm_frac": hbm_frac,
+                "fp32_tflops": valu_tf, "fp32_frac": valu_tf / FP32_PEAK_TFLOPS,
+                "fp32_unpacked_peak_tflops": FP32_VECTOR_PEAK_TFLOPS,
+                "fp32_unpacked_frac": valu_tf / FP32_VECTOR_PEAK_TFLOPS}
+        if valu_tf / FP32_PEAK_TFLOPS >= hbm_frac:
+            roof.update(bound="valu", achieved=valu_tf, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                        frac=valu_tf / FP32_PEAK_TFLOPS)
+        else:
+            roof.update(bound="hbm", achieved=nbytes / avg_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s", frac=hbm_frac)
+        return roof
+    # XMAC: Hermitian pairs incl. autos, 8 flops per complex MAC; spectra read once
+    avg_s = stage["xmac"]["ms"] / stage["xmac"]["launches"] / 1e3
+    nfreq = stage["xmac"]["launches"]
+    flops = 8.0 * nin * (nin + 1) / 2 * C_ * nblk * spec.nsubbands / nfreq
+    nbytes = nin * C_ * nblk * specbytes * spec.nsubbands / nfreq
+    peak = PEAK_TFLOPS[precision]
+    bf = "true" if precision == "bf16" else "false"
+    kname = (f"fxg::k_xmac_mfma_dl<2, {bf}>" if launch["schedule"]["xmac_streaming"]
+             else f"fxg::k_xmac_mfma<2, {bf}, 16>")
+    tf = flops / avg_s / 1e12
+    hbm_frac = nbytes / avg_s / 1e9 / HBM_PEAK_GBS
+    roof = {"kernel": kname, "avg_launch_ms": avg_s * 1e3, "traffic": profiled_traffic(kname),
+            "flops_per_launch": flops, "algorithmic_bytes_per_launch": nbytes, "tflops": tf, "mfma_frac": tf / peak,
+            "hbm_frac": hbm_frac}
+    if tf / peak >= hbm_frac:
+        roof.update(bound="mfma", achieved=tf, peak=peak, unit="TFLOP/s", frac=tf / peak)
+    else:
+        roof.update(bound="hbm", achieved=nbytes / avg_s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s", frac=hbm_frac)
+    return roof
+
+
+def small_config_cpu_baselines(orc, lib, ctx, torch):
+    """The restated reference CPU path on the two small configurations:
+    4 stations x 2 pol x 16 MHz, 256 channels, eight 1 s subintegrations in
+    full; and the first 4096 FFT blocks (131 ms) of a 1 s subintegration of
+    10 stations x 2 pol x 8 x 32 MHz, 1024 channels.  CPU_THREADS threads,
+    each a contiguous block range (core.cpp:2091-2095)."""
+    from fxcorr import JobSpec
+    from fxsynth import make_subint_device
+    out = {}
+    rng = np.random.default_rng(11)
+    spec = JobSpec(nstations=4, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, blockspersend=62500, crosspol=True,
+                   xmac="fp32")
+    tot_el, tot_ns = 0.0, 0
+    for k in range(8):
+        delays = [int(v) for v in rng.uniform(-50.0, 50.0, spec.nstations) * 32]     # us -> samples at 32 Ms/s
+        sd, raws = make_subint_device(spec, seed=100 + k, rho=0.05, delay_samples=delays,
+                                      rates_us_per_s=rng.uniform(-1.0, 1.0, spec.nstations), sub_sec=10 + k)
+        for s_, r in zip(sd.stations, raws):
+            s_.raw = r.cpu().numpy()
+        del raws
+        v, el, ns = cpu_baseline(orc, sd, spec, CPU_THREADS, spec.blockspersend)
+        tot_el += el
+        tot_ns += ns
+    out["4st_256ch_8s"] = {"value": tot_ns / tot_el / 1e9, "unit": "Gsamples/s", "cores": CPU_THREADS,
+                           "wall_s": tot_el, "gsamples": tot_ns / 1e9,
+                           "sample": "eight 1 s subintegrations (62500 blocks of 512 each), in full"}
+    log(f"  CPU 4-station 8 s: {out['4st_256ch_8s']['value']:.4f} Gsamples/s in {tot_el:.1f} s")
+    spec = JobSpec(nstations=10, nsubbands=8, npol=2, bw_mhz=32.0, nchan=1024, blockspersend=4096, crosspol=True,
+                   xmac="fp32")
+    delays = [int(v) for v in rng.uniform(-10000.0, 10000.0, spec.nstations) * 64]    # +-10 ms at 64 Ms/s
+    sd, raws = make_subint_device(spec, seed=200, rho=0.02, delay_samples=delays,
+                                  rates_us_per_s=rng.uniform(-3.0, 3.0, spec.nstations))
+    for s_, r in zip(sd.stations, raws):
+        s_.raw = r.cpu().numpy()
+    del raws
+    v, el, ns = cpu_baseline(orc, sd, spec, CPU_THREADS, spec.blockspersend)
+    out["10st_1024ch_slice"] = {"value": v, "unit": "Gsamples/s", "cores": CPU_THREADS, "wall_s": el,
+                                "gsamples": ns / 1e9,
+                                "sample": "first 4096 FFT blocks of 2048 (131 ms) of a 1 s subintegration"}
+    log(f"  CPU 10-station slice: {v:.4f} Gsamples/s in {el:.1f} s")
+    out["cpu_model"] = cpu_model()
+    out["nproc"] = os.cpu_count()
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(orc, sd_host, spec, threads, nblocks):
     from fxcorr import PlanInfo, check
     job, keep = spec.build()
@@ -345,8 +461,8 @@ def serving_bench(args):
         "server": {"budget_gb": stats.budget_bytes / 1e9, "used_gb": stats.used_bytes / 1e9,
                    "slots": stats.nslots, "refused_slots": stats.refused},
     }
-    print(json.dumps(out), flush=True)
     host.fxs_destroy(srv)
+    return out
 
 
 def uneven_bench(args):
@@ -435,9 +551,10 @@ def uneven_bench(args):
         "realtime_rate": total / args.subint / 1e9,
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
     }
-    print(json.dumps(out), flush=True)
     for j in jobs:
         j.close()
+    lib.fxg_ctx_destroy(ctx)
+    return out
 
 
 def main():
@@ -466,6 +583,8 @@ def main():
     ap.add_argument("--uneven", action="store_true",
                     help="the six-job uneven workload in one graph launch per step instead of the 32-station job")
     ap.add_argument("--subint", type=float, default=0.128, help="uneven: subintegration length in seconds")
+    ap.add_argument("--no-serving", action="store_true", help="default run: skip the serving workload")
+    ap.add_argument("--no-uneven", action="store_true", help="default run: skip the six uneven jobs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -474,14 +593,14 @@ def main():
         if args.gpus != 1:
             print("bench.py runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
             sys.exit(2)
-        serving_bench(args)
+        print(json.dumps(serving_bench(args)), flush=True)
         return
 
     if args.uneven:
         if args.gpus != 1:
             print("bench.py runs on one GPU; --gpus above 1 is not supported", file=sys.stderr)
             sys.exit(2)
-        uneven_bench(args)
+        print(json.dumps(uneven_bench(args)), flush=True)
         return
 
     if args.copy_calibration:
@@ -541,68 +660,58 @@ def main():
         log(f"  fp32 {v32:.1f} Gsamples/s")
 
     # ---- roofline of the dominant kernel (from the profiled step)
-    C_, L = spec.nchan, spec.fftchannels
-    nin = spec.nstations * spec.npol
-    specbytes = 8 if args.precision == "fp32" else 4
-    dom = max(("fstage", "xmac"), key=lambda k: stage.get(k, {"ms": 0})["ms"])
-    if dom == "fstage":
-        per_launch_blocks = info.chunk_blocks
-        # algorithmic bytes: 2-bit input read once + spectra written once
-        bytes_per_launch = (spec.nstations * spec.nrecbands * per_launch_blocks * L * 0.25 +
-                            spec.nstations * spec.nrecbands * per_launch_blocks * C_ * specbytes)
-        avg_s = stage["fstage"]["ms"] / stage["fstage"]["launches"] / 1e3
-        achieved = bytes_per_launch / avg_s / 1e9
-        kname = f"fxg::k_fstage<{L}, 2>"
-        roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kname),
-                "algorithmic_bytes_per_launch": bytes_per_launch,
-                "flops_per_launch": 5.0 * L * np.log2(L) * spec.nstations * spec.nrecbands * per_launch_blocks,
-                "avg_launch_ms": avg_s * 1e3}
-        roof["fp32_tflops"] = roof["flops_per_launch"] / avg_s / 1e12
-        # the F stage is bound by FP32 vector issue, not HBM: its nominal FFT
-        # flops (5 L log2 L per block) against the unpacked FP32 vector peak
-        # (256 CUs x 64 lanes x 2 flops x 2.4 GHz; packed FP32 is disabled)
-        roof["fp32_vector_peak_tflops"] = FP32_VECTOR_PEAK_TFLOPS
-        roof["fp32_vector_frac"] = roof["fp32_tflops"] / FP32_VECTOR_PEAK_TFLOPS
-    else:
-        # algorithmic flops: Hermitian pairs incl. autos, 8 flops per complex MAC
-        per_launch_blocks = info.chunk_blocks
-        flops = 8.0 * nin * (nin + 1) / 2 * C_ * per_launch_blocks
-        avg_s = stage["xmac"]["ms"] / stage["xmac"]["launches"] / 1e3
-        achieved = flops / avg_s / 1e12
-        peak = PEAK_TFLOPS[args.precision]
-        bf = "true" if args.precision == "bf16" else "false"
-        if launch["schedule"]["xmac_streaming"]:
-            kname = f"fxg::k_xmac_mfma_dl<2, {bf}>"
-        else:
-            kname = f"fxg::k_xmac_mfma<2, {bf}, 16>"
-        roof = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": profiled_traffic(kname), "avg_launch_ms": avg_s * 1e3}
+    roof = roofline(spec, info, stage, launch, args.precision)
 
     errors = None
     if not args.no_verify:
         log("visibility error vs the FP64 oracle on a 6-block sample")
         orc = fxcorr.load_oracle()
-        errors = vis_errors(lib, ctx, orc, args.seed + 1, 6, min(16, os.cpu_count() or 1), torch)
+        errors = vis_errors(lib, ctx, orc, args.seed + 1, 6, CPU_THREADS, torch)
         log(f"  {errors}")
 
-    cpu = None
+    cpu = cpu_extra = None
     if not args.no_cpu:
         orc = fxcorr.load_oracle()
-        threads = min(16, os.cpu_count() or 1)
-        nb = args.cpu_blocks or 2 * threads
+        nb = args.cpu_blocks or 4 * CPU_THREADS
         host = [r.cpu().numpy() for r in raws]
-        for s, h in zip(sd.stations, host):
-            s.raw = h
+        for s_, h in zip(sd.stations, host):
+            s_.raw = h
         spec_cpu = c3_spec("fp32", args.blocks)
-        log(f"CPU baseline: oracle FP32, {threads} threads, {nb} blocks")
-        v, el_cpu, ns = cpu_baseline(orc, sd, spec_cpu, threads, nb)
-        cpu = {"value": v, "unit": "Gsamples/s", "cores": threads, "kind": "port",
+        log(f"CPU baseline: oracle FP32, {CPU_THREADS} threads, {nb} blocks of the 32-station workload")
+        v, el_cpu, ns = cpu_baseline(orc, sd, spec_cpu, CPU_THREADS, nb)
+        cpu = {"value": v, "unit": "Gsamples/s", "cores": CPU_THREADS, "kind": "port",
                "sample": f"first {nb} of {args.blocks} FFT blocks of the same subintegration "
-                         f"({ns / 1e9:.3f} Gsamples, {el_cpu:.1f} s wall)"}
+                         f"({ns / 1e9:.3f} Gsamples, {el_cpu:.1f} s wall), {CPU_THREADS} threads each "
+                         f"taking a contiguous block range (core.cpp:2091-2095)",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
         log(f"  {v:.4f} Gsamples/s")
+        del host
+        for s_ in sd.stations:
+            s_.raw = None
+        cpu_extra = small_config_cpu_baselines(orc, lib, ctx, torch)
 
     lat_ms = np.array(lat) * 1e3
+    job_lat = {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))}
+    del raws, sd
+    torch.cuda.empty_cache()
+    serving = uneven = None
+    if not args.no_serving:
+        log("serving workload (3 concurrent jobs, Poisson arrivals)")
+        sargs = argparse.Namespace(**vars(args))
+        sargs.duration = min(args.duration, 6.0)
+        serving = serving_bench(sargs)
+        serving = {k: serving[k] for k in ("value", "unit", "offered_gsamples_per_s", "load_vs_realtime",
+                                          "duration_s", "latency_ms", "per_job", "server", "config")}
+        torch.cuda.empty_cache()
+    if not args.no_uneven:
+        log("six uneven jobs in one graph launch")
+        uargs = argparse.Namespace(**vars(args))
+        uargs.steps, uargs.warmup = 3, 1
+        uneven = uneven_bench(uargs)
+        uneven = {k: uneven[k] for k in ("value", "unit", "ms_per_step", "one_by_one", "realtime_rate",
+                                         "latency_ms", "config")}
+        torch.cuda.empty_cache()
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -621,13 +730,16 @@ def main():
                    "stations": spec.nstations, "subbands": spec.nsubbands, "pols": spec.npol,
                    "channels": spec.nchan, "blocks_per_subint": spec.blockspersend,
                    "xmac": args.precision, "fft": "fp32", "launch": launch},
-        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "latency_ms": job_lat,
         "stages_ms": {k: v["ms"] for k, v in stage.items()},
         "plain_fp32": fp32,
         "mixed_vs_fp32": (value / fp32["value"]) if fp32 else None,
         "vis_error_vs_fp64": errors,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_small_configs": cpu_extra,
+        "serving": serving,
+        "uneven_6jobs": uneven,
     }
     lib.fxg_ctx_destroy(ctx)
     print(json.dumps(out))
