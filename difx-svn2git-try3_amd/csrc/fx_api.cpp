@@ -108,7 +108,9 @@ struct fxg_job {
   int have_subint = 0;
   float scale = 1.0f;             // spectrum scale for reduced precision
   // device tables
-  DBuf d_st, d_bands, d_segs, d_blpairs, d_inoff, d_tw, d_lut8;
+  DBuf d_st, d_bands, d_segs, d_wterms, d_inoff, d_tw, d_lut8;
+  DBuf d_fblank, d_pbw;           // INTERLACEDVDIF: per-frame band blanking, per-(block, band) weights
+  std::vector<long long> pbw_off; // per datastream: offset of its weights in d_pbw (-1: none)
   // per-subint
   DBuf d_ss, d_flags, d_ftab, d_blocks, d_weights, d_results, d_ct;
   DBuf d_bandlist;                // global bands grouped by sample size
@@ -217,7 +219,8 @@ int set_error(int code, const char* msg) { return set_err(code, msg); }
 // come from the host-side stage 1 in fx_mux.cpp)
 int ctx_mux_launch(fxg_ctx* c, const uint8_t* src, const uint8_t* src_device, long long srcsize, uint8_t* dest,
                    const uint32_t* hdr, const unsigned long long* mask, const long long* off, long long nframes,
-                   int nthreads, int unitbits, int units, int outwords, int outframe, unsigned long long good) {
+                   int nthreads, int unitbits, int units, int outwords, int outframe, unsigned long long good,
+                   int partial) {
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   int rc;
@@ -238,7 +241,7 @@ int ctx_mux_launch(fxg_ctx* c, const uint8_t* src, const uint8_t* src_device, lo
   a.src = dsrc; a.dest = dest; a.srcoff = c->mux_off.as<long long>();
   a.mask = c->mux_mask.as<unsigned long long>(); a.headers = c->mux_hdr.as<unsigned>();
   a.nframes = (int)nframes; a.nthreads = nthreads; a.unitbits = unitbits; a.unitsper = units;
-  a.outwords = outwords; a.framebytes = outframe; a.goodmask = good;
+  a.outwords = outwords; a.framebytes = outframe; a.goodmask = good; a.partial = partial;
   HIPCHK(launch_vdifmux(a, s));
   HIPCHK(hipStreamSynchronize(s));
   return 0;
@@ -526,7 +529,8 @@ int fxg_unpack_noalloc(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data
 // jobs
 // ---------------------------------------------------------------------------
 static void job_free(fxg_job* j) {
-  DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_blpairs, &j->d_inoff, &j->d_tw, &j->d_lut8,
+  DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_wterms, &j->d_fblank, &j->d_pbw, &j->d_inoff, &j->d_tw,
+                  &j->d_lut8,
                   &j->d_ss, &j->d_flags, &j->d_ftab, &j->d_blocks, &j->d_weights, &j->d_results,
                   &j->d_spec, &j->d_V, &j->d_ct, &j->d_bandlist, &j->d_iota, &j->d_zero, &j->d_z, &j->d_rtab,
                   &j->d_pcal, &j->d_pcalaux, &j->d_pcaltone, &j->d_pcalpart, &j->d_vpart, &j->d_cexp,
@@ -620,8 +624,13 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
       inoff.push_back(x.in_off[i]);
     }
   }
-  std::vector<int2> blp(std::max(1, P.nbl));
-  for (int b = 0; b < P.nbl; b++) blp[b] = make_int2(desc->baselines[b].ds1, desc->baselines[b].ds2);
+  std::vector<DevWTerm> wt = P.wterms;
+  if (wt.empty()) wt.push_back(DevWTerm{0, -1, -1, -1});
+  // INTERLACEDVDIF datastreams: their per-(block, band) weights
+  j->pbw_off.assign(P.nds, -1);
+  long long npbw = 0;
+  for (int d = 0; d < P.nds; d++)
+    if (P.st[d].edv4) { j->pbw_off[d] = npbw; npbw += (long long)desc->blockspersend * P.st[d].nchan; }
   // split twiddle tables e^{-2 pi i m/L} = hi[m>>6] * lo[m&63], each entry
   // evaluated in double and rounded once: lo[64] then hi[L/64]
   std::vector<float2> tw(64 + P.L / 64);
@@ -639,11 +648,12 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   if ((rc = j->d_st.ensure(sizeof(DevStation) * P.st.size())) ||
       (rc = j->d_bands.ensure(sizeof(DevBand) * P.bands.size())) ||
       (rc = j->d_segs.ensure(sizeof(DevSeg) * std::max<size_t>(1, P.segs.size()))) ||
-      (rc = j->d_blpairs.ensure(sizeof(int2) * blp.size())) ||
+      (rc = j->d_wterms.ensure(sizeof(DevWTerm) * wt.size())) ||
+      (npbw > 0 && (rc = j->d_pbw.ensure(sizeof(float) * (size_t)npbw))) ||
       (rc = j->d_inoff.ensure(sizeof(long long) * std::max<size_t>(1, inoff.size()))) ||
       (rc = j->d_tw.ensure(sizeof(float2) * tw.size())) ||
       (rc = j->d_blocks.ensure(sizeof(DevBlock) * (size_t)P.nds * desc->blockspersend)) ||
-      (rc = j->d_weights.ensure(sizeof(float) * (size_t)(P.nbl + P.nds))) ||
+      (rc = j->d_weights.ensure(sizeof(float) * wt.size())) ||
       (rc = j->d_results.ensure(sizeof(float) * 2 * (size_t)std::max<long long>(1, P.info.result_length))) ||
       (rc = j->d_spec.ensure(specbytes)) ||
       (zbytes && (rc = j->d_z.ensure(zbytes))) ||
@@ -653,7 +663,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   if (hipMemcpy(j->d_st.p, P.st.data(), sizeof(DevStation) * P.st.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(j->d_bands.p, P.bands.data(), sizeof(DevBand) * P.bands.size(), hipMemcpyHostToDevice) != hipSuccess ||
       (P.segs.size() && hipMemcpy(j->d_segs.p, P.segs.data(), sizeof(DevSeg) * P.segs.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-      hipMemcpy(j->d_blpairs.p, blp.data(), sizeof(int2) * blp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(j->d_wterms.p, wt.data(), sizeof(DevWTerm) * wt.size(), hipMemcpyHostToDevice) != hipSuccess ||
       (inoff.size() && hipMemcpy(j->d_inoff.p, inoff.data(), sizeof(long long) * inoff.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(j->d_tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_err(FXG_EHIP, "table upload failed"));
@@ -925,8 +935,14 @@ static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stati
   if ((rc = j->d_ss.ensure(sizeof(DevStationSub) * ss.size())) ||
       (rc = j->d_flags.ensure(sizeof(int) * std::max<size_t>(1, flags.size()))) ||
       (rc = j->d_ftab.ensure(sizeof(int2) * (size_t)std::max<long long>(1, fbase))) ||
-      (rc = j->d_ct.ensure((size_t)std::max<long long>(4, ctbytes))))
+      (rc = j->d_ct.ensure((size_t)std::max<long long>(4, ctbytes))) ||
+      (P.any_edv4 && (rc = j->d_fblank.ensure(sizeof(unsigned long long) * (size_t)std::max<long long>(1, fbase)))))
     return rc;
+  for (int d = 0; d < P.nds; d++) {
+    DevStationSub& q = ss[d];
+    q.fblank = P.st[d].edv4 ? j->d_fblank.as<unsigned long long>() + q.fbase : nullptr;
+    q.pbw = P.st[d].edv4 ? j->d_pbw.as<float>() + j->pbw_off[d] : nullptr;
+  }
   // the tables go through the next pinned staging slot; its previous copy
   // must have run before the slot is refilled
   fxg_job::HostStage& hs = j->stage[j->stage_next];
@@ -1111,8 +1127,8 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
     if ((rc = j->ev_end())) return rc;
   }
   if ((rc = j->ev_begin(FXG_STAGE_WEIGHTS))) return rc;
-  HIPCHK(launch_weights(j->d_blocks.as<DevBlock>(), nblocks, j->d_blpairs.as<int2>(), P.nbl, P.nds,
-                        j->d_weights.as<float>(), s));
+  HIPCHK(launch_weights(j->d_st.as<DevStation>(), j->d_ss.as<DevStationSub>(), j->d_blocks.as<DevBlock>(), nblocks,
+                        j->d_wterms.as<DevWTerm>(), (int)P.wterms.size(), j->d_weights.as<float>(), s));
   if ((rc = j->ev_end())) return rc;
   // the slot is zeroed before accumulation (core.cpp:319) by a kernel, so
   // that the sequence can live in a graph
@@ -1613,6 +1629,29 @@ int fxg_tap_block_info(fxg_job* j, int ds, int block, fxg_block_info* out) {
   out->status = g.status; out->nearestsample = g.nearestsample; out->integerdelay = g.integerdelay;
   out->weight = g.status == 0 ? g.weight : 0.0f; out->fracsampleerror = g.fse; out->intwalltime = g.intwall;
   out->a = g.a; out->b = g.b; out->fracwalltime = g.fracwall; out->c = g.c;
+  return 0;
+}
+
+int fxg_tap_band_weights(fxg_job* j, int ds, int block, float* out) {
+  if (!j || !out) return set_err(FXG_EINVAL, "null argument");
+  if (!j->have_subint) return set_err(FXG_ESTATE, "no subintegration processed");
+  const Plan& P = j->plan;
+  if (ds < 0 || ds >= P.nds || block < 0 || block >= j->desc.blockspersend) return set_err(FXG_EINVAL, "index");
+  HIPCHK(hipSetDevice(j->ctx->device));
+  const int n = j->dss[ds].nrecbands;
+  if (P.st[ds].edv4) {
+    HIPCHK(hipMemcpy(out, j->d_pbw.as<float>() + j->pbw_off[ds] + (long long)block * n, sizeof(float) * n,
+                     hipMemcpyDeviceToHost));
+    DevBlock g;
+    HIPCHK(hipMemcpy(&g, j->d_blocks.as<DevBlock>() + (size_t)ds * j->desc.blockspersend + block, sizeof(g),
+                     hipMemcpyDeviceToHost));
+    if (g.status != 0) std::fill(out, out + n, 0.0f);
+    return 0;
+  }
+  fxg_block_info bi;
+  int rc = fxg_tap_block_info(j, ds, block, &bi);
+  if (rc) return rc;
+  std::fill(out, out + n, bi.status == 0 ? bi.weight : 0.0f);
   return 0;
 }
 

@@ -33,6 +33,9 @@ __device__ __forceinline__ float sample_value(const DevStation& S, const DevStat
   int ib = grp * S.groupbytes;
   int2 z = ftab[Q.fbase + f];
   if (ib < z.x || ib >= z.y) return 0.0f;
+  // INTERLACEDVDIF: bands the frame's EDV4 validity mask blanks
+  // (blank_vdif_EDV4, format_vdif.c:4699-4775)
+  if (Q.fblank && ((Q.fblank[f] >> band) & 1ull)) return 0.0f;
   int bit = ((tin - grp * S.groupsamples) * S.nchan + band) * S.nbits;
   unsigned char byte = Q.data[f * S.framebytes + S.headerbytes + ib + (bit >> 3)];
   if (S.nbits == 8) return lut8[byte];

@@ -26,6 +26,8 @@ struct DevStation {
   int order;            // fringe rotation order 0 (post-F), 1 (linear), 2 (quadratic)
   int cplx;             // complex samples (real part in the low nbits of each unit)
   int unitbits;         // bits of one band's sample: nbits, or 2 nbits if complex
+  int edv4;             // INTERLACEDVDIF: EDV4 per-band validity and weights
+  int pad0;
   double sampletime, bw;
   float lut[4];         // 2-bit level table (VDIF or Mark5B order)
 };
@@ -55,6 +57,11 @@ struct DevStationSub {
   // the subintegration's time (procslots[].offsets, Model::getScanStartSec),
   // the same in every datastream's entry
   int offsec, offns, scanstart, pad;
+  // INTERLACEDVDIF only (else null): per frame, the bands whose samples the
+  // EDV4 validity mask blanks (bit b = band b); per (block, band), the band's
+  // data weight (Mk5Mode perbandweights)
+  unsigned long long* fblank;
+  float* pbw;
 };
 
 // ---- per block, per datastream (Mode::process prologue result) -----------
@@ -67,6 +74,15 @@ struct DevBlock {
   double a, b, fracwall;
   double c;             // order 2: constant term minus integerdelay; order 0: delay at the block centre
   double wall;          // walltimesecs of the block start
+};
+
+// ---- weight table entries ---------------------------------------------------
+// Sum over the subintegration's blocks of w(d1, b1) * w(d2, b2) (a baseline
+// product, core.cpp:1028-1037) or of w(d1, b1) (d2 < 0: an autocorrelation
+// weight, mode.cpp:1300-1307), where w(d, b) is recorded band b's weight of
+// an INTERLACEDVDIF datastream and the block's data weight otherwise (b < 0).
+struct DevWTerm {
+  int d1, b1, d2, b2;
 };
 
 // ---- result scatter segments -----------------------------------------------
@@ -127,7 +143,9 @@ struct Plan {
   long long vsize = 0;                                 // complex elements of V
   std::vector<DevSeg> segs;                            // result scatter table
   int maxseglen = 0;
-  // weight table: [0, nbl) baseline weights, [nbl, nbl+nds) AC weights
+  // weight table entries (float segments copy entry voff)
+  std::vector<DevWTerm> wterms;
+  bool any_edv4 = false;                               // an INTERLACEDVDIF datastream
   int chunk_blocks = 0;
   // phase cal: result offsets per datastream, tones and first-tone offsets
   // per (datastream, recorded freq) (empty: no extraction), the device tables
@@ -157,7 +175,8 @@ int set_error(int code, const char* msg);
 // stage 2 of VDIF multiplexing (fx_api.cpp, kernel in fx_mux.hip)
 int ctx_mux_launch(fxg_ctx* c, const uint8_t* src, const uint8_t* src_device, long long srcsize, uint8_t* dest,
                    const uint32_t* hdr, const unsigned long long* mask, const long long* off, long long nframes,
-                   int nthreads, int unitbits, int units, int outwords, int outframe, unsigned long long good);
+                   int nthreads, int unitbits, int units, int outwords, int outframe, unsigned long long good,
+                   int partial);
 int64_t plan_device_bytes(const Plan& p, const fxg_job_desc* d);
 
 }  // namespace fxg

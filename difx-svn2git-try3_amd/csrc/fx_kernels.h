@@ -112,6 +112,7 @@ struct MuxArgs {
   int outwords;                 // payload words per output frame
   int framebytes;               // output frame bytes
   unsigned long long goodmask;
+  int partial;                  // corner-turn the threads an incomplete frame has
 };
 hipError_t launch_vdifmux(const MuxArgs& a, hipStream_t stream);
 
@@ -143,8 +144,8 @@ hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* z
 // V[i] (+)= sum over splits of P[s][i], i < n, in split order
 hipError_t launch_xmac_reduce(const float2* P, int nsplit, long long vsplit, float2* V, long long n, int accumulate,
                               hipStream_t stream);
-hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
-                          hipStream_t stream);
+hipError_t launch_weights(const DevStation* st, const DevStationSub* ss, const DevBlock* blocks, int nblocks,
+                          const DevWTerm* terms, int nterms, float* w, hipStream_t stream);
 hipError_t launch_pcal(const PcalArgs& a, int npcal, hipStream_t stream);
 // p[0, n) = 0 with a kernel (graph-safe clearing)
 hipError_t launch_zero(float* p, long long n, hipStream_t stream);

@@ -36,8 +36,8 @@ long long gcd_ll(long long a, long long b) { while (b) { long long t = a % b; a 
 int configure(const fxg_vdifmux_desc* d, MuxConfig* c, std::string* err) {
   if (!d) { *err = "null descriptor"; return FXG_EINVAL; }
   if (d->nthreads < 1 || d->nthreads > FXG_MUX_MAX_THREADS) { *err = "nthreads out of range"; return FXG_EINVAL; }
-  if (d->flags & ~(FXG_MUX_GOTOEND | FXG_MUX_RESPECTGRANULARITY | FXG_MUX_ENABLEVALIDITY)) {
-    *err = "legacy, complex and validity-propagating multiplexing are not built"; return FXG_EUNSUPPORTED;
+  if (d->flags & ~(FXG_MUX_GOTOEND | FXG_MUX_RESPECTGRANULARITY | FXG_MUX_ENABLEVALIDITY | FXG_MUX_PROPAGATEVALIDITY)) {
+    *err = "legacy and complex multiplexing are not built"; return FXG_EUNSUPPORTED;
   }
   if (d->input_frame_bytes <= 32 || d->input_frame_bytes % 8 || d->frames_per_second <= 0 || d->nsort < 0 ||
       d->channels_per_thread < 1) { *err = "frame size, rate, nsort or channels"; return FXG_EINVAL; }
@@ -163,9 +163,15 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* d, const uint8_t* src,
     const int ch = c.chan_of_thread[(word(p, 3) >> 16) & 0x3FF];
     if (ch < 0) { i += c.inframe; nwrong++; continue; }
     if (!have_header) {
-      // the output header: the first frame's, 16 extended bytes cleared, the
-      // channel count, thread 0, the output length, valid (vdifmux.c:664-690)
+      // the output header: the first frame's, 16 extended bytes cleared (or,
+      // propagating validity, EDV4: mask length = threads, version 4, sync
+      // word, the mask per frame below), the channel count, thread 0, the
+      // output length, valid (vdifmux.c:664-690)
       std::memcpy(proto, p, 16);
+      if (c.flags & FXG_MUX_PROPAGATEVALIDITY) {
+        setword(proto, 4, (4u << 24) | ((uint32_t)(c.nthreads & 0xFF) << 16));
+        setword(proto, 5, 0xACABFEEDu);
+      }
       unsigned logc = 0;
       for (int n = c.outchans; n > 1; n /= 2) logc++;
       setword(proto, 0, word(proto, 0) & 0x7FFFFFFFu);
@@ -204,7 +210,12 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* d, const uint8_t* src,
   if (processed < 0) processed = i;
   // the source dried up: drop the incomplete frames at the end so that the
   // next call completes them (vdifmux.c:785-873)
+  const bool propagate = (c.flags & FXG_MUX_PROPAGATEVALIDITY) != 0;
   auto drop_incomplete = [&](long long f) {
+    if (propagate) {                     // only frames with no thread at all (vdifmux.c:799-805)
+      if (mask[f] == 0) highest = f - 1;
+      return;
+    }
     if (mask[f] == c.good) return;
     highest = f - 1;
     for (int t = 0; t < c.nthreads; t++)
@@ -223,7 +234,7 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* d, const uint8_t* src,
 
   // ---- stage 2: headers on the host, payloads on the GPU
   const long long nframes = nout > 0 ? highest + 1 : 0;
-  long long good = 0, filler = 0;
+  long long good = 0, filler = 0, partial = 0;
   std::vector<uint32_t> hdr((size_t)nframes * 8);
   long long sec = first / c.fps, fno = first % c.fps;
   for (long long f = 0; f < nframes; f++) {
@@ -231,16 +242,25 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* d, const uint8_t* src,
     std::memcpy(h, proto, 32);
     h[0] = (h[0] & ~0x3FFFFFFFu) | ((uint32_t)sec & 0x3FFFFFFF);
     h[1] = (h[1] & ~0xFFFFFFu) | ((uint32_t)fno & 0xFFFFFF);
-    if (mask[f] == c.good) good++;
-    else { h[0] |= 0x80000000u; filler++; }
+    if (propagate && mask[f] != 0) {
+      // the threads present, as the EDV4 validity mask (vdifmux.c:899-927)
+      h[6] = (uint32_t)(mask[f] & 0xFFFFFFFFull);
+      h[7] = (uint32_t)(mask[f] >> 32);
+      if (mask[f] == c.good) good++; else partial++;
+    } else if (mask[f] == c.good) {
+      good++;
+    } else {
+      h[0] |= 0x80000000u;
+      filler++;
+    }
     if (++fno >= c.fps) { sec++; fno -= c.fps; }
   }
   if (nframes > 0) {
     rc = ctx_mux_launch(ctx, src, src_device, srcsize, dest, hdr.data(), mask.data(), off.data(), nframes,
-                            c.nthreads, c.bits * c.chans, c.units, c.outdata / 4, c.outframe, c.good);
+                        c.nthreads, c.bits * c.chans, c.units, c.outdata / 4, c.outframe, c.good, propagate ? 1 : 0);
     if (rc) return rc;
   }
-  long long src_used = processed, nout_frames = good + filler, start_fn = first;
+  long long src_used = processed, nout_frames = good + partial + filler, start_fn = first;
   if (nout_frames == 0 || processed == 0 || firstgood < 0 || seq_unprocessable >= nsort) {
     const long long h = scan_frames(c, src, srcsize, false);
     start_fn = h < 0 ? start_fn + (srcsize / c.inframe) / c.nthreads + 1 : h + 1;
@@ -252,8 +272,8 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* d, const uint8_t* src,
     fxg_vdifmux_stats s{};
     s.valid_frames = nvalid; s.invalid_frames = ninvalid; s.discarded_frames = ndiscard;
     s.wrong_thread_frames = nwrong; s.skipped_bytes = nskip; s.fill_bytes = nfill; s.duplicate_frames = ndup;
-    s.bytes_processed = processed; s.good_frames = good; s.partial_frames = 0; s.filler_frames = filler;
-    s.src_used = src_used; s.dest_used = (good + filler) * c.outframe; s.output_frame_bytes = c.outframe;
+    s.bytes_processed = processed; s.good_frames = good; s.partial_frames = partial; s.filler_frames = filler;
+    s.src_used = src_used; s.dest_used = (good + partial + filler) * c.outframe; s.output_frame_bytes = c.outframe;
     s.output_frames = nout_frames; s.start_frame_number = start_fn; s.output_frame_granularity = c.granularity;
     s.epoch = epoch;
     *stats = s;

@@ -17,7 +17,10 @@ __global__ __launch_bounds__(256) void k_vdifmux(MuxArgs A) {
   if (blockIdx.y == 0 && threadIdx.x < 8) frame[threadIdx.x] = A.headers[(long long)f * 8 + threadIdx.x];
   if (w >= A.outwords) return;
   unsigned* out = frame + 8;
-  if (A.mask[f] != A.goodmask) { out[w] = 0u; return; }
+  const unsigned long long m = A.mask[f];
+  // an incomplete frame is zero, or (validity propagation) keeps the threads
+  // it has, the missing threads' units zero
+  if (m != A.goodmask && (!A.partial || m == 0ull)) { out[w] = 0u; return; }
   const long long* off = A.srcoff + (long long)f * A.nthreads;
   const int U = A.unitbits;
   const unsigned umask = U == 32 ? 0xFFFFFFFFu : ((1u << U) - 1u);
@@ -27,7 +30,7 @@ __global__ __launch_bounds__(256) void k_vdifmux(MuxArgs A) {
     const long long u = (long long)w * per + j;      // output unit
     const int k = (int)(u & (A.unitsper - 1));       // thread
     const long long t = u / A.unitsper;              // sample-time
-    if (k >= A.nthreads) continue;
+    if (k >= A.nthreads || !((m >> k) & 1ull)) continue;
     const long long bit = t * U;
     const unsigned* p = (const unsigned*)(A.src + off[k]);
     v |= ((p[bit >> 5] >> (bit & 31)) & umask) << (j * U);

@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <map>
+#include <array>
 
 namespace fxg {
 
@@ -197,7 +199,14 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
   }
   for (int s = 0; s < d->ndatastreams; s++) {
     const fxg_datastream& ds = d->datastreams[s];
-    REQUIRE(ds.format == FXG_FORMAT_VDIF || ds.format == FXG_FORMAT_MARK5B, FXG_EINVAL, "format");
+    REQUIRE(ds.format == FXG_FORMAT_VDIF || ds.format == FXG_FORMAT_MARK5B || ds.format == FXG_FORMAT_INTERLACEDVDIF,
+            FXG_EINVAL, "format");
+    // INTERLACEDVDIF: the VDIF decoders plus Mk5Mode's EDV4 per-band blanking
+    // of real samples (blank_vdif_EDV4, format_vdif.c:4699-4775)
+    REQUIRE(!(ds.format == FXG_FORMAT_INTERLACEDVDIF && ds.sampling != FXG_SAMPLING_REAL), FXG_EUNSUPPORTED,
+            "INTERLACEDVDIF with complex sampling");
+    REQUIRE(!(ds.format == FXG_FORMAT_INTERLACEDVDIF && ds.headerbytes != 32), FXG_EINVAL,
+            "INTERLACEDVDIF frames carry the 32-byte EDV4 header");
     REQUIRE(ds.nrecfreqs > 0 && ds.nrecfreqs <= FXG_MAX_RECFREQS, FXG_EINVAL, "nrecfreqs");
     REQUIRE(ds.nrecbands > 0 && ds.nrecbands <= FXG_MAX_BANDS, FXG_EINVAL, "nrecbands");
     REQUIRE(power_of_two(ds.nrecbands), FXG_EUNSUPPORTED, "nrecbands must be a power of two");
@@ -215,7 +224,7 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
               "post-F fringe rotation is not supported for complex data (mode.cpp:1155-1158)");
       REQUIRE(!(ds.pcal_interval_mhz > 0), FXG_EUNSUPPORTED, "phase cal of complex data");
     }
-    if (ds.format == FXG_FORMAT_VDIF) {
+    if (ds.format != FXG_FORMAT_MARK5B) {
       REQUIRE(ds.nbits == 2 || ds.nbits == 8, FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
     } else {
       REQUIRE(ds.nbits == 2, FXG_EUNSUPPORTED, "Mark5B nbits must be 2");
@@ -380,7 +389,9 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     }
     if (f0.nchan != P.C) P.uniform = false;
     DevStation st{};
-    st.format = ds.format; st.nbits = ds.nbits; st.nchan = ds.nrecbands;
+    st.edv4 = ds.format == FXG_FORMAT_INTERLACEDVDIF ? 1 : 0;
+    if (st.edv4) P.any_edv4 = true;
+    st.format = st.edv4 ? FXG_FORMAT_VDIF : ds.format; st.nbits = ds.nbits; st.nchan = ds.nrecbands;
     st.framebytes = ds.framebytes; st.headerbytes = ds.headerbytes;
     st.databytes = ds.framebytes - ds.headerbytes; st.blanker = ds.blanker;
     st.cplx = ds.sampling == FXG_SAMPLING_COMPLEX ? 1 : 0;
@@ -392,7 +403,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     st.groupsamples = bitsper >= 8 ? 1 : 8 / bitsper;
     // decoders that stop at a blank group: real 8-bit with 2..4 bands
     // (format_vdif.c:1837-2020), complex 8-bit with any (:2829-3054)
-    st.quirk8 = ds.format == FXG_FORMAT_VDIF && ds.nbits == 8 &&
+    st.quirk8 = st.format == FXG_FORMAT_VDIF && ds.nbits == 8 &&
                 (st.cplx || (ds.nrecbands >= 2 && ds.nrecbands <= 4));
     // complex: fftchannels = nchan, sample time 1/bw (mode.cpp:82-87)
     st.C = f0.nchan; st.L = st.cplx ? f0.nchan : 2 * f0.nchan; st.s = P.arraystride[s];
@@ -419,7 +430,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       if (ds.recfreq_clockoffsetdelta_us[i] != 0.0) st.deltapol = 1;
     }
     st.band0 = band;
-    if (ds.format == FXG_FORMAT_VDIF) { st.lut[0] = -kHiMag; st.lut[1] = -1.0f; st.lut[2] = 1.0f; st.lut[3] = kHiMag; }
+    if (st.format == FXG_FORMAT_VDIF) { st.lut[0] = -kHiMag; st.lut[1] = -1.0f; st.lut[2] = 1.0f; st.lut[3] = kHiMag; }
     else { st.lut[0] = -kHiMag; st.lut[1] = 1.0f; st.lut[2] = -1.0f; st.lut[3] = kHiMag; }
     P.st.push_back(st);
     for (int k = 0; k < ds.nrecbands; k++) {
@@ -609,14 +620,42 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
       }
     }
   }
-  // baseline weights: floats, value = weight table entry b
+  // weight table entries: one per (datastream, band) pairing that a weight
+  // of the result reads; the band is -1 (the block's data weight) except on
+  // INTERLACEDVDIF datastreams, whose recorded bands carry their own weights
+  // (Mode::getDataWeight, mode.h:168), so that a job without them has one
+  // entry per baseline and one per datastream
+  P.wterms.clear();
+  std::map<std::array<int, 4>, long long> wterm_index;
+  auto wterm = [&](int d1, int b1, int d2, int b2) -> long long {
+    if (!P.st[d1].edv4) b1 = -1;
+    if (d2 >= 0 && !P.st[d2].edv4) b2 = -1;
+    if (d2 < 0) b2 = -1;
+    const std::array<int, 4> key{d1, b1, d2, b2};
+    auto it = wterm_index.find(key);
+    if (it != wterm_index.end()) return it->second;
+    P.wterms.push_back(DevWTerm{d1, b1, d2, b2});
+    wterm_index[key] = (long long)P.wterms.size() - 1;
+    return (long long)P.wterms.size() - 1;
+  };
+  auto record_band = [&](int s, int band) {
+    const fxg_datastream& ds = d->datastreams[s];
+    return band < ds.nrecbands ? band : ds.zoomband_parent[band - ds.nrecbands];
+  };
+  for (int b = 0; b < d->nbaselines; b++) wterm(d->baselines[b].ds1, -1, d->baselines[b].ds2, -1);
+  for (int s = 0; s < d->ndatastreams; s++) wterm(s, -1, -1, -1);
+  // baseline weights: floats, the product of the two record bands' weights
+  // summed over the blocks (core.cpp:1028-1037)
   for (int f = 0; f < d->nfreqs; f++) {
     if (!P.freqused[f]) continue;
     for (int b = 0; b < d->nbaselines; b++) {
       int lf = P.bllocal[b][f];
       if (lf < 0) continue;
-      for (int pp = 0; pp < d->baselines[b].npolproducts[lf]; pp++) {
-        DevSeg g{}; g.kind = SEG_FLOATS; g.resoff = 2 * P.bwoff[f][b] + pp; g.voff = b; g.nchan = 1;
+      const fxg_baseline& bl = d->baselines[b];
+      for (int pp = 0; pp < bl.npolproducts[lf]; pp++) {
+        DevSeg g{}; g.kind = SEG_FLOATS; g.resoff = 2 * P.bwoff[f][b] + pp;
+        g.voff = wterm(bl.ds1, record_band(bl.ds1, bl.band1[lf][pp]), bl.ds2, record_band(bl.ds2, bl.band2[lf][pp]));
+        g.nchan = 1;
         P.segs.push_back(g);
       }
     }
@@ -656,7 +695,7 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
         }
         at2 += nch;
         // AC weights: getWeight(false, parent) for both passes (core.cpp:1331-1366)
-        DevSeg w{}; w.kind = SEG_FLOATS; w.resoff = wat++; w.voff = d->nbaselines + s; w.nchan = 1;
+        DevSeg w{}; w.kind = SEG_FLOATS; w.resoff = wat++; w.voff = wterm(s, parent, -1, -1); w.nchan = 1;
         if (pass == 0 || d->crosspol_autocorrs) P.segs.push_back(w);
       }
     }

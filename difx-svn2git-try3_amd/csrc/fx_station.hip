@@ -34,7 +34,37 @@ constexpr unsigned long long kFill64 = 0x1122334411223344ULL;  // fill pattern, 
 constexpr unsigned kFill32 = 0x11223344U;
 
 // ---------------------------------------------------------------------------
+// EDV4 validity (blank_vdif_EDV4, format_vdif.c:4699-4775): when the first
+// frame of the buffer carries extended version 4 with a mask length of 1..64
+// (and frame length, channels and bits that match the datastream), a frame
+// whose invalid bit is clear and whose 64-bit validity mask (header bytes
+// 24-31) is incomplete has the samples of every band whose mask bit
+// (band / (nchan / mask length)) is clear blanked.  Returns those bands.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long edv4_blank(const DevStation& S, const unsigned char* first,
+                                                         const unsigned char* fr) {
+  const unsigned w4 = ld32(first + 16), w2 = ld32(first + 8), w3 = ld32(first + 12);
+  const int eversion = (int)(w4 >> 24), masklength = (int)((w4 >> 16) & 0xFFu);
+  if (eversion != 4 || masklength < 1 || masklength > 64) return 0ull;
+  const int nchan = 1 << ((w2 >> 24) & 0x1Fu);
+  const long long framelength = (long long)(w2 & 0xFFFFFFu) * 8;
+  const int nbits = (int)((w3 >> 26) & 0x1Fu) + 1;
+  if (nchan != S.nchan || framelength != S.framebytes || nbits != S.nbits) return 0ull;
+  const int d = nchan / masklength;
+  if (d < 1) return 0ull;
+  const unsigned long long good = nchan >= 64 ? ~0ull : ((1ull << nchan) - 1ull);
+  if ((ld32(fr) >> 31) & 1u) return 0ull;
+  const unsigned long long vmask = ld64(fr + 24);
+  if (vmask == good) return 0ull;
+  unsigned long long blank = 0ull;
+  for (int c = 0; c < nchan; c++)
+    if (((vmask >> (c / d)) & 1ull) == 0ull) blank |= 1ull << c;
+  return blank;
+}
+
+// ---------------------------------------------------------------------------
 // k_frames: one thread per frame -> (startvalid, endvalid) payload byte range
+// (and, INTERLACEDVDIF, the bands the frame's EDV4 mask blanks)
 // ---------------------------------------------------------------------------
 __global__ void k_frames(const DevStation* __restrict__ st, const DevStationSub* __restrict__ ss,
                          int2* __restrict__ ftab) {
@@ -85,6 +115,7 @@ __global__ void k_frames(const DevStation* __restrict__ st, const DevStationSub*
   // the blanker (mark5_format_mark5b.c:1591-1593)
   if (S.format == FXG_FORMAT_MARK5B && (fr[S.headerbytes - 11] & 0x80)) z = make_int2(kBig, 0);
   ftab[Q.fbase + f] = z;
+  if (Q.fblank) Q.fblank[f] = edv4_blank(S, Q.data, fr);
 }
 
 // number of samples in [t0, t1) of frame f (frame-local sample indices) whose
@@ -96,6 +127,24 @@ __device__ __forceinline__ long long valid_in_frame(const DevStation& S, int2 z,
   long long g1 = ((long long)z.y + S.groupbytes - 1) / S.groupbytes;
   long long a = max(t0, g0 * S.groupsamples), b = min(t1, g1 * S.groupsamples);
   return b > a ? b - a : 0;
+}
+
+// samples in [t0, t1) of frame f (absolute sample indices) of band b that
+// decode nonzero: the valid groups' samples for 2-bit levels, those whose
+// byte is not 128 for 8-bit ones ((b - 128) / 3.3 = 0)
+__device__ __forceinline__ long long nonzero_in_frame(const DevStation& S, const DevStationSub& Q, int2 z, long long f,
+                                                      long long t0, long long t1, int b) {
+  const long long base = f * (long long)S.framesamples;
+  if (S.nbits != 8) return valid_in_frame(S, z, t0 - base, t1 - base);
+  long long n = 0;
+  const unsigned char* pl = Q.data + f * S.framebytes + S.headerbytes;
+  for (long long t = t0; t < t1; t++) {
+    const int tin = (int)(t - base);
+    const int ib = tin * S.groupbytes;                 // 8-bit: one sample-time per group
+    if (ib < z.x || ib >= z.y) continue;
+    if (pl[ib + b] != 128) n++;
+  }
+  return n;
 }
 
 // ---------------------------------------------------------------------------
@@ -121,6 +170,9 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
     int w = index / FXG_FLAGS_PER_INT;
     ok = w < Q.nflags && ((flags[Q.flagoff + w] >> (index % FXG_FLAGS_PER_INT)) & 1);
   }
+  // INTERLACEDVDIF band weights stay zero unless the block unpacks (mode.cpp:661-668)
+  if (Q.pbw)
+    for (int b = 0; b < S.nchan; b++) Q.pbw[(long long)index * S.nchan + b] = 0.0f;
   if (!ok) { g.status = 1; out[(long long)s * nblocks + index] = g; return; }
   const double* ip = Q.interp;
   const int fftchannels = S.L;
@@ -169,6 +221,32 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   }
   g.weight = (float)good / (float)S.L;
   g.allvalid = good == S.L ? 1 : 0;
+  if (Q.fblank) {
+    // INTERLACEDVDIF (mk5mode.cpp:145-168): each band loses the nonzero
+    // samples that its frames' EDV4 masks blank, band weight = (good -
+    // invalid) / L, and the block keeps good - sum(invalid) / nchan
+    float gs = (float)good;
+    long long totalinvalid = 0;
+    bool blanked = false;
+    for (int b = 0; b < S.nchan; b++) {
+      long long inv = 0;
+      for (long long u = nearestsample; u < tcut;) {
+        const long long f = u / S.framesamples;
+        const long long fe = min(tcut, (f + 1) * (long long)S.framesamples);
+        if (f < Q.nframes && ((Q.fblank[f] >> b) & 1ull)) {
+          blanked = true;
+          inv += nonzero_in_frame(S, Q, ftab[Q.fbase + f], f, u, fe, b);
+        }
+        u = fe;
+      }
+      Q.pbw[(long long)index * S.nchan + b] = (gs - (float)inv) / (float)S.L;
+      totalinvalid += inv;
+    }
+    gs -= (float)totalinvalid / (float)S.nchan;
+    if (gs < 0.0f) gs = 0.0f;
+    g.weight = gs / (float)S.L;
+    if (blanked) g.allvalid = 0;
+  }
   if (!(g.weight > 0.0f)) { g.status = 3; out[(long long)s * nblocks + index] = g; return; }
   double nearestsampletime = nearestsample * S.sampletime;
   g.fse = (float)(starttime - nearestsampletime);

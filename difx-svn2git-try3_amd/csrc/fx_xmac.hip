@@ -107,36 +107,40 @@ __global__ __launch_bounds__(576) void k_xmac_f32(XmacArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// weights: per baseline the sum over blocks of w1*w2, per datastream the sum
-// of w, each added in block order
+// weights: per weight table entry the sum over blocks of w1*w2 (a baseline
+// product) or of w1 (an autocorrelation), each added in block order; w is a
+// recorded band's weight on INTERLACEDVDIF datastreams, else the block's
+// data weight
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_weights(const DevBlock* __restrict__ blocks, int nblocks,
-                                                 const int2* __restrict__ bl, int nbl, int nds,
+__device__ __forceinline__ float band_weight(const DevStation* st, const DevStationSub* ss, const DevBlock* blocks,
+                                             int nblocks, int d, int b, int k) {
+  const DevBlock& g = blocks[(long long)d * nblocks + k];
+  if (g.status != 0) return 0.0f;
+  if (b < 0) return g.weight;
+  return ss[d].pbw[(long long)k * st[d].nchan + b];
+}
+
+__global__ __launch_bounds__(256) void k_weights(const DevStation* __restrict__ st,
+                                                 const DevStationSub* __restrict__ ss,
+                                                 const DevBlock* __restrict__ blocks, int nblocks,
+                                                 const DevWTerm* __restrict__ terms, int nterms,
                                                  float* __restrict__ wout) {
-  // one workgroup per baseline, then one per datastream: the per-block terms
-  // are gathered in parallel into LDS and one lane adds them in block order
+  // one workgroup per entry: the per-block terms are gathered in parallel
+  // into LDS and one lane adds them in block order
   constexpr int CH = 4096;
   __shared__ float4 term4[CH / 4];
   float* term = (float*)term4;
   const int i = blockIdx.x;
-  if (i >= nbl + nds) return;
-  const bool isbl = i < nbl;
-  const int d1 = isbl ? bl[i].x : i - nbl;
-  const int d2 = isbl ? bl[i].y : d1;
+  if (i >= nterms) return;
+  const DevWTerm T = terms[i];
   float acc = 0.0f;
   for (int t0 = 0; t0 < nblocks; t0 += CH) {
     const int n = min(CH, nblocks - t0);
     for (int k = threadIdx.x; k < CH; k += blockDim.x) {
       float v = 0.0f;
       if (k < n) {
-        const DevBlock& g1 = blocks[(long long)d1 * nblocks + t0 + k];
-        const float w1 = g1.status == 0 ? g1.weight : 0.0f;
-        if (isbl) {
-          const DevBlock& g2 = blocks[(long long)d2 * nblocks + t0 + k];
-          v = w1 * (g2.status == 0 ? g2.weight : 0.0f);
-        } else {
-          v = w1;
-        }
+        const float w1 = band_weight(st, ss, blocks, nblocks, T.d1, T.b1, t0 + k);
+        v = T.d2 >= 0 ? w1 * band_weight(st, ss, blocks, nblocks, T.d2, T.b2, t0 + k) : w1;
       }
       term[k] = v;
     }
@@ -238,11 +242,10 @@ hipError_t launch_zero(float* p, long long n, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_weights(const DevBlock* blocks, int nblocks, const int2* bl, int nbl, int nds, float* w,
-                          hipStream_t stream) {
-  int n = nbl + nds;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_weights, dim3(n), dim3(256), 0, stream, blocks, nblocks, bl, nbl, nds, w);
+hipError_t launch_weights(const DevStation* st, const DevStationSub* ss, const DevBlock* blocks, int nblocks,
+                          const DevWTerm* terms, int nterms, float* w, hipStream_t stream) {
+  if (nterms == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_weights, dim3(nterms), dim3(256), 0, stream, st, ss, blocks, nblocks, terms, nterms, w);
   return hipGetLastError();
 }
 

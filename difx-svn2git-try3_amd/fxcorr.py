@@ -26,6 +26,7 @@ FXG_MAX_POLPRODUCTS = 4
 
 FXG_FORMAT_VDIF = 0
 FXG_FORMAT_MARK5B = 1
+FXG_FORMAT_INTERLACEDVDIF = 2
 FXG_BLANKER_FORMAT = 0
 FXG_BLANKER_NONE = 1
 FXG_SAMPLING_REAL = 0
@@ -107,6 +108,7 @@ class SubintDesc(C.Structure):
 
 FXG_MUX_MAX_THREADS = 31
 FXG_MUX_GOTOEND, FXG_MUX_RESPECTGRANULARITY, FXG_MUX_ENABLEVALIDITY = 0x01, 0x02, 0x04
+FXG_MUX_PROPAGATEVALIDITY = 0x40
 
 
 class VdifMuxDesc(C.Structure):
@@ -381,6 +383,7 @@ def load_gpu_lib():
     lib.fxg_job_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.fxg_job_stage_times.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int64), C.c_int]
     lib.fxg_tap_block_info.argtypes = [C.c_void_p, C.c_int, C.c_int, P(BlockInfo)]
+    lib.fxg_tap_band_weights.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     lib.fxg_tap_spectrum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     lib.fxg_vdifmux_output_shape.argtypes = [P(VdifMuxDesc), P(C.c_int32), P(C.c_int32)]
     lib.fxg_vdifmux.restype = C.c_int64

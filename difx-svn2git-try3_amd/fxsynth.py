@@ -146,13 +146,30 @@ def frame_stream(spec: JobSpec, codes: np.ndarray, first_frame_sec: int, first_f
         fn = first_frameno + f
         sec = first_frame_sec + fn // frames_per_sec
         fno = fn % frames_per_sec
-        if spec.fmt == FXG_FORMAT_VDIF:
+        if spec.fmt != FXG_FORMAT_MARK5B:
             out[f, : spec.headerbytes] = vdif_header(sec, fno, spec.framebytes, spec.nrecbands,
                                                      spec.nbits, station, f in inval,
                                                      spec.is_complex)[: spec.headerbytes]
         else:
             out[f, :16] = mark5b_header(fno, f in inval)
     return out.reshape(-1)
+
+
+def set_edv4(raw: np.ndarray, framebytes: int, masklength: int, masks: dict) -> np.ndarray:
+    """The frames of raw (VDIF, 32-byte headers) as EDV4 frames: extended
+    version 4 with the given validity mask length, the sync word 0xACABFEED
+    and a 64-bit validity mask per frame (masks: frame -> mask; other frames
+    get every one of the masklength bits set), in place; returns raw."""
+    fr = raw.reshape(-1, framebytes)
+    words = fr[:, :32].view(np.uint32)
+    words[:, 4] = np.uint32((4 << 24) | ((masklength & 0xFF) << 16))
+    words[:, 5] = np.uint32(0xACABFEED)
+    full = (1 << masklength) - 1
+    for f in range(fr.shape[0]):
+        m = int(masks.get(f, full))
+        words[f, 6] = np.uint32(m & 0xFFFFFFFF)
+        words[f, 7] = np.uint32(m >> 32)
+    return raw
 
 
 def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.05,

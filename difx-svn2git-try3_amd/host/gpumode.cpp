@@ -60,6 +60,7 @@ GpuMode::GpuMode(GpuCore* core, int dsindex, int numBufferedFFTs)
   freqs_.assign(nbuf_, std::vector<cf32>((size_t)nrec_ * nchan_, cf32{0, 0}));
   conj_ = freqs_;
   dataweight_.assign(nbuf_, 0.0f);
+  bandweight_.assign(nbuf_, std::vector<f32>((size_t)nrec_, 0.0f));
   for (int p = 0; p < 2; p++) {
     ac_[p].assign((size_t)nrec_ * nchan_, cf32{0, 0});
     acw_[p].assign(nrec_, 0.0f);
@@ -148,12 +149,15 @@ void GpuMode::process(int index, int subloopindex) {
   std::vector<cf32>& X = freqs_[subloopindex];
   fxg_block_info info{};
   dataweight_[subloopindex] = 0.0f;
+  std::vector<f32>& bw = bandweight_[subloopindex];
+  std::fill(bw.begin(), bw.end(), 0.0f);
   if (core_->station_block(ds_, index, (float*)X.data(), &info) != 0 || info.status != 0) {
     std::fill(X.begin(), X.end(), cf32{0, 0});
     std::fill(conj_[subloopindex].begin(), conj_[subloopindex].end(), cf32{0, 0});
     return;
   }
   dataweight_[subloopindex] = info.weight;
+  if (core_->band_weights(ds_, index, bw.data()) != 0) std::fill(bw.begin(), bw.end(), info.weight);
   std::vector<cf32>& Xc = conj_[subloopindex];
   for (size_t i = 0; i < X.size(); i++) Xc[i] = cf32{X[i].re, -X[i].im};
   // autocorrelations (mode.cpp:1292-1375)
@@ -168,7 +172,7 @@ void GpuMode::process(int index, int subloopindex) {
       const cf32* x = &X[(size_t)j * nchan_];
       cf32* a = &ac_[0][(size_t)j * nchan_];
       for (int c = 0; c < nchan_; c++) addprod_conj(a[c], x[c], x[c]);
-      acw_[0][j] += info.weight;
+      acw_[0][j] += bw[(size_t)j];                           // mode.cpp:1300-1307
     }
     if (count > 1 && cross) {
       const cf32* x0 = &X[(size_t)idx[0] * nchan_];
@@ -179,8 +183,11 @@ void GpuMode::process(int index, int subloopindex) {
         addprod_conj(a0[c], x0[c], x1[c]);
         addprod_conj(a1[c], x1[c], x0[c]);
       }
-      acw_[1][idx[0]] += info.weight;
-      acw_[1][idx[1]] += info.weight;
+      // mode.cpp:1365-1373: the product of the two bands' weights for
+      // INTERLACEDVDIF, else the block's weight
+      const f32 wx = d.format == FXG_FORMAT_INTERLACEDVDIF ? bw[(size_t)idx[0]] * bw[(size_t)idx[1]] : info.weight;
+      acw_[1][idx[0]] += wx;
+      acw_[1][idx[1]] += wx;
     }
   }
 }
@@ -199,7 +206,11 @@ const cf32* GpuMode::getConjugatedFreqs(int band, int sub) const {
   return &conj_[sub][(size_t)d.zoomband_parent[z] * nchan_ + d.zoomband_choffset[z]];
 }
 
-f32 GpuMode::getDataWeight(int, int sub) const { return dataweight_[sub]; }
+// mode.h:168: the record band's own weight (INTERLACEDVDIF), else the block's
+f32 GpuMode::getDataWeight(int outputband, int sub) const {
+  if (outputband >= 0 && outputband < nrec_) return bandweight_[sub][(size_t)outputband];
+  return dataweight_[sub];
+}
 
 const cf32* GpuMode::getAutocorrelation(bool crosspol, int band) const {
   const fxg_datastream& d = core_->dss_[ds_];
@@ -280,6 +291,13 @@ void GpuCore::gather(std::vector<fxg_station_input>& in, fxg_subint_desc& sub) c
   sub.ns = m0.offns_;
   sub.scan_start_sec = scanstart_;
   sub.model_ok = 1;
+}
+
+int GpuCore::band_weights(int ds, int block, float* out) {
+  if (!ok_) return FXG_ESTATE;
+  int rc = fxg_tap_band_weights(job_, ds, block, out);
+  if (rc) err_ = fxg_last_error();
+  return rc;
 }
 
 int GpuCore::station_block(int ds, int block, float* spectra, fxg_block_info* info) {

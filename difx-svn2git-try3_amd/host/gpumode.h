@@ -113,6 +113,9 @@ class GpuMode {
   // per subloop: spectra and conjugates of every recorded band, weights
   std::vector<std::vector<cf32>> freqs_, conj_;
   std::vector<f32> dataweight_;
+  // per subloop and recorded band: Mode::getDataWeight (INTERLACEDVDIF
+  // per-band weights, else the block's weight)
+  std::vector<std::vector<f32>> bandweight_;
   // autocorrelations[2][recorded band][chan] and weights[2][recorded band]
   std::vector<cf32> ac_[2];
   std::vector<f32> acw_[2];
@@ -177,6 +180,7 @@ class GpuCore {
   friend class GpuMode;
   friend class GpuMk5Mode;
   int station_block(int ds, int block, float* spectra, fxg_block_info* info);
+  int band_weights(int ds, int block, float* out);
   void gather(std::vector<fxg_station_input>& in, fxg_subint_desc& sub) const;
   bool ok_ = false;
   std::string err_;

@@ -55,6 +55,13 @@ extern "C" {
 /* Data formats handled by the Mk5Mode path (configuration.cpp:991-1011). */
 #define FXG_FORMAT_VDIF     0   /* VDIF, 32-byte (or 16-byte legacy) header */
 #define FXG_FORMAT_MARK5B   1   /* Mark5B, 16-byte header, 10000-byte payload */
+/* VDIF multiplexed from threads with per-thread validity (Configuration::
+ * INTERLACEDVDIF, vdifmux with validity propagation): decoded as VDIF, then
+ * the bands whose thread a frame's EDV4 validity mask marks absent are
+ * blanked and every recorded band carries its own data weight
+ * (mk5mode.cpp:78-90, 145-168; blank_vdif_EDV4, format_vdif.c:4699-4775);
+ * real sampling, 32-byte headers. */
+#define FXG_FORMAT_INTERLACEDVDIF 2
 
 /* Sampling of a datastream (Configuration::datasampling / complextype). */
 #define FXG_SAMPLING_REAL     0
@@ -351,12 +358,17 @@ int fxg_job_set_xmac_streaming(fxg_job* job, int on);
  * (:507-1044).  The frame search and ordering (stage 1) reads only headers
  * and runs on the host; the corner turn of the payloads (stage 2,
  * cornerturners.c) runs on the GPU into device memory.  Fanout 1, non-legacy
- * headers, real data, no EDV4 validity propagation; an output frame missing
- * a thread is marked invalid and its payload zeroed. */
+ * headers, real data.  An output frame missing a thread is marked invalid
+ * and its payload zeroed; with FXG_MUX_PROPAGATEVALIDITY (what mpifxcorr's
+ * datastreams use, vdiffile.cpp:536) the output headers are EDV4 (mask
+ * length = threads, sync word 0xACABFEED) and a frame with at least one
+ * thread stays valid with the threads present in its validity mask (header
+ * bytes 24-31), the missing threads' units zero. */
 #define FXG_MUX_MAX_THREADS        31
 #define FXG_MUX_GOTOEND          0x01  /* VDIF_MUX_FLAG_GOTOEND            */
 #define FXG_MUX_RESPECTGRANULARITY 0x02 /* VDIF_MUX_FLAG_RESPECTGRANULARITY */
 #define FXG_MUX_ENABLEVALIDITY   0x04  /* VDIF_MUX_FLAG_ENABLEVALIDITY     */
+#define FXG_MUX_PROPAGATEVALIDITY 0x40 /* VDIF_MUX_FLAG_PROPAGATEVALIDITY  */
 
 typedef struct {
   int32_t input_frame_bytes;     /* single-thread frame, 32-byte header included   */
@@ -479,6 +491,10 @@ int fxg_job_get_kurtosis(fxg_job* job, int thread, int datastream, int band, flo
 /* Debug taps valid after fxg_subint_process with keep_taps set. */
 int fxg_job_set_keep_taps(fxg_job* job, int keep);
 int fxg_tap_block_info(fxg_job* job, int datastream, int block, fxg_block_info* out);
+/* Data weight of every recorded band of one block (Mode::getDataWeight,
+ * mode.h:168): INTERLACEDVDIF datastreams' per-band weights, else the
+ * block's weight for every band; out[nrecbands]. */
+int fxg_tap_band_weights(fxg_job* job, int datastream, int block, float* out);
 /* Spectrum of one recorded band for one block (Mode::getFreqs, mode.h:199):
  * nchan cf32 values as FP32 (converted from the job's spectrum precision). */
 int fxg_tap_spectrum(fxg_job* job, int datastream, int band, int block, float* out);

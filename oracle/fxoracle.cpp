@@ -90,11 +90,17 @@ struct StreamFmt {
   int groupbytes;        /* bytes consumed per decoder iteration */
   int groupsamples;      /* sample-times produced per decoder iteration */
   bool noadvance_quirk;  /* 2/3/4-channel 8-bit VDIF decoders do not advance on blank */
+  bool edv4;             /* INTERLACEDVDIF: per-band weights from EDV4 validity masks */
   float lut[256];
 };
 
 static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
-  f->format = ds->format; f->nbits = ds->nbits; f->nchan = ds->nrecbands;
+  /* INTERLACEDVDIF decodes as VDIF; Mk5Mode adds the EDV4 per-band blanking
+   * and weights (mk5mode.cpp:78-90, 145-162) */
+  f->edv4 = ds->format == FXG_FORMAT_INTERLACEDVDIF;
+  if (f->edv4 && ds->sampling != FXG_SAMPLING_REAL)
+    return fail(FXG_EUNSUPPORTED, "INTERLACEDVDIF with complex sampling");
+  f->format = f->edv4 ? FXG_FORMAT_VDIF : ds->format; f->nbits = ds->nbits; f->nchan = ds->nrecbands;
   f->framebytes = ds->framebytes; f->headerbytes = ds->headerbytes;
   f->databytes = ds->framebytes - ds->headerbytes; f->blanker = ds->blanker;
   if (f->nchan <= 0 || (f->nchan & (f->nchan - 1)) != 0)
@@ -108,7 +114,7 @@ static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
    * granularity of 2, whose erasure in Mk5Mode::unpack (mk5mode.cpp:121-140)
    * runs over the complex array's floats as if they were real samples */
   if (f->cplx && bitsper < 8) return fail(FXG_EUNSUPPORTED, "complex sampling needs >= 8 bits per sample-time");
-  if (ds->format == FXG_FORMAT_VDIF) {
+  if (f->format == FXG_FORMAT_VDIF) {
     if (f->nbits == 2) lut_vdif2(f->lut);
     else if (f->nbits == 8) lut_vdif8(f->lut);
     else return fail(FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
@@ -126,7 +132,7 @@ static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
   /* decoders that do not advance past a blank group: real 8-bit with 2..4
    * channels (format_vdif.c:1837-2020), complex 8-bit with any channel count
    * (format_vdif.c:2829-3054) */
-  f->noadvance_quirk = ds->format == FXG_FORMAT_VDIF && f->nbits == 8 &&
+  f->noadvance_quirk = f->format == FXG_FORMAT_VDIF && f->nbits == 8 &&
                        (f->cplx || (f->nchan >= 2 && f->nchan <= 4));
   return 0;
 }
@@ -241,10 +247,69 @@ static int decode(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
   return nsamp - f.groupsamples * nblank;
 }
 
+/* blank_vdif_EDV4 (format_vdif.c:4699-4775) over the unpacked window:
+ * work[c*stride + i] holds sample unpackstart + i of channel c, i < nsamp.
+ * The first frame of the buffer decides whether the stream carries EDV4
+ * validity (extended version 4, mask length 1..64); then every frame of the
+ * window whose invalid bit is clear and whose validity mask is not complete
+ * has the samples of each channel whose mask bit (channel / (nchan / mask
+ * length)) is clear set to zero, counting those that were nonzero into
+ * invalid[c].  Returns false (invalid[] all zero) when the first frame is
+ * not EDV4.  Frames not wholly inside the buffer are skipped (the
+ * reference reads their headers past its buffer); masks are 64-bit. */
+static bool blank_edv4(const StreamFmt& f, const uint8_t* data, int64_t nbytes, int64_t unpackstart,
+                       float* work, int stride, int nsamp, std::vector<int>& invalid) {
+  invalid.assign((size_t)f.nchan, 0);
+  if (nbytes < 32) return false;
+  const uint32_t w4 = rd32(data + 16);
+  const int eversion = (int)(w4 >> 24), masklength = (int)((w4 >> 16) & 0xFF);
+  if (eversion != 4 || masklength < 1 || masklength > 64) return false;
+  const uint32_t w2 = rd32(data + 8), w3 = rd32(data + 12);
+  const int nChan = 1 << ((w2 >> 24) & 0x1F);
+  const int64_t frameLength = (int64_t)(w2 & 0xFFFFFF) * 8;
+  const int nbits = (int)((w3 >> 26) & 0x1F) + 1;
+  /* (a first header whose frame length, channels or bits differ from the
+   * datastream's is taken as carrying no EDV4 validity here) */
+  if (frameLength != f.framebytes || nChan != f.nchan || nbits != f.nbits) return false;
+  const int64_t sampPerFrame = (frameLength - 32) * 8 / (nChan * nbits);
+  const uint64_t goodMask = nChan >= 64 ? ~0ULL : ((1ULL << nChan) - 1ULL);
+  const int d = nChan / masklength;
+  if (sampPerFrame <= 0 || d <= 0) return false;
+  int64_t frameNum = unpackstart / sampPerFrame;
+  int64_t start = unpackstart % sampPerFrame;
+  int64_t end = start + nsamp < sampPerFrame ? start + nsamp : sampPerFrame;
+  int64_t left = nsamp, v = 0;
+  for (;;) {
+    const int64_t N = end - start;
+    const int64_t off = frameNum * frameLength;
+    if (frameNum >= 0 && off + frameLength <= nbytes) {
+      const uint8_t* h = data + off;
+      const bool hinv = (rd32(h) >> 31) & 1u;
+      const uint64_t vmask = rd64(h + 24);
+      if (!hinv && vmask != goodMask)
+        for (int c = 0; c < nChan && c < f.nchan; ++c)
+          if (((vmask >> (c / d)) & 1ULL) == 0)
+            for (int64_t q = 0; q < N; ++q) {
+              float& x = work[(size_t)c * stride + (size_t)(q + v)];
+              if (x != 0.0f) { x = 0.0f; ++invalid[(size_t)c]; }
+            }
+    }
+    left -= N;
+    if (left <= 0) break;
+    v += N;
+    ++frameNum;
+    start = 0;
+    end = left < sampPerFrame ? left : sampPerFrame;
+  }
+  return true;
+}
+
 /* Mk5Mode::unpack (mk5mode.cpp:104-177) for real sampling.  `work` holds
  * nchan*(unpacksamples+granularity) floats; on return the L samples starting
- * at sampleoffset for channel c are at work[c*stride + munged + n]. */
-struct Unpacked { int stride; int munged; float weight; };
+ * at sampleoffset for channel c are at work[c*stride + munged + n].
+ * INTERLACEDVDIF: bandweight[c] = (goodsamples - invalid[c]) / L, and the
+ * block weight loses the mean invalid count (mk5mode.cpp:145-162). */
+struct Unpacked { int stride; int munged; float weight; std::vector<float> bandweight; };
 static Unpacked mk5_unpack(const StreamFmt& f, const uint8_t* data, int64_t nbytes,
                            int64_t sampleoffset, int unpacksamples, std::vector<float>& work) {
   int samplestounpack = unpacksamples + (f.granularity > 1 ? f.granularity : 0);
@@ -267,8 +332,19 @@ static Unpacked mk5_unpack(const StreamFmt& f, const uint8_t* data, int64_t nbyt
       }
     goodsamples -= erased / (float)f.nchan;
   }
-  if (goodsamples < 0) goodsamples = 0;
   Unpacked u; u.stride = samplestounpack; u.munged = munged;
+  if (f.edv4 && !f.cplx) {
+    std::vector<int> invalid;
+    blank_edv4(f, data, nbytes, unpackstart, work.data(), samplestounpack, samplestounpack, invalid);
+    int totalinvalid = 0;
+    u.bandweight.assign((size_t)f.nchan, 0.0f);
+    for (int b = 0; b < f.nchan; ++b) {
+      u.bandweight[(size_t)b] = (goodsamples - invalid[(size_t)b]) / (float)unpacksamples;
+      totalinvalid += invalid[(size_t)b];
+    }
+    goodsamples -= (float)totalinvalid / (float)f.nchan;
+  }
+  if (goodsamples < 0) goodsamples = 0;
   u.weight = goodsamples / (float)unpacksamples;
   return u;
 }
@@ -1148,6 +1224,14 @@ struct CoreRun {
       w0[d].assign(ds.nrecbands, 0); w1[d].assign(ds.nrecbands, 0);
     }
     std::vector<float> dweight(nds);
+    /* getDataWeight per (datastream, recorded band): the INTERLACEDVDIF
+     * per-band weights, else the block's weight (mode.h:168) */
+    std::vector<std::vector<float>> bweight(nds);
+    std::vector<char> edv4(nds, 0);
+    for (int d = 0; d < nds; d++) {
+      bweight[d].assign((size_t)j->datastreams[d].nrecbands, 0.0f);
+      edv4[d] = geom[d].fmt.edv4 ? 1 : 0;
+    }
     /* this thread's phase-cal extractors per (datastream, recorded band),
      * created as Mode's constructor does (mode.cpp:412-428) */
     std::vector<std::vector<PcalExtractor<T>>> pcx(nds);
@@ -1180,6 +1264,8 @@ struct CoreRun {
         const fxg_datastream& ds = j->datastreams[d];
         BlockGeom bg = station_block<T>(j, geom[d], d, sub, st[d], blk, *ffts[d], work[d], spec[d].data());
         dweight[d] = bg.status == 0 ? bg.weight : 0.0f;
+        for (int jb = 0; jb < ds.nrecbands; jb++)
+          bweight[d][(size_t)jb] = bg.status != 0 ? 0.0f : edv4[d] ? work[d].up.bandweight[(size_t)jb] : bg.weight;
         if (bg.status != 0) continue;
         const int C = geom[d].C;
         /* phase cal of the block's L samples (mode.cpp:774-788) */
@@ -1195,7 +1281,7 @@ struct CoreRun {
             const cpx<T>* X = spec[d].data() + (size_t)jb * C;
             T* a0 = ac0[d].data() + (size_t)jb * C * 2;
             for (int c = 0; c < C; c++) addprod_conj<T, T>(a0 + 2 * c, X[c], X[c]);
-            w0[d][jb] += (T)bg.weight;
+            w0[d][jb] += (T)bweight[d][(size_t)jb];                  /* mode.cpp:1300-1307 */
           }
           if (count > 1 && j->crosspol_autocorrs) {            /* mode.cpp:1356-1375 */
             const cpx<T>* X0 = spec[d].data() + (size_t)idx[0] * C;
@@ -1203,7 +1289,8 @@ struct CoreRun {
             T* a10 = ac1[d].data() + (size_t)idx[0] * C * 2;
             T* a11 = ac1[d].data() + (size_t)idx[1] * C * 2;
             for (int c = 0; c < C; c++) { addprod_conj<T, T>(a10 + 2 * c, X0[c], X1[c]); addprod_conj<T, T>(a11 + 2 * c, X1[c], X0[c]); }
-            w1[d][idx[0]] += (T)bg.weight; w1[d][idx[1]] += (T)bg.weight;
+            const float wx = edv4[d] ? bweight[d][(size_t)idx[0]] * bweight[d][(size_t)idx[1]] : bg.weight;
+            w1[d][idx[0]] += (T)wx; w1[d][idx[1]] += (T)wx;          /* mode.cpp:1365-1373 */
           }
         }
       }
@@ -1220,7 +1307,10 @@ struct CoreRun {
             T* out = tx.data() + 2 * (size_t)(xoff[f][b] + (int64_t)p * nchan);
             for (int c = 0; c < nchan; c++) addprod_conj<T, T>(out + 2 * c, v1[c], v2[c]);
             T* bw = acc.data() + 2 * (size_t)lay.bwoff[f][b] + p;
-            *bw += (T)(dweight[bl.ds1] * dweight[bl.ds2]);
+            /* getDataWeight of each product's recorded (parent) band (core.cpp:1028-1037) */
+            const float wa = edv4[bl.ds1] ? bweight[bl.ds1][(size_t)record_band(bl.ds1, bl.band1[lf][p])] : dweight[bl.ds1];
+            const float wb = edv4[bl.ds2] ? bweight[bl.ds2][(size_t)record_band(bl.ds2, bl.band2[lf][p])] : dweight[bl.ds2];
+            *bw += (T)(wa * wb);
           }
         }
       }
@@ -1310,6 +1400,10 @@ struct CoreRun {
         }
       }
     }
+  }
+  int record_band(int d, int band) const {
+    const fxg_datastream& ds = j->datastreams[d];
+    return band < ds.nrecbands ? band : ds.zoomband_parent[band - ds.nrecbands];
   }
   const cpx<T>* band_spectrum(int d, int band, const std::vector<std::vector<cpx<T>>>& spec) const {
     const fxg_datastream& ds = j->datastreams[d];
@@ -1710,7 +1804,7 @@ static void cornerturn(const Mux& m, uint8_t* out, const std::vector<const uint8
     const long long u = ob / nBit;
     const long long t = u / nUnits;
     const int k = (int)(u % nUnits);
-    if (k >= m.nThread) continue;
+    if (k >= m.nThread || thr[k] == nullptr) continue;
     for (int b = 0; b < nBit; b++) {
       const long long ib = t * nBit + b;
       const int bit = (thr[k][ib >> 3] >> (ib & 7)) & 1;
@@ -1767,6 +1861,10 @@ static int mux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, con
     if (vhUnset) {
       std::memcpy(outHeader, cur, 16);
       std::memset(outHeader + 16, 0, 16);
+      if (m.flags & 0x40) {          /* VDIF_MUX_FLAG_PROPAGATEVALIDITY: EDV4 header (vdifmux.c:666-675) */
+        put32(outHeader, 4, (4u << 24) | ((uint32_t)(m.nThread & 0xFF) << 16));
+        put32(outHeader, 5, 0xACABFEEDu);
+      }
       uint32_t w2 = w32(outHeader, 2), w3 = w32(outHeader, 3), w0 = w32(outHeader, 0);
       unsigned logc = 0; for (int c = m.nOutputChan; c > 1; c /= 2) logc++;
       w2 = (w2 & ~(0x1Fu << 24)) | ((logc & 0x1F) << 24);
@@ -1809,7 +1907,12 @@ static int mux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, con
     }
   }
   if (bytesProcessed == -1) bytesProcessed = i;
+  const bool propagate = (m.flags & 0x40) != 0;
   auto backoff = [&](int f) {
+    if (propagate) {                 /* only frames with no thread (vdifmux.c:799-805, 841-847) */
+      if (mask[f] == 0) highestDestIndex = f - 1;
+      return;
+    }
     if (mask[f] != m.goodMask) {
       highestDestIndex = f - 1;
       for (int t = 0; t < m.nThread; ++t)
@@ -1835,7 +1938,15 @@ static int mux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, con
     std::memcpy(frame, outHeader, 32);
     put32(frame, 0, (w32(frame, 0) & ~0x3FFFFFFFu) | ((uint32_t)seconds & 0x3FFFFFFF));
     put32(frame, 1, (w32(frame, 1) & ~0xFFFFFFu) | ((uint32_t)frameNum & 0xFFFFFF));
-    if (mask[f] == m.goodMask) {
+    if (propagate && mask[f] != 0) {
+      /* the validity mask in header words 6-7; a missing thread's units are
+       * zero here (the reference points them at the start of the input,
+       * vdifmux.c:929-937) */
+      put32(frame, 6, (uint32_t)(mask[f] & 0xFFFFFFFFull));
+      put32(frame, 7, (uint32_t)(mask[f] >> 32));
+      cornerturn(m, frame + 32, ptr[f]);
+      if (mask[f] == m.goodMask) ++nGoodOutput; else ++nPartialOutput;
+    } else if (mask[f] == m.goodMask) {
       cornerturn(m, frame + 32, ptr[f]);
       ++nGoodOutput;
     } else {
@@ -1866,7 +1977,7 @@ static int mux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, con
 extern "C" int orc_vdifmux(uint8_t* dest, int destSize, const uint8_t* src, int srcSize, int inputFrameSize,
                            int framesPerSecond, int bitsPerSample, int nThread, const int* threadIds, int nSort,
                            int nGap, int flags, int channelsPerThread, int64_t startOutputFrame, int64_t* stats) {
-  if (flags & ~0x07) return -4;                      /* legacy, complex, EDV4 propagation: not restated */
+  if (flags & ~0x47) return -4;                      /* legacy and complex multiplexing: not restated */
   orcmux::Mux m;
   int r = orcmux::configure(&m, inputFrameSize, framesPerSecond, bitsPerSample, nThread, threadIds, nSort, nGap, flags,
                             channelsPerThread);
