@@ -633,7 +633,8 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     if (P.st[d].edv4) { j->pbw_off[d] = npbw; npbw += (long long)desc->blockspersend * P.st[d].nchan; }
   // split twiddle tables e^{-2 pi i m/L} = hi[m>>6] * lo[m&63], each entry
   // evaluated in double and rounded once: lo[64] then hi[L/64]
-  std::vector<float2> tw(64 + P.L / 64);
+  // then e^{-2 pi i m/256}, m < 256: the second FFT pass's twiddles, read whole
+  std::vector<float2> tw(64 + P.L / 64 + 256);
   for (int k = 0; k < 64; k++) {
     double ang = -kTwoPi * (double)k / (double)P.L;
     tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
@@ -641,6 +642,10 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   for (int h = 0; h < P.L / 64; h++) {
     double ang = -kTwoPi * (double)(64 * h) / (double)P.L;
     tw[64 + h] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  for (int m = 0; m < 256; m++) {
+    double ang = -kTwoPi * (double)m / 256.0;
+    tw[64 + P.L / 64 + m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
   }
   size_t specbytes = (size_t)P.nbands_total * chunk * P.C * spectrum_bytes(prec);
   const size_t zbytes = P.L > kFStageMaxL ? (size_t)P.nbands_total * P.zcap * P.L * sizeof(float2) : 0;

@@ -30,7 +30,7 @@ struct FStageArgs {
   const DevBlock* blocks;
   const int2* ftab;
   const float* lut8;
-  const float2* tw;           // e^{-2 pi i k / L}, k < L
+  const float2* tw;           // e^{-2 pi i m/L}: m < 64, then 64 h for h < L/64; then e^{-2 pi i m/256}, m < 256
   const unsigned char* ct;    // corner-turned per-band sample streams
   const int* bandlist;        // global band per grid row (null: row = band)
   void* spectra;

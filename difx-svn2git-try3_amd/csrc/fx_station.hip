@@ -324,15 +324,31 @@ template <> struct DFT<8> {
     v[3] = add2(e3, o3); v[7] = sub2(e3, o3);
   }
 };
+// 16-point DFT from the first butterflies' outputs: sm[r] = v[r] + v[r + 8],
+// df[r] = v[r] - v[r + 8] (r < 8), so that a caller can fold the products
+// that form v into them
+__device__ __forceinline__ void dft16_from_halves(const float2* sm, const float2* df, float2* v);
+
 template <> struct DFT<16> {
   __device__ static void run(float2* v) {
+    float2 sm[8], df[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) { sm[r] = add2(v[r], v[r + 8]); df[r] = sub2(v[r], v[r + 8]); }
+    dft16_from_halves(sm, df, v);
+  }
+};
+
+__device__ __forceinline__ void dft16_from_halves(const float2* sm, const float2* df, float2* v) {
+  {
     // n = 4 n1 + n2, k = k1 + 4 k2
     float2 y[4][4];
 #pragma unroll
     for (int n2 = 0; n2 < 4; n2++) {
-      float2 a = v[n2], b = v[4 + n2], c = v[8 + n2], d = v[12 + n2];
-      dft4(a, b, c, d);
-      y[n2][0] = a; y[n2][1] = b; y[n2][2] = c; y[n2][3] = d;
+      // dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]) with its first sums given
+      const float2 a = sm[n2], b = df[n2], c = sm[4 + n2], d = df[4 + n2];
+      const float2 md = mul_mi(d);
+      y[n2][0] = add2(a, c); y[n2][2] = sub2(a, c);
+      y[n2][1] = add2(b, md); y[n2][3] = sub2(b, md);
     }
     // twiddles W16^(n2*k1)
     const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
@@ -352,7 +368,7 @@ template <> struct DFT<16> {
       v[k1] = a; v[k1 + 4] = b; v[k1 + 8] = c; v[k1 + 12] = d;
     }
   }
-};
+}
 
 // L = 2^LOG = 16^a * 2^q.  Pass 1 is radix 16 (Ns = 1, straight from the
 // decoder), then NMID radix-16 passes, then a last pass of radix RL = 2^q (16
@@ -527,6 +543,18 @@ __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restric
 // NT: FP64 phases, FP32 residual sin/cos (mode.cpp:909-1097).  rot: the pre-F
 // sub/step (and quadratic) rotators; frac: the fractional-sample sub/step.
 // ---------------------------------------------------------------------------
+// e^{i 2 pi t} for t in turns (|t| < 256) by the hardware sine and cosine
+__device__ __forceinline__ float2 turn_rotator(float t) {
+  return make_float2(__builtin_amdgcn_cosf(t), __builtin_amdgcn_sinf(t));
+}
+// e^{i x} for an FP32 angle x in radians: x reduced exactly to turns in FP64,
+// then the hardware sine and cosine (the same value sinf / cosf give x)
+__device__ __forceinline__ float2 angle_rotator(float x) {
+  double t = (double)x * (1.0 / kTwoPi);
+  t -= rint(t);
+  return turn_rotator((float)t);
+}
+
 __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStation& S, const DevBlock& G, int L,
                                                int tid, int NT, bool rot, bool frac, float2* rsub,
                                                float2* rstep, float2* fsub, float2* fstep, float2* rquad) {
@@ -576,40 +604,33 @@ __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStatio
         ph += -looffset * G.fracwall;
         if (isfraclooffset) ph += -fraclooffset * G.intwall;
       }
-      float arg = (float)(-kTwoPi * (ph - (int)ph));
-      float sn, cs; sincosf(arg, &sn, &cs);
-      rsub[k] = make_float2(cs, sn);
+      // phase -2 pi frac(ph) (mode.cpp:962-967), as turns
+      rsub[k] = turn_rotator((float)((double)(int)ph - ph));
       if (S.order == 2) {
         const double qph = (subxoff * (2 * G.a * stepxoff1)) * lofreq;
-        arg = (float)(-kTwoPi * (qph - (int)qph));
-        sincosf(arg, &sn, &cs);
-        rquad[k] = make_float2(cs, sn);
+        rquad[k] = turn_rotator((float)((double)(int)qph - qph));
       }
     } else if (e < e2) {
       const int j = e - e1;
       const double stepxoff = ((double)(j * s)) / (double)L;
       double ph = (S.order == 1 ? stepxoff * G.a : stepxoff * G.b + (stepxoff * stepxoff) * G.a) * lofreq;
       if (looff) ph += (((j * s) * S.sampletime) / 1e6) * (-looffset);
-      float arg = (float)(-kTwoPi * (ph - (int)ph));
-      float sn, cs; sincosf(arg, &sn, &cs);
-      rstep[j] = make_float2(cs, sn);
+      rstep[j] = turn_rotator((float)((double)(int)ph - ph));
     } else if (e < e3) {
       const int k = e - e2;
       float cf = (float)((kTwoPi * k * S.bw) / C);
       float arg = cf * val;
       if (S.order == 0) arg = arg + postf;
-      float sn, cs; sincosf(arg, &sn, &cs);
       // the band's polarisation phase correction rides on every channel's
       // fractional-sample rotator (a constant factor)
-      fsub[k] = cmulf(make_float2(cs, sn), make_float2(B.polre, B.polim));
+      fsub[k] = cmulf(angle_rotator(arg), make_float2(B.polre, B.polim));
     } else {
       const int j = e - e3;
       // step channel frequencies (mode.cpp:328-334, 864-880): DSB, LSB, USB
       double dcf = B.dsb ? (kTwoPi * j * s * S.bw) / C - kTwoPi * S.bw / 2.0
                  : B.lsb ? (-kTwoPi * ((S.nfracsteps - j) * s) * S.bw) / C : (kTwoPi * j * s * S.bw) / C;
       float cf = (float)dcf;
-      float sn, cs; sincosf(cf * val, &sn, &cs);
-      fstep[j] = make_float2(cs, sn);
+      fstep[j] = angle_rotator(cf * val);
     }
   }
 }
@@ -697,6 +718,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   struct Smem {
     float2 tabs[Sh::TABS];                      // rotator sub/step tables, packed
     float2 twlo[64], twhi[L / 64];              // e^{-2 pi i m/L}, e^{-2 pi i 64h/L}
+    float2 tw256[Sh::NMID > 0 ? 256 : 1];       // e^{-2 pi i m/256}: the second pass's twiddles
     float lut8s[NBITS == 8 ? 256 : 4];          // level table indexed by the sample code
     unsigned raw_own[kAliasRaw ? 2 : RAWW + (RAWW & 1)];
     float2 buf[Sh::PAD];
@@ -706,6 +728,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   float2 (&tabs)[Sh::TABS] = sm.tabs;
   float2 (&twlo)[64] = sm.twlo;
   float2 (&twhi)[L / 64] = sm.twhi;
+  float2* tw256 = sm.tw256;
   float (&lut8s)[NBITS == 8 ? 256 : 4] = sm.lut8s;
   unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : sm.raw_own;
 
@@ -746,6 +769,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   float2* rquad = fstep + S.nfracsteps;         // order 2 only
   for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
   for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
+  if (Sh::NMID > 0)
+    for (int i = tid; i < 256; i += NT) tw256[i] = A.tw[64 + L / 64 + i];
 
   // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
   rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
@@ -773,6 +798,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   float2 v[16];
   if (tid < NTF) {
     const int j = tid;
+    bool transformed = false;                  // the branch ran the 16-point DFT itself
     if (kWordStep && fast && rotfast && S.order == 1) {
       const int bp = bitoff + NBITS * j;
       const unsigned* wp = raw + (bp >> 5);
@@ -780,14 +806,22 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
       const float2 r0 = rsub[j & (s - 1)];
       const float2* stp = rstep + (j >> slog);
       const int sstep = NTF >> slog;
+      // x[r] rot[r] and x[r + 8] rot[r + 8] go straight into the first
+      // butterflies: sum and difference as two FMAs on the first product
+      float2 sm[8], df[8];
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const unsigned code = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
-        const float x = lut8s[code];
-        const float2 rot = cmulf(r0, *stp);     // sub[k] * step[j], mode.cpp:975-977
-        stp += sstep;
-        v[r] = make_float2(x * rot.x, x * rot.y);
+      for (int r = 0; r < 8; r++) {
+        const unsigned c0 = (wp[r * WSTEP] >> sh) & ((1u << NBITS) - 1u);
+        const unsigned c8 = (wp[(r + 8) * WSTEP] >> sh) & ((1u << NBITS) - 1u);
+        const float x0 = lut8s[c0], x8 = lut8s[c8];
+        const float2 q0 = cmulf(r0, stp[r * sstep]);         // sub[k] * step[j], mode.cpp:975-977
+        const float2 q8 = cmulf(r0, stp[(r + 8) * sstep]);
+        const float2 p = make_float2(x0 * q0.x, x0 * q0.y);
+        sm[r] = make_float2(__builtin_fmaf(x8, q8.x, p.x), __builtin_fmaf(x8, q8.y, p.y));
+        df[r] = make_float2(__builtin_fmaf(-x8, q8.x, p.x), __builtin_fmaf(-x8, q8.y, p.y));
       }
+      dft16_from_halves(sm, df, v);
+      transformed = true;
     } else if (kWordStep && fast && S.order == 0) {
       // post-F fringe rotation: the samples are transformed unrotated
       const int bp = bitoff + NBITS * j;
@@ -849,7 +883,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
         v[r] = S.order == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
       }
     }
-    DFT<16>::run(v);
+    if (!transformed) DFT<16>::run(v);
   }
   if (kAliasRaw) __syncthreads();            // every stream word read before buf is written
   if (tid < NTF) {
@@ -868,7 +902,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
     if (tid < NTF) {
       const int k = j & (Ns - 1);
       float2 w[16];
-      twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
+      if (p == 0) {
+        // second pass (Ns = 16): W_L^(r k L/256) = W_256^(r k), r k < 256, read whole
+#pragma unroll
+        for (int r = 0; r < 16; r++) w[r] = tw256[r * k];
+      } else {
+        twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
+      }
       const float2* ip = buf + pidx(j);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
@@ -969,6 +1009,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   struct Smem {
     float2 tabs[Sh::TABS];
     float2 twlo[64], twhi[L / 64];
+    float2 tw256[Sh::NMID > 0 ? 256 : 1];       // e^{-2 pi i m/256}: the second pass's twiddles
     float lut8s[NBITS == 8 ? 256 : 4];
     unsigned raw_own[kAliasRaw ? 2 : RAWW + (RAWW & 1)];
     float2 buf[Sh::PAD];
@@ -978,6 +1019,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   float2 (&tabs)[Sh::TABS] = sm.tabs;
   float2 (&twlo)[64] = sm.twlo;
   float2 (&twhi)[L / 64] = sm.twhi;
+  float2* tw256 = sm.tw256;
   float (&lut8s)[NBITS == 8 ? 256 : 4] = sm.lut8s;
   unsigned* raw = kAliasRaw ? (unsigned*)(buf + Sh::PAD) - RAWW : sm.raw_own;
 
@@ -1012,6 +1054,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   float2* rquad = fstep + S.nfracsteps;
   for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
   for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
+  if (Sh::NMID > 0)
+    for (int i = tid; i < 256; i += NT) tw256[i] = A.tw[64 + L / 64 + i];
   rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
   if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
   else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
@@ -1095,7 +1139,13 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
     if (tid < NTF) {
       const int k = j & (Ns - 1);
       float2 w[16];
-      twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
+      if (p == 0) {
+        // second pass (Ns = 16): W_L^(r k L/256) = W_256^(r k), r k < 256, read whole
+#pragma unroll
+        for (int r = 0; r < 16; r++) w[r] = tw256[r * k];
+      } else {
+        twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
+      }
       const float2* ip = buf + pidx(j);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
