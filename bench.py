@@ -353,6 +353,7 @@ def serving_bench(args):
     host = fxcorr.load_host_lib()
     srv = C.c_void_p()
     check(host, host.fxs_create(0, 0, C.byref(srv)), "fxs_create")
+    check(host, host.fxs_set_max_inflight(srv, args.inflight), "fxs_set_max_inflight")
     cfgs = []
     for i, (nst, nchan) in enumerate(SERVING_JOBS):
         L = 2 * nchan
@@ -457,7 +458,8 @@ def serving_bench(args):
                        "max": float(lat.max())},
         "per_job": per, "dtype": args.precision, "data": "synthetic 2-bit VDIF generated on the GPU",
         "config": {"workload": "3 jobs (10/20/32 stations; 1024/4096/8192 ch) x 8 subbands x 64 MHz, 2-bit, "
-                               "0.125 s subints, Poisson arrivals", "slots_per_job": args.slots},
+                               "0.125 s subints, Poisson arrivals", "slots_per_job": args.slots,
+                   "max_inflight": args.inflight},
         "server": {"budget_gb": stats.budget_bytes / 1e9, "used_gb": stats.used_bytes / 1e9,
                    "slots": stats.nslots, "refused_slots": stats.refused},
     }
@@ -576,6 +578,8 @@ def main():
     ap.add_argument("--load", type=float, default=1.5, help="serving: offered load as a multiple of real time")
     ap.add_argument("--duration", type=float, default=8.0, help="serving: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=2, help="serving: concurrent slots per job")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="serving: requests on the device at once, started first come first served (0: no bound)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the library's autotuner (XMAC schedule and pass length)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",

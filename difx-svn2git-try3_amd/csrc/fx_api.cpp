@@ -607,7 +607,7 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
     // whose scattered 64-byte pieces cost more than the grouped reads save)
     bool grouped = prec != FXG_XMAC_FP32 && P.C % 16 == 0 && P.L <= kFStageMaxL;
     for (const XmacFreq& x : P.xf) {
-      if (!xmac_mfma_supported((int)x.in_station.size(), x.nchan)) grouped = false;
+      if (!xmac_mfma_streaming_supported((int)x.in_station.size(), x.nchan)) grouped = false;
       for (long long o : x.in_off) if (o % 16) grouped = false;     // zoom channel offsets
     }
     j->gstride = grouped ? 16 * chunk : 16;
@@ -1098,7 +1098,12 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       xa.nblocks = cn;
       xmac_shape(xa.nin, xa.nchan, &xa, j->xmac_ct, j->xmac_tb);
       const bool fp8 = prec == FXG_XMAC_FP8;
-      const bool mfma = prec != FXG_XMAC_FP32 && xmac_mfma_supported(xa.nin, xa.nchan);
+      // above 64 inputs only the streamed kernel runs on the matrix cores
+      const bool big = xa.nin > 64;
+      const bool mfma = prec != FXG_XMAC_FP32 &&
+                        (big ? !fp8 && xmac_mfma_streaming_supported(xa.nin, xa.nchan) &&
+                                   xmac_mfma_streaming_ok(xa, xf.in_off.data())
+                             : xmac_mfma_supported(xa.nin, xa.nchan));
       if (fp8) { xa.cexp = j->d_cexp.as<signed char>(); xa.eoff = j->d_eoff.as<int>() + j->inoff_base[x]; }
       float2* V = j->d_V.as<float2>() + xf.voff;
       // few channel groups: split the blocks over grid rows too, into partial
@@ -1113,7 +1118,7 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       if (fp8) {
         HIPCHK(launch_xmac_fp8(xa, s));
       } else if (mfma) {
-        if (j->xmac_streaming && xmac_mfma_streaming_ok(xa, xf.in_off.data()))
+        if ((j->xmac_streaming || big) && xmac_mfma_streaming_ok(xa, xf.in_off.data()))
           HIPCHK(launch_xmac_mfma_streaming(xa, prec, j->d_zero.p, s));
         else
           HIPCHK(launch_xmac_mfma(xa, prec, s));
@@ -1258,8 +1263,9 @@ static std::vector<fxg_schedule> xmac_candidates(const fxg_job* j, const fxg_sch
   bool mfma = false, f32 = false;
   for (const XmacFreq& xf : P.xf) {
     const int nin = (int)xf.in_station.size();
-    if (j->desc.xmac_precision != FXG_XMAC_FP32 && xmac_mfma_supported(nin, xf.nchan)) mfma = true;
-    else f32 = true;
+    const bool reduced = j->desc.xmac_precision != FXG_XMAC_FP32;
+    if (reduced && xmac_mfma_supported(nin, xf.nchan)) mfma = true;
+    else if (!(reduced && nin > 64 && xmac_mfma_streaming_supported(nin, xf.nchan))) f32 = true;   // (streamed only)
   }
   if (mfma) {
     for (int st = 0; st < 2; st++) { fxg_schedule c = base; c.xmac_streaming = st; out.push_back(c); }

@@ -140,6 +140,9 @@ hipError_t launch_xmac_mfma(const XmacArgs& a, int prec, hipStream_t stream);
 // fp8 e4m3 spectra on the microscaled matrix cores (nin <= 64, nchan % 16 == 0)
 hipError_t launch_xmac_fp8(const XmacArgs& a, hipStream_t stream);
 bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host);
+// the streamed kernel also takes more than 64 inputs (chunks of 64 in pairs)
+constexpr int kXmacMfmaMaxInputs = 512;
+bool xmac_mfma_streaming_supported(int nin, int nchan);
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream);
 // V[i] (+)= sum over splits of P[s][i], i < n, in split order
 hipError_t launch_xmac_reduce(const float2* P, int nsplit, long long vsplit, float2* V, long long n, int accumulate,

@@ -136,23 +136,64 @@ __global__ __launch_bounds__(64 * CG) void k_xmac_mfma(XmacArgs A) {
 
 // ---------------------------------------------------------------------------
 // Streaming variant: the spectra go straight from HBM into LDS with gfx950's
-// global_load_lds_dwordx4 (no registers), double-buffered, so the loads of
-// the next 16 blocks are in flight while the matrix cores work on the
-// current ones.  LDS per stage: [input][block][16 channels], one 64-byte row
-// per (input, block), inputs padded to 260 dwords.  One load instruction
-// fills 16 rows (16 blocks) of one input; rows past the last block or past
-// the last input read a 16-byte zero buffer.
+// global_load_lds_dwordx4 (no registers) through a ring of NBUF stages of
+// 8 blocks, NBUF - 1 of them in flight while the matrix cores work on the
+// current one: one workgroup per CU keeps ~100 KB of loads outstanding, what
+// HBM's latency needs for full rate (one stage in flight held it near half).
+// LDS per stage: [row][block][CG channels], one 16 CG-byte row per (input,
+// block), rows padded by 4 dwords.  One load instruction (2 CG lanes) fills
+// the 8 blocks of one row; rows past the last block or past the last input
+// read a 16-byte zero buffer.
+//
+// More than 64 inputs: the inputs form chunks of 64, and grid z runs over
+// chunk pairs.  A diagonal workgroup (CROSS false, T = 2) loads chunk
+// p = blockIdx.z as its 64 rows and accumulates that chunk's tiles I <= J; a
+// cross workgroup (CROSS true, T = 4) loads chunk p as rows 0-63 and chunk
+// q > p as rows 64-127, the z-th pair in (p, q) order, and accumulates the
+// four 32x32 tiles between them.
 // ---------------------------------------------------------------------------
-template <int T, bool BF>
-__global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* __restrict__ zero16) {
-  constexpr int CG = 16;                 // channels (wavefronts) per workgroup
-  constexpr int TT = 16;                 // blocks per stage
-  constexpr int NR = 32 * T;             // inputs, padded to the tile size
-  constexpr int NTILE = T * (T + 1) / 2;
-  constexpr int SDW = TT * CG + 4;       // dwords per input per stage
+// 16-byte load lanes per row and stage: 8 blocks of CG channels
+constexpr int TT_LANES(int cg) { return 8 * cg / 4; }
+
+template <int T, bool CROSS>
+struct DlTiles {
+  static constexpr int N = CROSS ? 4 : T * (T + 1) / 2;
+  // tile pair q: rows of tile I, columns of tile J
+  static constexpr __host__ __device__ int I(int q) {
+    if (CROSS) return q >> 1;
+    int i = 0;
+    while (q >= T - i) { q -= T - i; i++; }
+    return i;
+  }
+  static constexpr __host__ __device__ int J(int q) {
+    if (CROSS) return 2 + (q & 1);
+    int i = 0;
+    while (q >= T - i) { q -= T - i; i++; }
+    return i + q;
+  }
+};
+
+template <int T, bool BF, bool CROSS>
+__global__ __launch_bounds__(CROSS ? 512 : 1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* __restrict__ zero16) {
+  static_assert(!CROSS || T == 4, "a cross workgroup holds two 64-input chunks");
+  // channels (wavefronts) per workgroup: a cross wavefront's 4 tiles of
+  // accumulators need the registers of 2 wavefronts per SIMD
+  constexpr int CG = CROSS ? 8 : 16;
+  constexpr int LLANES = TT_LANES(CG);   // load lanes: 16-byte pieces of one row-stage
+  constexpr int TT = 8;                  // blocks per stage: one K step of the matrix cores
+  constexpr int NR = 32 * T;             // rows, padded to the tile size
+  static_assert(LLANES == TT * CG / 4, "one load lane per 16-byte piece");
+  using TL = DlTiles<T, CROSS>;
+  constexpr int NTILE = TL::N;
+  constexpr int SDW = TT * CG + 4;       // dwords per row per stage
   constexpr int STAGE = NR * SDW;        // dwords per stage
+  constexpr int NBUF = T == 1 ? 8 : 4;   // ring of stages: 132-136 KB of LDS
   constexpr int LPW = NR / CG;           // load instructions per wave per stage
-  __shared__ __align__(16) unsigned lds[2 * STAGE];
+  // s_waitcnt with vmcnt = (NBUF - 2) LPW (the later stages may stay in
+  // flight), expcnt and lgkmcnt left at their maxima
+  constexpr int VMW = (NBUF - 2) * LPW;
+  constexpr int WAITIMM = (VMW & 15) | (((VMW >> 4) & 3) << 14) | (7 << 4) | (15 << 8);
+  __shared__ __align__(16) unsigned lds[NBUF * STAGE];
 
   const char* spec = (const char*)A.spec;
   const int tid = threadIdx.x;
@@ -162,89 +203,103 @@ __global__ __launch_bounds__(1024) void k_xmac_mfma_dl(XmacArgs A, const uint4* 
   const int grp = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;   // XCD-aware groups
   const int c0 = grp * CG;
   const int half = l >> 5, row = l & 31;
-  const int lrow = l >> 2, lq = l & 3;   // load lane: block row and 4-channel quad
+  const int lrow = l / (CG / 4), lq = l % (CG / 4);   // load lane: block row and 4-channel quad
   const int sb0 = blockIdx.y * A.split_blocks;               // this row's blocks
   const int nb = min(A.nblocks - sb0, A.split_blocks);
   float2* const Vout = A.V + (long long)blockIdx.y * A.vsplit;
-  // the inputs this wavefront loads, with their offsets read once into
-  // scalar registers (a vector load here would make every later wait for
-  // it also wait for the loads in flight)
+  // the two input chunks of this workgroup's rows (rows 0-63: chunk p,
+  // rows 64-127: chunk q)
+  int p = blockIdx.z, q = blockIdx.z;
+  if (CROSS) {
+    const int nch = (A.nin + 63) >> 6;
+    int rem = blockIdx.z;
+    p = 0;
+    while (rem >= nch - 1 - p) { rem -= nch - 1 - p; p++; }
+    q = p + 1 + rem;
+  }
+  auto input_of = [&](int r) { return r < 64 ? 64 * p + r : 64 * q + r - 64; };
+  // the rows this wavefront loads, with their offsets read once into scalar
+  // registers (a vector load here would make every later wait for it also
+  // wait for the loads in flight)
   const int wu = __builtin_amdgcn_readfirstlane(w);
   long long ioff[LPW];
 #pragma unroll
   for (int u = 0; u < LPW; u++) {
-    const int a = wu * LPW + u;
+    const int a = input_of(wu * LPW + u);
     ioff[u] = a < A.nin ? A.in_off[a] + cidx(c0, A.gstride) : -1;
   }
 
+  // every stage issues LPW loads per wavefront, also past the last block
+  // (zeros), so that the wait counts below are the same for every stage
   auto issue = [&](int stage, int buf) {
-    const int t = stage * TT + lrow;
+    if (l < LLANES) {
+      const int t = stage * TT + lrow;
 #pragma unroll
-    for (int u = 0; u < LPW; u++) {
-      const void* src = zero16;
-      if (ioff[u] >= 0 && t < nb)
-        src = spec + 4 * (ioff[u] + (long long)(sb0 + t) * A.block_stride + 4 * lq);
-      __builtin_amdgcn_global_load_lds(src,
-                                       (__attribute__((address_space(3))) void*)(lds + buf * STAGE + (wu * LPW + u) * SDW),
-                                       16, 0, 0);
+      for (int u = 0; u < LPW; u++) {
+        const void* src = zero16;
+        if (ioff[u] >= 0 && t < nb)
+          src = spec + 4 * (ioff[u] + (long long)(sb0 + t) * A.block_stride + 4 * lq);
+        __builtin_amdgcn_global_load_lds(
+            src, (__attribute__((address_space(3))) void*)(lds + buf * STAGE + (wu * LPW + u) * SDW), 16, 0, 0);
+      }
     }
+  };
+  // workgroup barrier without __syncthreads' memory fence, which would wait
+  // for every load in flight; the empty asm statements keep the compiler
+  // from moving LDS accesses across it
+  auto barrier = [] {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
   };
 
   f16_t accR[NTILE], accI[NTILE];
 #pragma unroll
-  for (int q = 0; q < NTILE; q++)
+  for (int k = 0; k < NTILE; k++)
 #pragma unroll
-    for (int e = 0; e < 16; e++) { accR[q][e] = 0.0f; accI[q][e] = 0.0f; }
+    for (int e = 0; e < 16; e++) { accR[k][e] = 0.0f; accI[k][e] = 0.0f; }
 
   const int nstage = (nb + TT - 1) / TT;
-  issue(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
+#pragma unroll
+  for (int st = 0; st < NBUF - 1; st++) issue(st, st);
+  __builtin_amdgcn_s_waitcnt(WAITIMM);   // stage 0 has landed
+  barrier();
   for (int sg = 0; sg < nstage; sg++) {
-    const int buf = sg & 1;
-    if (sg + 1 < nstage) issue(sg + 1, buf ^ 1);
-    const unsigned* base = lds + buf * STAGE + w;                 // this wavefront's channel
+    // the buffer the previous iteration read (everyone passed its barrier)
+    issue(sg + NBUF - 1, (sg + NBUF - 1) % NBUF);
+    const unsigned* base = lds + (sg % NBUF) * STAGE + w;      // this wavefront's channel
+    uint4 f[T], fh[T];
 #pragma unroll
-    for (int ks = 0; ks < TT / 8; ks++) {
-      uint4 f[T];
-#pragma unroll
-      for (int I = 0; I < T; I++) {
-        const unsigned* p = base + (32 * I + row) * SDW + (8 * ks + 4 * half) * CG;
-        f[I] = make_uint4(p[0], p[CG], p[2 * CG], p[3 * CG]);
-      }
-      int q = 0;
-#pragma unroll
-      for (int I = 0; I < T; I++) {
-        const uint4 fh = make_uint4(hat(f[I].x), hat(f[I].y), hat(f[I].z), hat(f[I].w));
-#pragma unroll
-        for (int J = I; J < T; J++, q++) {
-          accR[q] = mfma<BF>(f[I], f[J], accR[q]);
-          accI[q] = mfma<BF>(fh, f[J], accI[q]);
-        }
-      }
+    for (int I = 0; I < T; I++) {
+      const unsigned* ptr = base + (32 * I + row) * SDW + 4 * half * CG;
+      f[I] = make_uint4(ptr[0], ptr[CG], ptr[2 * CG], ptr[3 * CG]);
+      fh[I] = make_uint4(hat(f[I].x), hat(f[I].y), hat(f[I].z), hat(f[I].w));
     }
-    __builtin_amdgcn_s_waitcnt(0);       // this wavefront's loads of the next stage have landed
-    __syncthreads();                     // everyone's have, and nobody still reads this stage
+#pragma unroll
+    for (int k = 0; k < NTILE; k++) {
+      accR[k] = mfma<BF>(f[TL::I(k)], f[TL::J(k)], accR[k]);
+      accI[k] = mfma<BF>(fh[TL::I(k)], f[TL::J(k)], accI[k]);
+    }
+    __builtin_amdgcn_s_waitcnt(WAITIMM);   // this wavefront's loads of stage sg + 1 have landed
+    barrier();                             // everyone's have, and nobody still reads stage sg
   }
+  __builtin_amdgcn_s_waitcnt(0);           // the zero-filled stages past the end, before exit
 
   const int c = c0 + w;
   if (c >= A.nchan) return;
-  int q = 0;
 #pragma unroll
-  for (int I = 0; I < T; I++)
+  for (int k = 0; k < NTILE; k++) {
+    const int b = input_of(32 * TL::J(k) + row);
 #pragma unroll
-    for (int J = I; J < T; J++, q++) {
-      const int b = 32 * J + row;
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int a = 32 * I + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (a >= A.nin || b >= A.nin) continue;
-        float2* p = Vout + ((long long)a * A.nin + b) * A.nchan + c;
-        float2 v = make_float2(accR[q][r], accI[q][r]);
-        if (A.accumulate) { float2 o = *p; v.x += o.x; v.y += o.y; }
-        *p = v;
-      }
+    for (int r = 0; r < 16; r++) {
+      const int a = input_of(32 * TL::I(k) + 8 * (r >> 2) + 4 * half + (r & 3));
+      if (a >= A.nin || b >= A.nin) continue;
+      float2* ptr = Vout + ((long long)a * A.nin + b) * A.nchan + c;
+      float2 v = make_float2(accR[k][r], accI[k][r]);
+      if (A.accumulate) { float2 o = *ptr; v.x += o.x; v.y += o.y; }
+      *ptr = v;
     }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -397,6 +452,7 @@ hipError_t launch_xmac_fp8(const XmacArgs& a, hipStream_t stream) {
 }
 
 bool xmac_mfma_supported(int nin, int nchan) { return nin <= 64 && nchan % 16 == 0; }
+bool xmac_mfma_streaming_supported(int nin, int nchan) { return nin <= kXmacMfmaMaxInputs && nchan % 16 == 0; }
 
 bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host) {
   if (a.block_stride % 4) return false;
@@ -406,17 +462,29 @@ bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host) {
 }
 
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream) {
-  if (!xmac_mfma_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
-  dim3 grid((unsigned)(a.nchan / 16), (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks));
+  if (!xmac_mfma_streaming_supported(a.nin, a.nchan)) return hipErrorInvalidValue;
+  const unsigned gx = (unsigned)(a.nchan / 16), gy = (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks);
   const bool bf = prec == FXG_XMAC_BF16;
   const uint4* z = (const uint4*)zero16;
+#define FX_DL(T, NZ)                                                                                           \
+  do {                                                                                                         \
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma_dl<T, true, false>), dim3(gx, gy, NZ), dim3(1024), 0, stream, a, z); \
+    else hipLaunchKernelGGL((k_xmac_mfma_dl<T, false, false>), dim3(gx, gy, NZ), dim3(1024), 0, stream, a, z);   \
+  } while (0)
   if (a.nin <= 32) {
-    if (bf) hipLaunchKernelGGL((k_xmac_mfma_dl<1, true>), grid, dim3(1024), 0, stream, a, z);
-    else hipLaunchKernelGGL((k_xmac_mfma_dl<1, false>), grid, dim3(1024), 0, stream, a, z);
+    FX_DL(1, 1);
+  } else if (a.nin <= 64) {
+    FX_DL(2, 1);
   } else {
-    if (bf) hipLaunchKernelGGL((k_xmac_mfma_dl<2, true>), grid, dim3(1024), 0, stream, a, z);
-    else hipLaunchKernelGGL((k_xmac_mfma_dl<2, false>), grid, dim3(1024), 0, stream, a, z);
+    // every chunk of 64 inputs with itself, then every pair of chunks (8
+    // channels per workgroup)
+    const unsigned nch = (unsigned)((a.nin + 63) / 64);
+    FX_DL(2, nch);
+    const dim3 gc(2 * gx, gy, nch * (nch - 1) / 2);
+    if (bf) hipLaunchKernelGGL((k_xmac_mfma_dl<4, true, true>), gc, dim3(512), 0, stream, a, z);
+    else hipLaunchKernelGGL((k_xmac_mfma_dl<4, false, true>), gc, dim3(512), 0, stream, a, z);
   }
+#undef FX_DL
   return hipGetLastError();
 }
 

@@ -435,6 +435,7 @@ def load_host_lib():
     lib.fxs_submit.argtypes = [C.c_void_p, C.c_int, P(SubintDesc), P(StationInput), C.c_void_p, P(C.c_int64)]
     lib.fxs_wait.argtypes = [C.c_void_p, C.c_int64, P(C.c_int64)]
     lib.fxs_stats.argtypes = [C.c_void_p, P(ServeStats)]
+    lib.fxs_set_max_inflight.argtypes = [C.c_void_p, C.c_int]
     lib.fxds_data_bytes.restype = C.c_int64
     lib.fxds_data_bytes.argtypes = [P(JobDesc), C.c_int, C.c_double]
     lib.fxds_format_of.argtypes = [P(JobDesc), C.c_int, C.c_double, P(DsFormat)]

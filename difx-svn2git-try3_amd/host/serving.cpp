@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -51,7 +52,11 @@ struct fxs_server {
   int device = 0;
   int64_t budget = 0, used = 0;
   std::mutex m;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_gpu;
+  // requests a slot has taken, waiting to start on the device: they start
+  // in arrival (id) order, at most max_inflight at a time (0: no limit)
+  std::set<int64_t> ready;
+  int max_inflight = 0, inflight = 0;
   bool stop = false;
   std::vector<std::unique_ptr<Config>> cfgs;
   std::vector<std::thread> workers;
@@ -63,7 +68,9 @@ struct fxs_server {
 
 namespace {
 
-// worker of one slot: takes its configuration's requests in arrival order
+// worker of one slot: takes its configuration's requests in arrival order,
+// then starts each when it is the oldest request taken by any slot and the
+// device has room for one more (max_inflight)
 void slot_loop(fxs_server* s, Config* c, fxg_job* job) {
   for (;;) {
     Request* r = nullptr;
@@ -75,7 +82,14 @@ void slot_loop(fxs_server* s, Config* c, fxg_job* job) {
       c->queue.pop_front();
       c->busy++;
       s->st.queued--;
+      s->ready.insert(r->id);
+      s->cv_gpu.wait(lk, [&] {
+        return (s->max_inflight <= 0 || s->inflight < s->max_inflight) && *s->ready.begin() == r->id;
+      });
+      s->ready.erase(r->id);
+      s->inflight++;
     }
+    s->cv_gpu.notify_all();                         // the next oldest may start too
     int rc = fxg_subint_process(job, &r->sub, r->st.data(), 1, r->results, 1);
     {
       std::lock_guard<std::mutex> lk(s->m);
@@ -83,8 +97,10 @@ void slot_loop(fxs_server* s, Config* c, fxg_job* job) {
       r->rc = rc;
       r->finished = true;
       c->busy--;
+      s->inflight--;
       s->st.completed++;
     }
+    s->cv_gpu.notify_all();
     s->cv_done.notify_all();
   }
 }
@@ -274,6 +290,16 @@ int fxs_wait(fxs_server* s, int64_t request, int64_t* latency_ns) {
   s->reqs.erase(it);
   delete r;
   return rc;
+}
+
+int fxs_set_max_inflight(fxs_server* s, int n) {
+  if (!s || n < 0) return FXG_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->max_inflight = n;
+  }
+  s->cv_gpu.notify_all();
+  return 0;
 }
 
 int fxs_stats(fxs_server* s, fxs_stats_t* out) {

@@ -50,6 +50,12 @@ int fxs_submit(fxs_server* s, int config_id, const fxg_subint_desc* sub,
                const fxg_station_input* stations, float* results_device, int64_t* request);
 /* Blocks until the request completes; its latency in ns (may be NULL). */
 int fxs_wait(fxs_server* s, int64_t request, int64_t* latency_ns);
+/* At most n requests on the device at once (0: no limit, the default);
+ * requests start in submission order among those a slot has taken, so a
+ * bound trades overlap for first-come service of large requests.  No
+ * reference counterpart (FxManager hands every free Core a subintegration,
+ * fxmanager.cpp:306-342). */
+int fxs_set_max_inflight(fxs_server* s, int n);
 int fxs_stats(fxs_server* s, fxs_stats_t* out);
 
 #ifdef __cplusplus

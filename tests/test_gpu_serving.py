@@ -3,8 +3,10 @@ request loop (include/fxcorr_serve.h) on the GPU.
 
   - a submitted subintegration gives the same results as fxg_subint_process;
   - several configurations of different sizes, with requests interleaved so
-    that work of different configurations and slots overlaps: every result
-    equals the result of the same subintegration processed alone, bit for bit;
+    that work of different configurations and slots overlaps (with no bound
+    on the requests on the device at once, and with one or two started in
+    submission order): every result equals the result of the same
+    subintegration processed alone, bit for bit;
   - admission by HBM footprint: a configuration that does not fit the budget
     is refused, and a configuration's slots never exceed its limit or the
     budget.
@@ -86,7 +88,8 @@ def test_submit_wait_equals_process(gpu):
 
 
 @pytest.mark.gpu
-def test_results_alone_equal_results_in_the_mix(gpu):
+@pytest.mark.parametrize("inflight", [0, 1, 2])
+def test_results_alone_equal_results_in_the_mix(gpu, inflight):
     lib, ctx = gpu
     host = fxcorr.load_host_lib()
     jobs, alone, subs, lens = [], [], [], []
@@ -105,6 +108,7 @@ def test_results_alone_equal_results_in_the_mix(gpu):
     srv = C.c_void_p()
     check(host, host.fxs_create(0, 0, C.byref(srv)), "fxs_create")
     try:
+        check(host, host.fxs_set_max_inflight(srv, inflight), "max_inflight")
         cids = []
         for job, keep in jobs:
             cid = C.c_int()

@@ -195,7 +195,7 @@ def test_host_library_exports_serving_abi():
     host = fxcorr.load_host_lib()
     hdr = open(os.path.join(REPO, "include", "fxcorr_serve.h")).read()
     syms = sorted(set(re.findall(r"^\s*int\s+(fxs_\w+)\s*\(", hdr, re.M)))
-    assert len(syms) == 6
+    assert len(syms) == 7
     for s in syms:
         assert hasattr(host, s), s
 
