@@ -1104,12 +1104,16 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
                         (big ? !fp8 && xmac_mfma_streaming_supported(xa.nin, xa.nchan) &&
                                    xmac_mfma_streaming_ok(xa, xf.in_off.data())
                              : xmac_mfma_supported(xa.nin, xa.nchan));
+      // FP32 spectra: the FP32 matrix cores under the streamed schedule, the
+      // VALU register tiles otherwise
+      const bool mfma32 = prec == FXG_XMAC_FP32 && j->xmac_streaming && xmac_mfma_f32_ok(xa, xf.in_off.data());
       if (fp8) { xa.cexp = j->d_cexp.as<signed char>(); xa.eoff = j->d_eoff.as<int>() + j->inoff_base[x]; }
       float2* V = j->d_V.as<float2>() + xf.voff;
       // few channel groups: split the blocks over grid rows too, into partial
       // matrices summed afterwards in row order
       const long long vsz = (long long)xa.nin * xa.nin * xa.nchan;
-      const int nsplit = xmac_splits(mfma ? xa.nchan / 16 : (xa.nchan + xa.CT - 1) / xa.CT, cn, vsz, P.vpart_elems);
+      const int nsplit =
+          xmac_splits(mfma || mfma32 ? xa.nchan / 16 : (xa.nchan + xa.CT - 1) / xa.CT, cn, vsz, P.vpart_elems);
       xa.split_blocks = (cn + nsplit - 1) / nsplit;
       const int rows = (cn + xa.split_blocks - 1) / xa.split_blocks;
       if (rows > 1) { xa.V = j->d_vpart.as<float2>(); xa.vsplit = vsz; xa.accumulate = 0; }
@@ -1117,6 +1121,8 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
       if ((rc = j->ev_begin(FXG_STAGE_XMAC))) return rc;
       if (fp8) {
         HIPCHK(launch_xmac_fp8(xa, s));
+      } else if (mfma32) {
+        HIPCHK(launch_xmac_mfma_f32(xa, j->d_zero.p, s));
       } else if (mfma) {
         if ((j->xmac_streaming || big) && xmac_mfma_streaming_ok(xa, xf.in_off.data()))
           HIPCHK(launch_xmac_mfma_streaming(xa, prec, j->d_zero.p, s));
@@ -1271,10 +1277,12 @@ static std::vector<fxg_schedule> xmac_candidates(const fxg_job* j, const fxg_sch
     for (int st = 0; st < 2; st++) { fxg_schedule c = base; c.xmac_streaming = st; out.push_back(c); }
   }
   if (f32) {
+    // the FP32 matrix cores (streamed schedule), then the VALU register tiles
+    if (j->desc.xmac_precision == FXG_XMAC_FP32) { fxg_schedule c = base; c.xmac_streaming = 1; out.push_back(c); }
     for (int ct : {0, 4, 8, 16})
       for (int tb : {0, 4, 8, 16}) {
         if ((ct == 0) != (tb == 0)) continue;
-        fxg_schedule c = base; c.xmac_ct = ct; c.xmac_tb = tb; out.push_back(c);
+        fxg_schedule c = base; c.xmac_streaming = 0; c.xmac_ct = ct; c.xmac_tb = tb; out.push_back(c);
       }
   }
   if (out.empty()) out.push_back(base);

@@ -144,6 +144,10 @@ bool xmac_mfma_streaming_ok(const XmacArgs& a, const long long* in_off_host);
 constexpr int kXmacMfmaMaxInputs = 512;
 bool xmac_mfma_streaming_supported(int nin, int nchan);
 hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* zero16, hipStream_t stream);
+// FP32 spectra (row-major) on the FP32 matrix cores: nchan % 16 == 0, even
+// input offsets and block stride
+bool xmac_mfma_f32_ok(const XmacArgs& a, const long long* in_off_host);
+hipError_t launch_xmac_mfma_f32(const XmacArgs& a, const void* zero16, hipStream_t stream);
 // V[i] (+)= sum over splits of P[s][i], i < n, in split order
 hipError_t launch_xmac_reduce(const float2* P, int nsplit, long long vsplit, float2* V, long long n, int accumulate,
                               hipStream_t stream);

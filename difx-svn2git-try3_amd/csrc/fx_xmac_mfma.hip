@@ -303,6 +303,132 @@ __global__ __launch_bounds__(CROSS ? 512 : 1024) void k_xmac_mfma_dl(XmacArgs A,
 }
 
 // ---------------------------------------------------------------------------
+// FP32 on the matrix cores: v_mfma_f32_32x32x2_f32 takes one block per
+// instruction (K = 2: lanes 0-31 give the real part of their row's input,
+// lanes 32-63 the imaginary part), so with the same tiles as above
+//     Re V += A A^T, Im V += Ah A^T,  Ah = (im, -re)
+// in FP32 products and FP32 accumulation.  The spectra are row-major
+// [band][block][channel] float2; a stage holds 8 blocks of CG channels of
+// every row in LDS ([row][block][CG float2], rows padded by 4 dwords),
+// filled by global_load_lds (one 16-byte piece, two channels, per lane) and
+// double-buffered: the matrix cores need ~5 us per stage, the loads of the
+// next one far less.  Rows and chunk pairs as in k_xmac_mfma_dl.
+// ---------------------------------------------------------------------------
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+template <int T, bool CROSS>
+__global__ __launch_bounds__(CROSS ? 512 : 1024) void k_xmac_mfma_f32(XmacArgs A, const uint4* __restrict__ zero16) {
+  static_assert(!CROSS || T == 4, "a cross workgroup holds two 64-input chunks");
+  constexpr int CG = CROSS ? 8 : 16;     // channels (wavefronts) per workgroup
+  constexpr int TT = 8;                  // blocks per stage
+  constexpr int NR = 32 * T;
+  using TL = DlTiles<T, CROSS>;
+  constexpr int NTILE = TL::N;
+  constexpr int RBW = 2 * CG;            // dwords per (row, block)
+  constexpr int SDW = TT * RBW + 4;      // dwords per row per stage
+  constexpr int STAGE = NR * SDW;
+  constexpr int LLANES = TT * CG / 2;    // 16-byte pieces per row-stage
+  constexpr int LPW = NR / CG;           // rows each wavefront loads
+  __shared__ __align__(16) unsigned lds[2 * STAGE];
+
+  const char* spec = (const char*)A.spec;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int nwg = gridDim.x;
+  const int bid = blockIdx.x;
+  const int grp = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;   // XCD-aware groups
+  const int c0 = grp * CG;
+  const int half = l >> 5, row = l & 31;
+  const int lt = l / (CG / 2), lp = l % (CG / 2);   // load lane: block and 2-channel piece
+  const int sb0 = blockIdx.y * A.split_blocks;
+  const int nb = min(A.nblocks - sb0, A.split_blocks);
+  float2* const Vout = A.V + (long long)blockIdx.y * A.vsplit;
+  int p = blockIdx.z, q = blockIdx.z;
+  if (CROSS) {
+    const int nch = (A.nin + 63) >> 6;
+    int rem = blockIdx.z;
+    p = 0;
+    while (rem >= nch - 1 - p) { rem -= nch - 1 - p; p++; }
+    q = p + 1 + rem;
+  }
+  auto input_of = [&](int r) { return r < 64 ? 64 * p + r : 64 * q + r - 64; };
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  long long ioff[LPW];
+#pragma unroll
+  for (int u = 0; u < LPW; u++) {
+    const int a = input_of(wu * LPW + u);
+    ioff[u] = a < A.nin ? A.in_off[a] + c0 : -1;
+  }
+  auto issue = [&](int stage, int buf) {
+    if (l < LLANES) {
+      const int t = stage * TT + lt;
+#pragma unroll
+      for (int u = 0; u < LPW; u++) {
+        const void* src = zero16;
+        if (ioff[u] >= 0 && t < nb) src = spec + 8 * (ioff[u] + (long long)(sb0 + t) * A.block_stride + 2 * lp);
+        __builtin_amdgcn_global_load_lds(
+            src, (__attribute__((address_space(3))) void*)(lds + buf * STAGE + (wu * LPW + u) * SDW), 16, 0, 0);
+      }
+    }
+  };
+  auto barrier = [] {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+
+  f32x16_t accR[NTILE], accI[NTILE];
+#pragma unroll
+  for (int k = 0; k < NTILE; k++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) { accR[k][e] = 0.0f; accI[k][e] = 0.0f; }
+
+  const int nstage = (nb + TT - 1) / TT;
+  issue(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  barrier();
+  for (int sg = 0; sg < nstage; sg++) {
+    const int buf = sg & 1;
+    if (sg + 1 < nstage) issue(sg + 1, buf ^ 1);
+    const float* base = (const float*)(lds + buf * STAGE) + 2 * w;   // this wavefront's channel
+    const int nt = min(TT, nb - sg * TT);
+    for (int t = 0; t < nt; t++) {
+      float f[T], fh[T];
+#pragma unroll
+      for (int I = 0; I < T; I++) {
+        const float* ptr = base + (32 * I + row) * SDW + t * RBW;
+        f[I] = ptr[half];                               // re (lanes 0-31), im (32-63)
+        const float o = ptr[1 - half];
+        fh[I] = half ? -o : o;                          // im, -re
+      }
+#pragma unroll
+      for (int k = 0; k < NTILE; k++) {
+        accR[k] = __builtin_amdgcn_mfma_f32_32x32x2f32(f[TL::I(k)], f[TL::J(k)], accR[k], 0, 0, 0);
+        accI[k] = __builtin_amdgcn_mfma_f32_32x32x2f32(fh[TL::I(k)], f[TL::J(k)], accI[k], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);       // this wavefront's loads of the next stage have landed
+    barrier();                           // everyone's have, and nobody still reads this stage
+  }
+
+  const int c = c0 + w;
+  if (c >= A.nchan) return;
+#pragma unroll
+  for (int k = 0; k < NTILE; k++) {
+    const int b = input_of(32 * TL::J(k) + row);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int a = input_of(32 * TL::I(k) + 8 * (r >> 2) + 4 * half + (r & 3));
+      if (a >= A.nin || b >= A.nin) continue;
+      float2* ptr = Vout + ((long long)a * A.nin + b) * A.nchan + c;
+      float2 v = make_float2(accR[k][r], accI[k][r]);
+      if (A.accumulate) { float2 o = *ptr; v.x += o.x; v.y += o.y; }
+      *ptr = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP8 variant: spectra stored as e4m3 (re, im) byte pairs, each (band,
 // channel) scaled by 2^e in the F stage; v_mfma_scale_f32_32x32x64_f8f6f4
 // takes 32 blocks per step (K = 64) and applies 2^-e per input row and
@@ -485,6 +611,29 @@ hipError_t launch_xmac_mfma_streaming(const XmacArgs& a, int prec, const void* z
     else hipLaunchKernelGGL((k_xmac_mfma_dl<4, false, true>), gc, dim3(512), 0, stream, a, z);
   }
 #undef FX_DL
+  return hipGetLastError();
+}
+
+bool xmac_mfma_f32_ok(const XmacArgs& a, const long long* in_off_host) {
+  if (a.nin > kXmacMfmaMaxInputs || a.nchan % 16 || a.block_stride % 2) return false;
+  for (int i = 0; i < a.nin; i++)
+    if (in_off_host[i] % 2) return false;      // 16-byte aligned channel pairs
+  return true;
+}
+
+hipError_t launch_xmac_mfma_f32(const XmacArgs& a, const void* zero16, hipStream_t stream) {
+  const unsigned gx = (unsigned)(a.nchan / 16), gy = (unsigned)((a.nblocks + a.split_blocks - 1) / a.split_blocks);
+  const uint4* z = (const uint4*)zero16;
+  if (a.nin <= 32) {
+    hipLaunchKernelGGL((k_xmac_mfma_f32<1, false>), dim3(gx, gy), dim3(1024), 0, stream, a, z);
+  } else if (a.nin <= 64) {
+    hipLaunchKernelGGL((k_xmac_mfma_f32<2, false>), dim3(gx, gy), dim3(1024), 0, stream, a, z);
+  } else {
+    const unsigned nch = (unsigned)((a.nin + 63) / 64);
+    hipLaunchKernelGGL((k_xmac_mfma_f32<2, false>), dim3(gx, gy, nch), dim3(1024), 0, stream, a, z);
+    hipLaunchKernelGGL((k_xmac_mfma_f32<4, true>), dim3(2 * gx, gy, nch * (nch - 1) / 2), dim3(512), 0, stream, a,
+                       z);
+  }
   return hipGetLastError();
 }
 

@@ -346,9 +346,12 @@ int fxg_job_set_profiling(fxg_job* job, int on);
  * reset != 0 clears the accumulators after reading. */
 int fxg_job_stage_times(fxg_job* job, double* ms, int64_t* launches, int reset);
 
-/* Reduced-precision XMAC staging: 1 (default) streams the spectra from HBM
- * into LDS with direct loads, double-buffered, where the layout allows;
- * 0 stages them through registers. */
+/* XMAC staging: 1 (default) streams the spectra from HBM into LDS with
+ * direct loads where the layout allows (reduced precision: the ring of
+ * stages; FP32: the FP32 matrix cores, 16-channel multiples); 0 stages them
+ * through registers (reduced precision: the register-staged matrix-core
+ * kernel up to 64 inputs; FP32: the VALU register tiles, xmac_ct/xmac_tb).
+ * Above 64 inputs reduced precision always streams. */
 int fxg_job_set_xmac_streaming(fxg_job* job, int on);
 
 /* ---- VDIF thread multiplexing (libraries/vdifio/src/vdifmux.c) ----------
@@ -412,8 +415,8 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* desc, const uint8_t* s
  * by a different pass length agree to rounding. */
 typedef struct {
   int32_t xmac_streaming;  /* 0 register staging, 1 direct-to-LDS streaming    */
-  int32_t xmac_ct;         /* FP32 XMAC channels per workgroup, 0 = automatic  */
-  int32_t xmac_tb;         /* FP32 XMAC blocks per LDS stage, 0 = automatic    */
+  int32_t xmac_ct;         /* FP32 VALU XMAC channels per workgroup, 0 = auto  */
+  int32_t xmac_tb;         /* FP32 VALU XMAC blocks per LDS stage, 0 = auto    */
   int32_t chunk_blocks;    /* blocks per F/X pass, 0 = the plan's chunk_blocks */
 } fxg_schedule;
 

@@ -335,6 +335,31 @@ def test_xmac_streaming_equals_register_staging(gpu, prec, mk):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
+@pytest.mark.parametrize("mk", ["c1", "c2", "wide", "x80"])
+def test_fp32_xmac_matrix_cores_and_valu_tiles(gpu, oracle, mk):
+    """FP32 spectra cross-multiplied on the FP32 matrix cores (the streamed
+    schedule, the default) and on the VALU register tiles (streaming off):
+    both within 1e-5 of the FP32 oracle, cross products and
+    autocorrelations, for 8, 20, 40 and 80 inputs (the last: a 64-input
+    chunk, a partial chunk and their cross pair) over 37 blocks."""
+    lib, ctx = gpu
+    mkspec = {"c1": c1_spec, "c2": c2_spec, "wide": wide_spec,
+              "x80": lambda **kw: JobSpec(nstations=40, nsubbands=1, npol=2, bw_mhz=16.0, nchan=128, **kw)}[mk]
+    spec = mkspec(blockspersend=37)
+    job, keep = spec.build()
+    sd = make_data(spec, invalid_frames={0: [1]})
+    ref = oracle_results(oracle, job, sd, nthreads=8)
+    info = plan_info(oracle, "orc_", job)
+    xend, a0, a1 = region_slices(oracle, job, info)
+    for on in (1, 0):
+        g = GpuJob(lib, ctx, job)
+        check(lib, lib.fxg_job_set_xmac_streaming(g.h, on), "streaming")
+        got = g.process(sd)
+        g.close()
+        assert rel_err(got[:xend], ref[:xend]) < FP32_TOL, (on, rel_err(got[:xend], ref[:xend]))
+        assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL, on
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("nst", [16, 32])
 def test_xmac_streaming_many_stages(gpu, prec, nst):
