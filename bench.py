@@ -232,7 +232,8 @@ def roofline(spec, info, stage, launch, precision):
     samples = spec.nstations * spec.nrecbands * nblk * L
     if dom == "fstage":
         avg_s = stage["fstage"]["ms"] / stage["fstage"]["launches"] / 1e3
-        kname = f"fxg::k_fstage<{L}, 2>"
+        ppt = launch["schedule"].get("fft_ppt") or 16
+        kname = f"fxg::k_fstage<{L}, 2, {ppt}>"
         # 2-bit input read once + spectra written once
         nbytes = samples * 0.25 + spec.nstations * spec.nrecbands * nblk * C_ * specbytes
         # FFT 5 log2 L, fringe rotation 6, fractional-sample rotation 3 flops per sample
@@ -258,7 +259,7 @@ def roofline(spec, info, stage, launch, precision):
     nbytes = nin * C_ * nblk * specbytes * spec.nsubbands / nfreq
     peak = PEAK_TFLOPS[precision]
     bf = "true" if precision == "bf16" else "false"
-    kname = (f"fxg::k_xmac_mfma_dl<2, {bf}>" if launch["schedule"]["xmac_streaming"]
+    kname = (f"fxg::k_xmac_mfma_dl<2, {bf}, false>" if launch["schedule"]["xmac_streaming"]
              else f"fxg::k_xmac_mfma<2, {bf}, 16>")
     tf = flops / avg_s / 1e12
     hbm_frac = nbytes / avg_s / 1e9 / HBM_PEAK_GBS
