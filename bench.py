@@ -232,8 +232,7 @@ def roofline(spec, info, stage, launch, precision):
     samples = spec.nstations * spec.nrecbands * nblk * L
     if dom == "fstage":
         avg_s = stage["fstage"]["ms"] / stage["fstage"]["launches"] / 1e3
-        ppt = launch["schedule"].get("fft_ppt") or 16
-        kname = f"fxg::k_fstage<{L}, 2, {ppt}>"
+        kname = f"fxg::k_fstage<{L}, 2>"
         # 2-bit input read once + spectra written once
         nbytes = samples * 0.25 + spec.nstations * spec.nrecbands * nblk * C_ * specbytes
         # FFT 5 log2 L, fringe rotation 6, fractional-sample rotation 3 flops per sample

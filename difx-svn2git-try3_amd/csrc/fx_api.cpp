@@ -155,7 +155,6 @@ struct fxg_job {
   DBuf d_iota;                    // 0..nbands-1: band lists of single datastreams
   DBuf d_zero;                    // 64 zero bytes: the rows past the data for streaming loads
   int xmac_streaming = 1;         // direct-to-LDS MFMA XMAC where the layout allows
-  int fft_ppt = 0;                // points per F-stage FFT thread (0: automatic)
   int xmac_ct = 0, xmac_tb = 0;   // FP32 XMAC channel tile / LDS stage (0: automatic)
   int chunk_blocks = 0;           // blocks per F/X pass (<= the plan's; 0: the plan's)
   // the subintegration as one graph launch, captured per launch shape
@@ -529,13 +528,6 @@ int fxg_unpack_noalloc(fxg_ctx* c, const fxg_datastream* ds, const uint8_t* data
 // ---------------------------------------------------------------------------
 // jobs
 // ---------------------------------------------------------------------------
-// points per F-stage FFT thread: the schedule's choice where the transform
-// has that variant, else 16
-static int fstage_ppt(const fxg_job* j) {
-  const int want = j->fft_ppt ? j->fft_ppt : 16;
-  return fstage_ppt_supported(j->plan.L, j->plan.st[0].cplx, want) ? want : 16;
-}
-
 static void job_free(fxg_job* j) {
   DBuf* bufs[] = {&j->d_st, &j->d_bands, &j->d_segs, &j->d_wterms, &j->d_fblank, &j->d_pbw, &j->d_inoff, &j->d_tw,
                   &j->d_lut8,
@@ -1081,11 +1073,11 @@ static int launch_back(fxg_job* j, float* dres, hipStream_t s) {
     if ((rc = j->ev_begin(FXG_STAGE_FSTAGE))) return rc;
     if (j->nb2) {
       fa.bandlist = j->d_bandlist.as<int>();
-      HIPCHK(launch_fstage(P.L, 2, P.st[0].cplx, fstage_ppt(j), fa, j->nb2, cn, s));
+      HIPCHK(launch_fstage(P.L, 2, P.st[0].cplx, fa, j->nb2, cn, s));
     }
     if (j->nb8) {
       fa.bandlist = j->d_bandlist.as<int>() + j->nb2;
-      HIPCHK(launch_fstage(P.L, 8, P.st[0].cplx, fstage_ppt(j), fa, j->nb8, cn, s));
+      HIPCHK(launch_fstage(P.L, 8, P.st[0].cplx, fa, j->nb8, cn, s));
     }
     if ((rc = j->ev_end())) return rc;
     if (j->kurt_on) {
@@ -1186,7 +1178,7 @@ static std::vector<unsigned char> graph_key(const fxg_job* j, const LaunchShape&
   const void* ptrs[4] = {j->d_ss.p, j->d_flags.p, j->d_ftab.p, j->d_ct.p};
   put(ptrs, sizeof(ptrs));
   const int sched[8] = {j->xmac_streaming, j->xmac_ct, j->xmac_tb, j->chunk_blocks,
-                        j->kurt_on, j->kurt_maxch, j->kurt_nrec, fstage_ppt(j)};
+                        j->kurt_on, j->kurt_maxch, j->kurt_nrec, 0};
   put(sched, sizeof(sched));
   const void* kptrs[3] = {j->d_kurt.p, j->d_kurtout.p, j->d_kurtnz.p};
   put(kptrs, sizeof(kptrs));
@@ -1253,13 +1245,10 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
 int fxg_job_set_schedule(fxg_job* j, const fxg_schedule* sc) {
   if (!j || !sc) return set_err(FXG_EINVAL, "null argument");
   if (sc->xmac_ct < 0 || sc->xmac_tb < 0 || sc->chunk_blocks < 0) return set_err(FXG_EINVAL, "negative schedule entry");
-  if (sc->fft_ppt != 0 && sc->fft_ppt != 16 && sc->fft_ppt != 32)
-    return set_err(FXG_EINVAL, "points per FFT thread must be 0, 16 or 32");
   j->xmac_streaming = sc->xmac_streaming ? 1 : 0;
   j->xmac_ct = sc->xmac_ct;
   j->xmac_tb = sc->xmac_tb;
   j->chunk_blocks = sc->chunk_blocks;
-  j->fft_ppt = sc->fft_ppt;
   return 0;
 }
 
@@ -1269,7 +1258,6 @@ int fxg_job_get_schedule(const fxg_job* j, fxg_schedule* sc) {
   sc->xmac_ct = j->xmac_ct;
   sc->xmac_tb = j->xmac_tb;
   sc->chunk_blocks = j->chunk_blocks;
-  sc->fft_ppt = j->fft_ppt;
   return 0;
 }
 
@@ -1298,16 +1286,6 @@ static std::vector<fxg_schedule> xmac_candidates(const fxg_job* j, const fxg_sch
       }
   }
   if (out.empty()) out.push_back(base);
-  return out;
-}
-
-// the F stage's FFT decomposition: 16 or 32 points per thread
-static std::vector<fxg_schedule> fft_candidates(const fxg_job* j, const fxg_schedule& base) {
-  std::vector<fxg_schedule> out;
-  for (int ppt : {16, 32}) {
-    if (!fstage_ppt_supported(j->plan.L, j->plan.st[0].cplx, ppt)) continue;
-    fxg_schedule c = base; c.fft_ppt = ppt; out.push_back(c);
-  }
   return out;
 }
 
@@ -1365,10 +1343,8 @@ int fxg_job_autotune(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_i
     r->pad = 0;
     return 0;
   };
-  for (int phase = 0; phase < 3 && !rc; phase++) {
-    std::vector<fxg_schedule> cand = phase == 0   ? xmac_candidates(j, best)
-                                     : phase == 1 ? fft_candidates(j, best)
-                                                  : chunk_candidates(j, best);
+  for (int phase = 0; phase < 2 && !rc; phase++) {
+    std::vector<fxg_schedule> cand = phase == 0 ? xmac_candidates(j, best) : chunk_candidates(j, best);
     for (const fxg_schedule& c : cand) {
       fxg_tune_result r{};
       if ((rc = run(c, &r))) break;
@@ -1632,7 +1608,7 @@ int fxg_station_process_block(fxg_job* j, const fxg_subint_desc* sub, const fxg_
   fill_fstage_args(j, &fa);
   fa.chunk0 = block;
   fa.bandlist = j->d_iota.as<int>() + P.st[ds].band0;
-  HIPCHK(launch_fstage(P.L, P.st[ds].nbits, P.st[ds].cplx, fstage_ppt(j), fa, P.st[ds].nchan, 1, j->stream));
+  HIPCHK(launch_fstage(P.L, P.st[ds].nbits, P.st[ds].cplx, fa, P.st[ds].nchan, 1, j->stream));
   HIPCHK(hipStreamSynchronize(j->stream));
   j->ev_collect();
   j->have_subint = 1;

@@ -129,10 +129,7 @@ hipError_t launch_cornerturn(const DevStation* st, const DevStationSub* ss, unsi
 hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const int* flags,
                            const int2* ftab, int nblocks, DevBlock* out, int nst, hipStream_t stream);
 // cplx: complex samples (L = nchan, every output of the transform kept)
-// ppt: points per FFT thread, 16 or (real samples, L 4096..16384) 32
-bool fstage_ppt_supported(int L, int cplx, int ppt);
-hipError_t launch_fstage(int L, int nbits, int cplx, int ppt, const FStageArgs& a, int nbands, int nblocks,
-                         hipStream_t stream);
+hipError_t launch_fstage(int L, int nbits, int cplx, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream);
 hipError_t launch_unpack(const DevStation* st, const DevStationSub* ss, const int2* ftab, const float* lut8,
                          long long first, int nsamp, int nchan, long long validend, float* out,
                          hipStream_t stream);

@@ -3,7 +3,7 @@
 //               format_vdif.c:4009-4060, blanker_mark5.c:53-165, 282-312)
 //   k_prologue  FP64 block geometry + data weight      (mode.cpp:649-813,
 //               mk5mode.cpp:104-177)
-//   k_fstage<L, NBITS, PPT> LUT unpack -> fringe rotation -> L-point FP32 Stockham FFT
+//   k_fstage<L> LUT unpack -> fringe rotation -> L-point FP32 Stockham FFT
 //               -> sideband select -> fractional-sample rotation -> spectra
 //               (mode.cpp:909-1310)
 //   k_unpack    standalone Mk5Mode::unpack equivalent
@@ -373,19 +373,15 @@ __device__ __forceinline__ void dft16_from_halves(const float2* sm, const float2
 // L = 2^LOG = 16^a * 2^q.  Pass 1 is radix 16 (Ns = 1, straight from the
 // decoder), then NMID radix-16 passes, then a last pass of radix RL = 2^q (16
 // when q = 0) that computes only the half of the bins the sideband keeps.
-// Every FFT thread holds PPT points in every pass: HB = PPT / 16 butterflies
-// j = tid + h NT of each radix-16 pass (16 points: one workgroup of L/16
-// threads; 32 points: half the wavefronts, so half the per-wavefront set-up,
-// and the two butterflies of a middle pass share one twiddle set).
-template <int L, int PPT = 16> struct FftShape {
+// Every FFT thread holds 16 points in every pass.
+template <int L> struct FftShape {
   static constexpr int LOG = (L == 128) ? 7 : (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10
                            : (L == 2048) ? 11 : (L == 4096) ? 12 : (L == 8192) ? 13 : 14;
   static constexpr int Q = LOG % 4;
   static constexpr int RL = Q ? (1 << Q) : 16;                    // last-pass radix
   static constexpr int NMID = Q ? LOG / 4 - 1 : LOG / 4 - 2;      // radix-16 passes between
-  static constexpr int NTF = L / 16;                              // radix-16 butterflies per pass
-  static constexpr int HB = PPT / 16;                             // butterflies per thread
-  static constexpr int NT = NTF / HB < 64 ? 64 : NTF / HB;        // block size
+  static constexpr int NTF = L / 16;                              // FFT threads
+  static constexpr int NT = NTF < 64 ? 64 : NTF;                  // block size
   static constexpr int PAD = L + L / 16;                          // padded LDS length
   static constexpr int NSL = L / RL;                              // Ns of the last pass
   static constexpr int TABS = fxg_rotator_table_entries(L);        // rotator sub/step/quad points
@@ -709,10 +705,10 @@ __device__ __forceinline__ void last_pass_r2(const float2* buf, const float2* tw
   }
 }
 
-template <int L, int NBITS, int PPT>
-__global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A) {
-  using Sh = FftShape<L, PPT>;
-  constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL, HB = Sh::HB;
+template <int L, int NBITS>
+__global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
+  using Sh = FftShape<L>;
+  constexpr int NT = Sh::NT, NTF = Sh::NTF, RL = Sh::RL, NSL = Sh::NSL;
   constexpr int RAWW = (L * NBITS) / 32 + 2;    // staged stream words of one block
   // 8-bit samples at the largest L: the staged stream words share the tail of
   // the FFT buffer (pass 1 reads them all before its first write, below)
@@ -799,12 +795,9 @@ __global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A)
   const bool rotfast = slog >= 0 && (NTF & (s - 1)) == 0;
 
   // ---- pass 1 (Ns = 1): decode, fringe-rotate, radix-16 DFT, to LDS
-  float2 vh[HB][16];
-#pragma unroll
-  for (int h = 0; h < HB; h++) {
-    const int j = tid + h * NT;
-    if (j >= NTF) continue;
-    float2* v = vh[h];
+  float2 v[16];
+  if (tid < NTF) {
+    const int j = tid;
     bool transformed = false;                  // the branch ran the 16-point DFT itself
     if (kWordStep && fast && rotfast && S.order == 1) {
       const int bp = bitoff + NBITS * j;
@@ -893,13 +886,10 @@ __global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A)
     if (!transformed) DFT<16>::run(v);
   }
   if (kAliasRaw) __syncthreads();            // every stream word read before buf is written
+  if (tid < NTF) {
+    float2* o = buf + 17 * tid;              // pidx(16 j + r) = 17 j + r
 #pragma unroll
-  for (int h = 0; h < HB; h++) {
-    const int j = tid + h * NT;
-    if (j >= NTF) continue;
-    float2* o = buf + 17 * j;                // pidx(16 j + r) = 17 j + r
-#pragma unroll
-    for (int r = 0; r < 16; r++) o[r] = vh[h][r];
+    for (int r = 0; r < 16; r++) o[r] = v[r];
   }
   __syncthreads();
 
@@ -907,11 +897,11 @@ __global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A)
 #pragma unroll
   for (int p = 0; p < Sh::NMID; p++) {
     const int Ns = 16 << (4 * p);
-    float2 vm[HB][16];
-    // the butterflies tid + h NT have one twiddle set when NT is a multiple of Ns
-    const bool share = HB > 1 && NT % Ns == 0;
-    float2 w[16];
-    auto twiddles = [&](int k) {
+    float2 v[16];
+    const int j = tid;
+    if (tid < NTF) {
+      const int k = j & (Ns - 1);
+      float2 w[16];
       if (p == 0) {
         // second pass (Ns = 16): W_L^(r k L/256) = W_256^(r k), r k < 256, read whole
 #pragma unroll
@@ -919,14 +909,6 @@ __global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A)
       } else {
         twiddles16(twlo, twhi, k * (L / (Ns * 16)), w);
       }
-    };
-    if (share) twiddles(tid & (Ns - 1));
-#pragma unroll
-    for (int h = 0; h < HB; h++) {
-      const int j = tid + h * NT;
-      if (j >= NTF) continue;
-      if (!share) twiddles(j & (Ns - 1));
-      float2* v = vm[h];
       const float2* ip = buf + pidx(j);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
@@ -936,49 +918,38 @@ __global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A)
       DFT<16>::run(v);
     }
     __syncthreads();
-#pragma unroll
-    for (int h = 0; h < HB; h++) {
-      const int j = tid + h * NT;
-      if (j >= NTF) continue;
+    if (tid < NTF) {
       const int k = j & (Ns - 1);
       const int idxD = (j / Ns) * Ns * 16 + k;
       float2* op = buf + pidx(idxD);           // Ns is a multiple of 16
 #pragma unroll
-      for (int r = 0; r < 16; r++) op[r * (Ns + Ns / 16)] = vm[h][r];
+      for (int r = 0; r < 16; r++) op[r * (Ns + Ns / 16)] = v[r];
     }
     __syncthreads();
   }
 
   // ---- last pass (radix RL, Ns = L/RL): only the bins of the kept sideband
   //      (USB [0, C), real LSB [C, 2C)), then the fractional-sample rotation
-  if (RL == 2 && rotfast) {
+  if (RL == 2 && tid < NTF && rotfast) {
     // the sideband is a branch around the whole pass, not a select per output
 #define FX_LAST_R2(LSB)                                                                                           \
     if (A.prec == FXG_XMAC_FP32)                                                                                  \
       last_pass_r2<L, LSB, float2*>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f, (float2*)A.spectra + outbase,    \
-                                    A.gstride, jj);                                                              \
+                                    A.gstride, tid);                                                              \
     else if (A.prec == FXG_XMAC_FP16)                                                                             \
       last_pass_r2<L, LSB, __half2*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale,                              \
-                                     (__half2*)A.spectra + outbase, A.gstride, jj);                              \
+                                     (__half2*)A.spectra + outbase, A.gstride, tid);                              \
     else if (A.prec == FXG_XMAC_FP8)                                                                              \
       last_pass_r2<L, LSB, Fp8Out>(buf, twlo, twhi, fsub, fstep, s, slog, 1.0f,                                   \
                                    Fp8Out{(unsigned short*)A.spectra + outbase, A.cexp + (long long)gband * C},   \
-                                   A.gstride, jj);                                                               \
+                                   A.gstride, tid);                                                               \
     else                                                                                                          \
       last_pass_r2<L, LSB, __hip_bfloat162*>(buf, twlo, twhi, fsub, fstep, s, slog, A.scale,                      \
-                                             (__hip_bfloat162*)A.spectra + outbase, A.gstride, jj);
-#pragma unroll
-    for (int h = 0; h < HB; h++) {
-      const int jj = tid + h * NT;
-      if (jj >= NTF) continue;
-      if (B.lsb) { FX_LAST_R2(true) } else { FX_LAST_R2(false) }
-    }
+                                             (__hip_bfloat162*)A.spectra + outbase, A.gstride, tid);
+    if (B.lsb) { FX_LAST_R2(true) } else { FX_LAST_R2(false) }
 #undef FX_LAST_R2
-  } else {
-#pragma unroll
-   for (int h = 0; h < HB; h++) {
-    const int j = tid + h * NT;
-    if (j >= NTF) continue;
+  } else if (tid < NTF) {
+    const int j = tid;
     const int mh = B.lsb ? RL / 2 : 0;
 #pragma unroll
     for (int u = 0; u < 16 / RL; u++) {
@@ -1014,7 +985,6 @@ __global__ __launch_bounds__((FftShape<L, PPT>::NT)) void k_fstage(FStageArgs A)
         }
       }
     }
-   }
   }
 }
 
@@ -1499,12 +1469,10 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   }
 }
 
-#define INSTANTIATE(Lv, P) template __global__ void k_fstage<Lv, 2, P>(FStageArgs); \
-                           template __global__ void k_fstage<Lv, 8, P>(FStageArgs);
-INSTANTIATE(128, 16) INSTANTIATE(256, 16) INSTANTIATE(512, 16) INSTANTIATE(1024, 16)
-INSTANTIATE(2048, 16) INSTANTIATE(4096, 16) INSTANTIATE(8192, 16) INSTANTIATE(16384, 16)
-INSTANTIATE(4096, 32) INSTANTIATE(8192, 32) INSTANTIATE(16384, 32)
-#undef INSTANTIATE
+#define INSTANTIATE(Lv) template __global__ void k_fstage<Lv, 2>(FStageArgs); \
+                        template __global__ void k_fstage<Lv, 8>(FStageArgs);
+INSTANTIATE(128) INSTANTIATE(256) INSTANTIATE(512) INSTANTIATE(1024)
+INSTANTIATE(2048) INSTANTIATE(4096) INSTANTIATE(8192) INSTANTIATE(16384)
 template __global__ void k_bigfft_a<2048, 2>(FStageArgs);
 template __global__ void k_bigfft_a<2048, 8>(FStageArgs);
 template __global__ void k_bigfft_a<4096, 2>(FStageArgs);
@@ -1567,13 +1535,7 @@ hipError_t launch_prologue(const DevStation* st, const DevStationSub* ss, const 
   return hipGetLastError();
 }
 
-bool fstage_ppt_supported(int L, int cplx, int ppt) {
-  return ppt == 16 || (ppt == 32 && !cplx && (L == 4096 || L == 8192 || L == 16384));
-}
-
-hipError_t launch_fstage(int L, int nbits, int cplx, int ppt, const FStageArgs& a, int nbands, int nblocks,
-                         hipStream_t stream) {
-  if (!fstage_ppt_supported(L, cplx, ppt)) return hipErrorInvalidValue;
+hipError_t launch_fstage(int L, int nbits, int cplx, const FStageArgs& a, int nbands, int nblocks, hipStream_t stream) {
   dim3 grid(nblocks, nbands);
   if (nbits != 2 && nbits != 8) return hipErrorInvalidValue;
   if (cplx) {
@@ -1588,22 +1550,10 @@ hipError_t launch_fstage(int L, int nbits, int cplx, int ppt, const FStageArgs& 
     }
     return hipGetLastError();
   }
-  if (ppt == 32) {
-    switch (L) {
-#define CASE(Lv) case Lv: \
-      if (nbits == 2) hipLaunchKernelGGL((k_fstage<Lv, 2, 32>), grid, dim3(FftShape<Lv, 32>::NT), 0, stream, a); \
-      else hipLaunchKernelGGL((k_fstage<Lv, 8, 32>), grid, dim3(FftShape<Lv, 32>::NT), 0, stream, a); \
-      break;
-      CASE(4096) CASE(8192) CASE(16384)
-#undef CASE
-      default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-  }
   switch (L) {
 #define CASE(Lv) case Lv: \
-      if (nbits == 2) hipLaunchKernelGGL((k_fstage<Lv, 2, 16>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
-      else hipLaunchKernelGGL((k_fstage<Lv, 8, 16>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
+      if (nbits == 2) hipLaunchKernelGGL((k_fstage<Lv, 2>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
+      else hipLaunchKernelGGL((k_fstage<Lv, 8>), grid, dim3(FftShape<Lv>::NT), 0, stream, a); \
       break;
     CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096) CASE(8192) CASE(16384)
 #undef CASE

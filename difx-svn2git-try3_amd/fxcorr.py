@@ -138,7 +138,7 @@ class DsFormat(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("xmac_streaming", C.c_int32), ("xmac_ct", C.c_int32), ("xmac_tb", C.c_int32),
-                ("chunk_blocks", C.c_int32), ("fft_ppt", C.c_int32)]
+                ("chunk_blocks", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

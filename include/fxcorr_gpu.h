@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FXG_ABI_VERSION 4
+#define FXG_ABI_VERSION 3
 
 /* ---- limits of the fixed-size descriptors ------------------------------ */
 #define FXG_MAX_FREQS        64   /* frequency table entries per job          */
@@ -406,13 +406,11 @@ int64_t fxg_vdifmux(fxg_ctx* ctx, const fxg_vdifmux_desc* desc, const uint8_t* s
                     int64_t dest_bytes, int64_t start_frame, fxg_vdifmux_stats* stats);
 
 /* ---- launch schedules and the autotuner --------------------------------
- * No counterpart in the reference (its FFT plan is built once per length,
- * mpifxcorr/src/architecture.h.in:825-838, and its XMAC is a fixed library
- * call).  A schedule picks, per job, the XMAC's staging (through registers
- * or streamed into LDS with direct loads; FP32: matrix cores or VALU tiles),
- * the FP32 VALU XMAC's tile (channels per workgroup, blocks per LDS stage),
- * the F stage's FFT decomposition (points per thread) and the blocks per
- * F/X pass (spectra resident between the F stage and the XMAC).
+ * No counterpart in the reference (its FFT and XMAC are library calls with
+ * fixed strategies).  A schedule picks, per job, the reduced-precision XMAC's
+ * staging (through registers or streamed into LDS with direct loads), the
+ * FP32 XMAC's tile (channels per workgroup, blocks per LDS stage) and the
+ * blocks per F/X pass (spectra resident between the F stage and the XMAC).
  * Every schedule computes the same products; FP32 sums grouped differently
  * by a different pass length agree to rounding. */
 typedef struct {
@@ -420,10 +418,6 @@ typedef struct {
   int32_t xmac_ct;         /* FP32 VALU XMAC channels per workgroup, 0 = auto  */
   int32_t xmac_tb;         /* FP32 VALU XMAC blocks per LDS stage, 0 = auto    */
   int32_t chunk_blocks;    /* blocks per F/X pass, 0 = the plan's chunk_blocks */
-  int32_t fft_ppt;         /* F-stage FFT points per thread: 16, or 32 (two
-                              radix-16 butterflies per pass sharing one twiddle
-                              set; real samples of 2048, 4096 or 8192
-                              channels), 0 = automatic                        */
 } fxg_schedule;
 
 typedef struct {

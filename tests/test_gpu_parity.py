@@ -360,38 +360,6 @@ def test_fp32_xmac_matrix_cores_and_valu_tiles(gpu, oracle, mk):
         assert rel_err(got[a0:a1], ref[a0:a1]) < FP32_TOL, on
 
 
-@pytest.mark.parametrize("order", [0, 1, 2])
-@pytest.mark.parametrize("nchan,nbits,prec", [(2048, 2, "fp32"), (4096, 2, "fp32"), (4096, 8, "fp16"),
-                                              (8192, 2, "fp32")])
-def test_fft_points_per_thread(gpu, oracle, nchan, nbits, prec, order):
-    """The F stage with 32 points per FFT thread (two radix-16 butterflies
-    per pass, one twiddle set) computes the same spectra as with 16, bit for
-    bit: visibilities identical for fringe orders 0/1/2 over a block on an
-    invalid frame (general decoder) and the word-aligned blocks; the FP32
-    ones within 1e-5 of the FP32 oracle."""
-    lib, ctx = gpu
-    spec = JobSpec(nstations=3, nsubbands=1, npol=2, bw_mhz=64.0, nchan=nchan, blockspersend=6, nbits=nbits,
-                   fringe_order=order, xmac=prec)
-    job, keep = spec.build()
-    sd = make_data(spec, invalid_frames={1: [3]})
-    outs = []
-    for ppt in (16, 32):
-        g = GpuJob(lib, ctx, job)
-        sc = fxcorr.Schedule()
-        check(lib, lib.fxg_job_get_schedule(g.h, C.byref(sc)), "get_schedule")
-        sc.fft_ppt = ppt
-        check(lib, lib.fxg_job_set_schedule(g.h, C.byref(sc)), "set_schedule")
-        outs.append(g.process(sd))
-        g.close()
-    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
-    if prec == "fp32":
-        ref = oracle_results(oracle, job, sd, nthreads=8)
-        info = plan_info(oracle, "orc_", job)
-        xend, a0, a1 = region_slices(oracle, job, info)
-        assert rel_err(outs[1][:xend], ref[:xend]) < FP32_TOL
-        assert rel_err(outs[1][a0:a1], ref[a0:a1]) < FP32_TOL
-
-
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("nst", [16, 32])
 def test_xmac_streaming_many_stages(gpu, prec, nst):

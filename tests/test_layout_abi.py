@@ -188,7 +188,7 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 20
     for s in syms:
         assert hasattr(lib, s), s
-    assert lib.fxg_abi_version() == 4
+    assert lib.fxg_abi_version() == 3
 
 
 def test_host_library_exports_serving_abi():
