@@ -103,11 +103,13 @@ struct DevSeg {
 struct DevPcal {
   int gband;            // global recorded band
   int nbins;            // accumulation bins N
-  int cplx;             // counter-rotated complex bins (the shifting extractor)
-  int period;           // rotator period in samples (cplx)
+  int cplx;             // complex bins (the shifting extractors, complex samples)
+  int rot;              // samples times the mixer e^{i dphi ((so + n) mod period)}
+  int csamp;            // complex samples (unit = 2 x nbits, real part low)
+  int period;           // rotator period in samples (rot)
   int ntones;
   int toneoff;          // first entry of the band's tone-bin table (-1 entries give 0)
-  long long rotoff;     // float2 offset of the rotator table in the aux buffer (cplx)
+  long long rotoff;     // float2 offset of the rotator table in the aux buffer (rot)
   long long twoff;      // float2 offset of e^{-2 pi i m / N}, m < N, in the aux buffer
   long long partoff;    // float offset of the band's partial sums [group][N (x2 if cplx)]
   long long resoff;     // cf32 offset of the band's tones in the result

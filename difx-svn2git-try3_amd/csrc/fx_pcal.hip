@@ -1,11 +1,13 @@
 // fx_pcal.hip — multi-tone phase-cal extraction for gfx950 (Mode::process,
-// mode.cpp:774-788; extractors pcal.cpp:333-470, 498-690, 927-1104).
+// mode.cpp:774-788; extractors pcal.cpp:333-470, 498-690, 927-1104 for real
+// samples, 707-925 and 1107-1285 for complex samples).
 //
 // Every processed block adds its L-sample window to the band's bins: sample
 // n of the window has absolute number so + n since the second boundary
 // (so = datasamples + nearestsample, mode.cpp:777) and goes to bin
-// (so + n) mod N, as it is (first tone at DC, or the implicit shift) or
-// times the mixer value e^{i dphi ((so + n) mod P)} (the shifting extractor).
+// (so + n) mod N, as it is (first tone at DC, or the implicit shifts) or
+// times the mixer value e^{i dphi ((so + n) mod P)} (the shifting
+// extractors).  Complex samples (L = the channel count) fill complex bins.
 // The tones are bins of the forward N-point DFT of the summed bins.
 //
 //   k_pcal_accum  one workgroup per (band, block group); each thread owns the
@@ -35,6 +37,9 @@ __global__ __launch_bounds__(256) void k_pcal_accum(PcalArgs A) {
   const unsigned* stream = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
   const long long nwords = Q.ctstride >> 2;
   const unsigned mask = (1u << S.nbits) - 1u;
+  const int UB = pc.csamp ? 2 * S.nbits : S.nbits;          // bits per sample of the band's stream
+  const unsigned cmask = (1u << UB) - 1u;
+  const float* lutc = S.nbits == 8 ? A.lut8 : A.st[B.station].lut;   // one component's levels
 
   for (int b = tid; b < N; b += 256) {
     if (pc.cplx) { acc[2 * b] = 0.0f; acc[2 * b + 1] = 0.0f; }
@@ -47,13 +52,40 @@ __global__ __launch_bounds__(256) void k_pcal_accum(PcalArgs A) {
     const long long so = (long long)Q.datasamples + t0;
     int som = (int)(so % N);
     if (som < 0) som += N;
-    int sor = pc.cplx ? (int)(so % pc.period) : 0;
+    int sor = pc.rot ? (int)(so % pc.period) : 0;
     if (sor < 0) sor += pc.period;
     const bool fast = G.allvalid != 0;
     for (int b = tid; b < N; b += 256) {
       int n = b - som;
       if (n < 0) n += N;
       float ar = 0.0f, ai = 0.0f;
+      if (pc.csamp) {
+        for (; n < L; n += N) {
+          float2 x;
+          if (fast) {
+            const long long bp = (t0 + n) * UB;
+            const unsigned w = (bp >> 5) < nwords ? stream[bp >> 5] : 0u;
+            const unsigned code = (w >> (bp & 31)) & cmask;   // a unit never straddles a word
+            x = S.nbits == 8 ? make_float2(A.lut8[code & 255u], A.lut8[code >> 8])
+                             : make_float2(pick4(lut2, code & 3u), pick4(lut2, code >> 2));
+          } else {
+            x = sample_value_c(S, Q, A.ftab, lutc, t0 + n, G.validend, B.band);
+          }
+          if (pc.rot) {                        // vectorMul_cf32 by the mixer
+            int m = sor + n;
+            if (m >= pc.period) m %= pc.period;
+            const float2 r = A.aux[pc.rotoff + m];
+            ar += x.x * r.x - x.y * r.y;
+            ai += x.x * r.y + x.y * r.x;
+          } else {
+            ar += x.x;
+            ai += x.y;
+          }
+        }
+        acc[2 * b] += ar;
+        acc[2 * b + 1] += ai;
+        continue;
+      }
       for (; n < L; n += N) {
         float x;
         if (fast) {
@@ -64,7 +96,7 @@ __global__ __launch_bounds__(256) void k_pcal_accum(PcalArgs A) {
         } else {
           x = sample_value(S, Q, A.ftab, A.lut8, lut2, t0 + n, G.validend, B.band);
         }
-        if (pc.cplx) {
+        if (pc.rot) {
           int m = sor + n;
           if (m >= pc.period) m %= pc.period;
           const float2 r = A.aux[pc.rotoff + m];

@@ -124,11 +124,13 @@ void config_tones(const fxg_freq& fq, double interval, double base, int* ntones,
   *ntones = n;
 }
 
-// The extractor PCal::getNew picks for real sampling with the 1 MHz minimum
-// resolution Mode sets (mode.cpp:412-428; pcal.cpp:139-203), its bins and the
-// DFT bin of each tone (-1: output zero).  kind 0: no tone (zeros), 1: first
+// The extractor PCal::getNew picks with the 1 MHz minimum resolution Mode
+// sets (mode.cpp:412-428; pcal.cpp:139-203), its bins and the DFT bin of each
+// tone (-1: output zero).  kind 0: no tone (zeros); real samples 1: first
 // tone at DC (pcal.cpp:333-350, 448-470), 2: implicit shift (:927-960,
-// 1064-1104), 3: shifting (:498-540, 652-690).
+// 1064-1104), 3: shifting (:498-540, 652-690); complex samples (sample rate
+// = bandwidth) 4: implicit shift (:1107-1128, 1239-1285), 5: shifting
+// (:707-760, 890-925).  Double sideband reads the DFT reversed (bin N-1-b).
 struct PcalShape {
   int kind = 0;
   long long nbins = 0, period = 0;
@@ -137,15 +139,36 @@ struct PcalShape {
   std::vector<int> tonebin;
 };
 
-PcalShape pcal_shape(double bw_hz, double spacing_hz, int offset_hz) {
+PcalShape pcal_shape(double bw_hz, double spacing_hz, int offset_hz, bool cplx, bool dsb) {
   PcalShape p;
-  const double fs = 2.0 * bw_hz, minres = 1e6;
+  const double fs = cplx ? bw_hz : 2.0 * bw_hz, minres = 1e6;
   if (offset_hz == -1) return p;
   long long No = (long long)(fs / gcd_d(fs, (double)offset_hz));
   const long long Np = (long long)(fs / gcd_d(fs, spacing_hz));
   const int Nt = count_tones(bw_hz, (double)offset_hz, spacing_hz);
   while (fs / (2 * No) > minres) No *= 2;
   if (Nt == 0) return p;
+  if (cplx) {
+    const bool implicit = No % Np == 0;
+    p.kind = implicit ? 4 : 5;
+    p.nbins = (long long)(fs / gcd_d(fs, implicit ? (double)offset_hz : spacing_hz));
+    while (fs / (2 * p.nbins) > minres) p.nbins *= 2;
+    p.ntones = Nt;
+    if (!implicit) {
+      // the mixer's values repeat with the offset's period (the reference's
+      // table runs over the lcm of that and N, a whole number of periods)
+      p.period = (long long)(fs / gcd_d(fs, offset_hz));
+      p.dphi = kTwoPi * (-offset_hz / fs);
+    }
+    const long long step = (long long)std::floor(double(p.nbins) * (spacing_hz / fs));
+    const long long off = implicit ? (long long)std::floor(double(p.nbins) * (offset_hz / fs)) : 0;
+    for (int i = 0; i < p.ntones; i++) {
+      const long long idx = off + i * step;
+      // the reference stops copying at the first bin past the DFT; zero
+      p.tonebin.push_back(idx >= p.nbins ? -1 : (int)(dsb ? p.nbins - 1 - idx : idx));
+    }
+    return p;
+  }
   if (offset_hz == 0) {
     p.kind = 1;
     p.nbins = (long long)(fs / gcd_d(fs, spacing_hz));
@@ -222,7 +245,6 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
       REQUIRE(ds.nrecbands * 2 * ds.nbits >= 8, FXG_EUNSUPPORTED, "complex sampling needs >= 8 bits per sample-time");
       REQUIRE(d->fringe_order != 0, FXG_EUNSUPPORTED,
               "post-F fringe rotation is not supported for complex data (mode.cpp:1155-1158)");
-      REQUIRE(!(ds.pcal_interval_mhz > 0), FXG_EUNSUPPORTED, "phase cal of complex data");
     }
     if (ds.format != FXG_FORMAT_MARK5B) {
       REQUIRE(ds.nbits == 2 || ds.nbits == 8, FXG_EUNSUPPORTED, "VDIF nbits must be 2 or 8");
@@ -476,34 +498,39 @@ int build_plan(const fxg_job_desc* d, Plan* p, std::string* err) {
     const fxg_datastream& ds = d->datastreams[s];
     if (P.pcal_ntones[s].empty()) continue;
     const double bwhz = 1e6 * d->freqs[ds.recfreq_index[0]].bw_mhz;
+    const bool csamp = ds.sampling == FXG_SAMPLING_COMPLEX;
+    const double fsr = csamp ? bwhz : 2 * bwhz;
     long long at3 = P.pcaloff[s];
     for (int k = 0; k < ds.nrecbands; k++) {
       const int lf = ds.recband_freq[k];
       const int nt = P.pcal_ntones[s][lf];
       REQUIRE(P.pcal_offhz[s][lf] >= -1, FXG_EUNSUPPORTED,
               "phase cal: the tone base puts the first tone outside the band");
-      PcalShape sh = pcal_shape(bwhz, 1e6 * ds.pcal_interval_mhz, P.pcal_offhz[s][lf]);
+      PcalShape sh = pcal_shape(bwhz, 1e6 * ds.pcal_interval_mhz, P.pcal_offhz[s][lf], csamp,
+                                csamp && ds.complex_dsb != 0);
       const long long resoff = at3;
       at3 += nt;
       if (sh.kind == 0) continue;                       // no tone in the band: zeros
       REQUIRE(sh.ntones == nt, FXG_EUNSUPPORTED,
               "phase cal: the extractor's tone count differs from the configuration's");
-      REQUIRE(std::fabs(bwhz * 2 - std::floor(bwhz * 2 + 0.5)) < 1e-6, FXG_EUNSUPPORTED,
+      REQUIRE(std::fabs(fsr - std::floor(fsr + 0.5)) < 1e-6, FXG_EUNSUPPORTED,
               "phase cal needs an integral sample rate in Hz");
-      REQUIRE(sh.nbins * (sh.kind == 3 ? 2 : 1) <= kPcalMaxFloats, FXG_EUNSUPPORTED,
+      REQUIRE(sh.nbins * (sh.kind >= 3 ? 2 : 1) <= kPcalMaxFloats, FXG_EUNSUPPORTED,
               "phase cal: too many accumulation bins for one workgroup");
       REQUIRE(sh.period <= (1 << 22), FXG_EUNSUPPORTED, "phase cal: rotator period too long");
       DevPcal pc{};
       pc.gband = P.st[s].band0 + k;
       pc.nbins = (int)sh.nbins;
-      pc.cplx = sh.kind == 3;
+      pc.cplx = sh.kind >= 3;
+      pc.rot = sh.kind == 3 || sh.kind == 5;
+      pc.csamp = csamp;
       pc.period = (int)sh.period;
       pc.ntones = nt;
       pc.toneoff = (int)P.pcal_tonebin.size();
       P.pcal_tonebin.insert(P.pcal_tonebin.end(), sh.tonebin.begin(), sh.tonebin.end());
       pc.resoff = resoff;
-      if (pc.cplx) {
-        // the shifting extractor's mixer, (f32) of the FP64 value (pcal.cpp:530-535)
+      if (pc.rot) {
+        // the shifting extractors' mixer, (f32) of the FP64 value (pcal.cpp:530-535, 751-756)
         pc.rotoff = (long long)P.pcal_aux.size() / 2;
         for (long long m = 0; m < sh.period; m++) {
           const double arg = sh.dphi * double(m);

@@ -191,7 +191,11 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
     pcal: phase-cal comb added to every band before quantisation, locked to
     the second boundary: {"amp": a, "spacing_hz": s, "offsets_hz": [per
     subband], "ntones": [per subband], "slope_deg": d}; tone k of a band is
-    a * sin(2 pi (offset + k s) n / fs + k d) at absolute sample n.
+    a * sin(2 pi (offset + k s) n / fs + p_k) at absolute sample n, p_k =
+    k d, or "phases_deg"[k] when the dict has that list (real
+    sampling, fs = 2 x bandwidth), a * exp(i (2 pi (offset + k s) n / fs +
+    k d)) (complex sampling, fs = bandwidth; double sideband: the comb shifted
+    down by half the band, the LO sitting at the band centre).
     """
     rng = np.random.default_rng(seed)
     ts = spec.sampletime_us
@@ -226,8 +230,6 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
     hi = max(dsamp)
     stations = []
     cplx = spec.is_complex
-    if cplx and pcal:
-        raise ValueError("phase-cal combs are generated for real sampling only")
     codes_all = [np.empty((T, spec.nrecbands), dtype=np.uint16 if cplx and spec.nbits == 8 else np.uint8)
                  for _ in range(nst)]
     quant = quantise2 if spec.nbits == 2 else quantise8
@@ -239,19 +241,22 @@ def make_subint(spec: JobSpec, seed: int = 1, rho: float | Sequence[float] = 0.0
         r = float(rhos[sb])
         comb = 0.0
         if pcal:
-            fs_hz = 2e6 * spec.bw_mhz
+            fs_hz = (1e6 if cplx else 2e6) * spec.bw_mhz
             nabs = datasamples + np.arange(T, dtype=np.float64)
-            comb = np.zeros(T)
+            comb = np.zeros(T, np.complex128 if cplx else np.float64)
             for k in range(pcal["ntones"][sb]):
                 f = pcal["offsets_hz"][sb] + k * pcal["spacing_hz"]
-                ph = 2 * np.pi * np.mod(f * nabs, fs_hz) / fs_hz
-                comb += pcal["amp"] * np.sin(ph + np.radians(k * pcal.get("slope_deg", 0.0)))
+                if spec.sampling == "complex_dsb":
+                    f -= fs_hz / 2
+                pk = pcal["phases_deg"][k] if "phases_deg" in pcal else k * pcal.get("slope_deg", 0.0)
+                ph = 2 * np.pi * np.mod(f * nabs, fs_hz) / fs_hz + np.radians(pk)
+                comb += pcal["amp"] * (np.exp(1j * ph) if cplx else np.sin(ph))
         for s in range(nst):
             gs = g[dsamp[s] - lo: dsamp[s] - lo + T]
             for p in range(npol):
                 if cplx:
                     n = rng.standard_normal(T) + 1j * rng.standard_normal(T)
-                    x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * n
+                    x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * n + comb
                     q = quant(x.real).astype(np.uint16) | (quant(x.imag).astype(np.uint16) << spec.nbits)
                 else:
                     x = math.sqrt(r) * gs + math.sqrt(1.0 - r) * rng.standard_normal(T) + comb

@@ -26,7 +26,9 @@
  *   pcal     mpifxcorr/src/mode.cpp:412-428, 774-788 (extraction per block);
  *            pcal.cpp:139-229 (getNew, calcNumTones), :253-331 (the direct
  *            reference method), :333-470 (first tone at DC), :498-690
- *            (shifting), :927-1104 (implicit shift), :1376-1450 (dummy);
+ *            (shifting), :927-1104 (implicit shift), :707-925 and
+ *            :1107-1285 (complex samples: shifting, implicit shift),
+ *            :1376-1450 (dummy);
  *            mathutil.cpp:8-30 (gcd); core.cpp:1124-1157 (copyPCalTones)
  *
  * Two arithmetic flavours share one code path through a template parameter:
@@ -493,9 +495,12 @@ static int validate(const fxg_job_desc* j) {
  * binning (configuration.cpp:2212-2441), plus the used / equivalent-used
  * frequency flags (configuration.cpp:325-367) and stride selection. */
 /* phase-cal extractor kinds (PCal::getNew, pcal.cpp:139-203) */
-enum { PC_DUMMY = 0, PC_TRIVIAL = 1, PC_SHIFT = 2, PC_IMPLICIT = 3 };
+enum { PC_DUMMY = 0, PC_TRIVIAL = 1, PC_SHIFT = 2, PC_IMPLICIT = 3,
+       PC_CSHIFT = 4,        /* complex samples, mixer shift (PCalExtractorComplex) */
+       PC_CIMPLICIT = 5 };   /* complex samples, implicit shift (PCalExtractorComplexImplicitShift) */
 static void recfreq_pcal_tones(const fxg_freq& fq, double interval, double base, int* ntones, int* offsethz);
 static int pcal_kind(double bw_hz, double spacing_hz, int offset_hz, double minres);
+static int pcal_kind_c(double bw_hz, double spacing_hz, int offset_hz, double minres);
 static int pcal_ntones_of(int kind, double bw_hz, double spacing_hz, int offset_hz);
 
 static int make_layout(const fxg_job_desc* j, Layout* L) {
@@ -609,7 +614,9 @@ static int make_layout(const fxg_job_desc* j, Layout* L) {
     const double bwhz = 1e6 * j->freqs[ds.recfreq_index[0]].bw_mhz;
     for (int i = 0; i < ds.nrecfreqs; i++) {
       if (L->pcal_offhz[d][i] < -1) return fail(FXG_EUNSUPPORTED, "phase cal: first tone outside the band");
-      const int kind = pcal_kind(bwhz, 1e6 * ds.pcal_interval_mhz, L->pcal_offhz[d][i], 1e6);
+      const int kind = ds.sampling == FXG_SAMPLING_COMPLEX
+                           ? pcal_kind_c(bwhz, 1e6 * ds.pcal_interval_mhz, L->pcal_offhz[d][i], 1e6)
+                           : pcal_kind(bwhz, 1e6 * ds.pcal_interval_mhz, L->pcal_offhz[d][i], 1e6);
       if (kind != PC_DUMMY && pcal_ntones_of(kind, bwhz, 1e6 * ds.pcal_interval_mhz, L->pcal_offhz[d][i]) !=
                                   L->pcal_ntones[d][i])
         return fail(FXG_EUNSUPPORTED, "phase cal: extractor and configuration tone counts differ");
@@ -635,7 +642,6 @@ static int make_geom(const fxg_job_desc* j, const Layout& lay, int d, DsGeom* g)
   g->order = j->fringe_order;
   if (g->cplx && g->order == 0)
     return fail(FXG_EUNSUPPORTED, "post-F fringe rotation is not supported for complex data (mode.cpp:1155-1158)");
-  if (g->cplx && ds.pcal_interval_mhz > 0) return fail(FXG_EUNSUPPORTED, "phase cal of complex data");
   g->fractionalLoFreq = false; g->deltapoloffsets = false;
   for (int i = 0; i < ds.nrecfreqs; i++) {
     const fxg_freq& fi = j->freqs[ds.recfreq_index[i]];
@@ -1018,24 +1024,53 @@ static cpx<double> dft_bin(const cpx<T>* x, long N, long k) {
   cpx<double> o; o.re = re; o.im = im; return o;
 }
 
-/* The real-sampling extractors of pcal.cpp, with their state: trivial
- * (:333-470), shifting (:498-690), implicit shift (:927-1104), dummy
- * (:1376-1450).  T is the accumulator type (f32 in the reference). */
+/* The extractors of pcal.cpp, with their state: for real samples trivial
+ * (:333-470), shifting (:498-690), implicit shift (:927-1104); for complex
+ * samples shifting (:707-925) and implicit shift (:1107-1285); dummy
+ * (:1376-1450).  T is the accumulator type (f32 in the reference).  The
+ * complex kinds take len complex samples (float pairs, real first), sample at
+ * the band's rate fs = bandwidth. */
 template <typename T>
 struct PcalExtractor {
   int kind = PC_DUMMY;
   double fs = 0, spacing = 0;
   long offset = 0;
   int nbins = 0, ntones = 0, tonestep = 0;
+  bool dsb = false;                     /* complex double sideband: the DFT read reversed */
   size_t rotatorlen = 0, rotator_index = 0, pcal_index = 0;
   uint64_t samplecount = 0;
   bool finalized = false;
   std::vector<T> preal;
   std::vector<cpx<T>> pcplx, rotated, rotator;
 
-  void init(int k, double bw_hz, double spacing_hz, int offset_hz, double minres) {
-    kind = k; spacing = std::fabs(spacing_hz); offset = offset_hz; fs = 2 * bw_hz;
-    if (kind == PC_TRIVIAL) {
+  bool complex_kind() const { return kind == PC_CSHIFT || kind == PC_CIMPLICIT; }
+  void make_rotator() {                 /* f32(cos(arg)), f32(sin(arg)) over 2 x rotatorlen */
+    rotator.resize(2 * rotatorlen);
+    const double dphi = 2 * M_PI * (-offset / fs);
+    for (size_t n = 0; n < 2 * rotatorlen; n++) {
+      const double arg = dphi * double(n);
+      rotator[n].re = (T)std::cos(arg);
+      rotator[n].im = (T)std::sin(arg);
+    }
+    rotated.assign(2 * rotatorlen, cpx<T>{0, 0});
+  }
+  void init(int k, double bw_hz, double spacing_hz, int offset_hz, double minres, bool dsb_ = false) {
+    kind = k; spacing = std::fabs(spacing_hz); offset = offset_hz; fs = 2 * bw_hz; dsb = false;
+    if (complex_kind()) {
+      fs = bw_hz;
+      dsb = dsb_;
+      ntones = calc_num_tones(bw_hz, (double)offset, spacing);
+      if (kind == PC_CIMPLICIT) {         /* pcal.cpp:1107-1128 */
+        nbins = (int)(fs / gcd_d(fs, (double)offset));
+        while (fs / (2 * nbins) > minres) nbins *= 2;
+      } else {                            /* pcal.cpp:707-760 */
+        nbins = (int)(fs / gcd_d(fs, spacing));
+        while (fs / (2 * nbins) > minres) nbins *= 2;
+        rotatorlen = (size_t)(fs / gcd_d(fs, (double)offset));
+        rotatorlen = nbins * rotatorlen / gcd_l((long)rotatorlen, (long)nbins);
+        make_rotator();
+      }
+    } else if (kind == PC_TRIVIAL) {
       offset = 0;
       nbins = (int)(fs / gcd_d(fs, spacing));
       ntones = calc_num_tones(bw_hz, 0.0f, spacing) - 1;
@@ -1045,14 +1080,7 @@ struct PcalExtractor {
       ntones = calc_num_tones(bw_hz, offset, spacing);
       rotatorlen = (size_t)(fs / gcd_d(fs, (double)offset));
       rotatorlen = nbins * rotatorlen / gcd_l((long)rotatorlen, (long)nbins);
-      rotator.resize(2 * rotatorlen);
-      const double dphi = 2 * M_PI * (-offset / fs);
-      for (size_t n = 0; n < 2 * rotatorlen; n++) {      /* f32(cos(arg)) in the reference */
-        const double arg = dphi * double(n);
-        rotator[n].re = (T)std::cos(arg);
-        rotator[n].im = (T)std::sin(arg);
-      }
-      rotated.assign(2 * rotatorlen, cpx<T>{0, 0});
+      make_rotator();
     } else if (kind == PC_IMPLICIT) {
       nbins = (int)(fs / gcd_d(fs, (double)offset));
       ntones = calc_num_tones(bw_hz, offset, spacing);
@@ -1067,14 +1095,59 @@ struct PcalExtractor {
     samplecount = 0; finalized = false;
     pcplx.assign(2 * (size_t)std::max(nbins, 1), cpx<T>{0, 0});
     preal.assign(2 * (size_t)std::max(nbins, 1), (T)0);
-    if (kind == PC_SHIFT) rotated.assign(2 * rotatorlen, cpx<T>{0, 0});
+    if (kind == PC_SHIFT || kind == PC_CSHIFT) rotated.assign(2 * rotatorlen, cpx<T>{0, 0});
   }
   void adjust(size_t so) {
-    if (kind == PC_SHIFT) { rotator_index = so % rotatorlen; pcal_index = so % nbins; }
+    if (kind == PC_SHIFT || kind == PC_CSHIFT) { rotator_index = so % rotatorlen; pcal_index = so % nbins; }
     else if (kind == PC_TRIVIAL || kind == PC_IMPLICIT) { rotator_index = 0; pcal_index = so % nbins; }
+    else if (kind == PC_CIMPLICIT) pcal_index = so % nbins;
+  }
+  /* complex samples: src holds len (re, im) pairs */
+  void extract_c(const float* src_f, size_t len) {
+    if (finalized) return;
+    const cpx<float>* src = reinterpret_cast<const cpx<float>*>(src_f);
+    cpx<T>* dst = &pcplx[pcal_index];
+    auto add = [&](const cpx<T>* p, size_t n) {      /* vectorAdd_cf32_I */
+      for (size_t i = 0; i < n; i++) { dst[i].re += p[i].re; dst[i].im += p[i].im; }
+    };
+    if (kind == PC_CIMPLICIT) {           /* pcal.cpp:1195-1226 */
+      std::vector<cpx<T>> x(nbins);
+      auto take = [&](size_t n) { for (size_t i = 0; i < n; i++) x[i] = cpx<T>{(T)src[i].re, (T)src[i].im}; };
+      const size_t tail = len % nbins, end = len - tail;
+      for (size_t n = 0; n < end; n += nbins, src += nbins) { take(nbins); add(x.data(), nbins); }
+      if (tail != 0) {
+        take(tail); add(x.data(), tail);
+        pcal_index = (pcal_index + tail) % nbins;
+      }
+    } else if (kind == PC_CSHIFT) {       /* pcal.cpp:836-878 */
+      const size_t tail = len % rotatorlen, end = len - tail;
+      auto mix = [&](const cpx<float>* x, size_t n) {  /* vectorMul_cf32 */
+        for (size_t i = 0; i < n; i++) {
+          const cpx<T> r = rotator[rotator_index + i];
+          rotated[rotator_index + i].re = (T)x[i].re * r.re - (T)x[i].im * r.im;
+          rotated[rotator_index + i].im = (T)x[i].re * r.im + (T)x[i].im * r.re;
+        }
+      };
+      for (size_t n = 0; n < end; n += rotatorlen, src += rotatorlen) {
+        mix(src, rotatorlen);
+        const cpx<T>* pulse = &rotated[rotator_index];
+        for (size_t p = 0; p < rotatorlen / nbins; p++) { add(pulse, nbins); pulse += nbins; }
+      }
+      if (tail != 0) {
+        mix(src, tail);
+        const cpx<T>* pulse = &rotated[rotator_index];
+        const size_t tail2 = tail % nbins, end2 = tail - tail2;
+        for (size_t p = 0; p < end2 / (size_t)nbins; p++) { add(pulse, nbins); pulse += nbins; }
+        add(pulse, tail2);
+        rotator_index = (rotator_index + tail) % rotatorlen;
+        pcal_index = (pcal_index + tail2) % nbins;
+      }
+    }
+    samplecount += len;
   }
   void extract(const float* src, size_t len) {
     if (finalized) return;
+    if (complex_kind()) { extract_c(src, len); return; }
     if (kind == PC_TRIVIAL || kind == PC_IMPLICIT) {
       T* dst = &preal[pcal_index];
       const size_t tail = len % nbins, end = len - tail;
@@ -1133,7 +1206,19 @@ struct PcalExtractor {
       }
     }
     auto bin = [&](long k) { cpx<double> d = dft_bin(pcplx.data(), nbins, k); return cpx<T>{(T)d.re, (T)d.im}; };
-    if (kind == PC_TRIVIAL) {
+    if (complex_kind()) {
+      /* pcal.cpp:906-923 and 1262-1283: double sideband reverses the DFT
+       * output (n <-> N-1-n); tone n at bin [offset bin +] n x step, the
+       * copying stops at the first bin past the DFT (zeros here; the
+       * reference leaves those tones as they were) */
+      const size_t step = (size_t)(std::floor(double(nbins) * (spacing / fs)));
+      const size_t off = kind == PC_CIMPLICIT ? (size_t)(std::floor(double(nbins) * (offset / fs))) : 0;
+      for (size_t n = 0; n < (size_t)ntones; n++) {
+        const size_t idx = off + n * step;
+        if (idx >= (size_t)nbins) { out[n] = cpx<T>{0, 0}; continue; }
+        out[n] = bin(dsb ? (long)(nbins - 1 - idx) : (long)idx);
+      }
+    } else if (kind == PC_TRIVIAL) {
       for (int i = 0; i < ntones; i++) out[i] = bin((long)(i + 1) * tonestep);
     } else if (kind == PC_IMPLICIT) {
       const size_t step = (size_t)(std::floor(double(nbins) * (spacing / fs)));
@@ -1169,6 +1254,19 @@ static int pcal_kind(double bw_hz, double spacing_hz, int offset_hz, double minr
   if (offset_hz == 0.0f) return PC_TRIVIAL;
   if ((No % Np) == 0) return PC_IMPLICIT;
   return PC_SHIFT;
+}
+
+/* PCal::getNew for complex data (pcal.cpp:139-168, 186-191): the sample
+ * rate is the bandwidth; the same period test picks the implicit shift */
+static int pcal_kind_c(double bw_hz, double spacing_hz, int offset_hz, double minres) {
+  const double fs = bw_hz;
+  if (offset_hz == -1) return PC_DUMMY;
+  long No = (long)(fs / gcd_d(fs, (double)offset_hz));
+  const long Np = (long)(fs / gcd_d(fs, spacing_hz));
+  const int Nt = calc_num_tones(bw_hz, (double)offset_hz, spacing_hz);
+  while (fs / (2 * No) > minres) No *= 2;
+  if (Nt == 0) return PC_DUMMY;
+  return (No % Np) == 0 ? PC_CIMPLICIT : PC_CSHIFT;
 }
 
 static int pcal_ntones_of(int kind, double bw_hz, double spacing_hz, int offset_hz) {
@@ -1243,7 +1341,9 @@ struct CoreRun {
       pcx[d].resize(ds.nrecbands);
       for (int k = 0; k < ds.nrecbands; k++) {
         const int off = lay.pcal_offhz[d][ds.recband_freq[k]];
-        pcx[d][k].init(pcal_kind(bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6), bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6);
+        const int kind = geom[d].cplx ? pcal_kind_c(bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6)
+                                      : pcal_kind(bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6);
+        pcx[d][k].init(kind, bwhz, 1e6 * ds.pcal_interval_mhz, off, 1e6, geom[d].dsb);
       }
       datasamples[d] = static_cast<int>(st[d].ns / (geom[d].sampletime * 1e3) + 0.5);   /* mode.cpp:1572 */
     }
@@ -1271,7 +1371,8 @@ struct CoreRun {
         /* phase cal of the block's L samples (mode.cpp:774-788) */
         for (size_t k = 0; k < pcx[d].size(); k++) {
           pcx[d][k].adjust((size_t)(datasamples[d] + bg.nearestsample));
-          pcx[d][k].extract(work[d].unpacked.data() + k * work[d].up.stride + work[d].up.munged, (size_t)geom[d].L);
+          const size_t at = (size_t)k * work[d].up.stride + work[d].up.munged;   /* complex: float pairs */
+          pcx[d][k].extract(work[d].unpacked.data() + (geom[d].cplx ? 2 * at : at), (size_t)geom[d].L);
         }
         for (int i = 0; i < ds.nrecfreqs; i++) {
           int idx[FXG_MAX_BANDS]; int count = 0;
@@ -1501,6 +1602,24 @@ int orc_pcal_extract(double bw_hz, double spacing_hz, int offset_hz, int kind, d
   PcalExtractor<float> x;
   x.init(kind, bw_hz, spacing_hz, offset_hz, minres);
   for (int i = 0; i < nseg; i++) { x.adjust((size_t)so[i]); x.extract(data + start[i], (size_t)len[i]); }
+  std::vector<cpx<float>> o(x.ntones + 1);
+  *samples = (int64_t)x.final(o.data());
+  *ntones = x.ntones;
+  for (int i = 0; i < x.ntones; i++) { out[2 * i] = o[i].re; out[2 * i + 1] = o[i].im; }
+  return kind;
+}
+
+/* The same for complex samples (data: float pairs, start and len in
+ * complex samples): kind -1 = PCal::getNew's choice for complex data, else
+ * PC_CSHIFT / PC_CIMPLICIT; dsb = double sideband. */
+int orc_pcal_extract_c(double bw_hz, double spacing_hz, int offset_hz, int kind, double minres, int dsb, int nseg,
+                       const int64_t* so, const int64_t* start, const int64_t* len, const float* data,
+                       float* out, int32_t* ntones, int64_t* samples) {
+  if (kind < 0) kind = pcal_kind_c(bw_hz, spacing_hz, offset_hz, minres);
+  if (kind != PC_DUMMY && kind != PC_CSHIFT && kind != PC_CIMPLICIT) return fail(FXG_EINVAL, "kind");
+  PcalExtractor<float> x;
+  x.init(kind, bw_hz, spacing_hz, offset_hz, minres, dsb != 0);
+  for (int i = 0; i < nseg; i++) { x.adjust((size_t)so[i]); x.extract(data + 2 * start[i], (size_t)len[i]); }
   std::vector<cpx<float>> o(x.ntones + 1);
   *samples = (int64_t)x.final(o.data());
   *ntones = x.ntones;

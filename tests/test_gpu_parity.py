@@ -461,12 +461,20 @@ def test_uneven_workload_jobs(gpu, oracle, spec):
 
 # phase-cal configurations: the implicit-shift extractor (fractional-MHz band
 # edge), the shifting extractor (No % Np != 0), lower sideband, and a comb
-# shifted onto DC (the first-tone-at-DC extractor)
+# shifted onto DC (the first-tone-at-DC extractor); complex samples (sample
+# rate = bandwidth): the complex implicit-shift extractor (1600 bins), the
+# complex shifting extractor (16 bins, 2 MHz mixer), lower sideband, 8-bit,
+# and double sideband (the DFT read reversed)
 PCAL_CASES = {
     "implicit": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0),
     "shifting": dict(base_freq_mhz=8401.0, pcal_interval_mhz=3.0),
     "lsb": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0, lsb=(1, 1)),
     "dc": dict(base_freq_mhz=8400.0, pcal_interval_mhz=3.0, pcal_base_mhz=-3.0, nsubbands=1),
+    "c_implicit": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0, sampling="complex"),
+    "c_shifting": dict(base_freq_mhz=8401.0, pcal_interval_mhz=3.0, sampling="complex"),
+    "c_lsb": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0, lsb=(1, 1), sampling="complex"),
+    "c_8bit": dict(base_freq_mhz=8401.0, pcal_interval_mhz=3.0, sampling="complex", nbits=8),
+    "c_dsb": dict(base_freq_mhz=8400.01, pcal_interval_mhz=1.0, sampling="complex_dsb"),
 }
 
 
@@ -474,7 +482,10 @@ PCAL_CASES = {
 def test_phase_cal(gpu, oracle, case):
     """Phase-cal tones of every recorded band (Mode::process extraction,
     copyPCalTones) against the oracle's restated extractors, FP32, with a
-    flagged block, a bad frame and a comb injected at known phases."""
+    flagged block, a bad frame and a comb injected at known phases p_k
+    (Schroeder's 180 k^2 / K degrees, whose low crest factor keeps the 2-bit
+    quantiser from distorting the comb): tone k reads p_k - 90 degrees for
+    sines (real samples), p_k for complex exponentials."""
     lib, ctx = gpu
     kw = dict(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, blockspersend=24)
     kw.update(PCAL_CASES[case])
@@ -486,7 +497,9 @@ def test_phase_cal(gpu, oracle, case):
         check(lib, lib.fxg_plan_pcal_tones(C.byref(job), 0, sb, C.byref(nt), C.byref(off)), "pcal tones")
         ntones.append(nt.value)
         offs.append(max(off.value, 0))
-    comb = dict(amp=0.5, spacing_hz=1e6 * spec.pcal_interval_mhz, offsets_hz=offs, ntones=ntones, slope_deg=5.0)
+    K = max(ntones)
+    phases = [180.0 * k * k / K for k in range(K)]
+    comb = dict(amp=0.5, spacing_hz=1e6 * spec.pcal_interval_mhz, offsets_hz=offs, ntones=ntones, phases_deg=phases)
     sd = make_data(spec, invalid_frames={1: [2]}, invalid_blocks={2: [3, 4]}, pcal=comb)
     g = GpuJob(lib, ctx, job)
     got = g.process(sd)
@@ -499,16 +512,17 @@ def test_phase_cal(gpu, oracle, case):
     assert np.max(np.abs(got[p0:] - ref[p0:])) < 1e-4 * np.max(np.abs(ref[p0:]))
     # the rest of the result is unchanged by the extraction
     assert rel_err(got[:p0], ref[:p0]) < FP32_TOL
-    if case in ("implicit", "shifting", "lsb"):
-        # station 0, band 0: tone k of the comb has phase -90 + 5 k degrees
+    if case in ("implicit", "shifting", "lsb", "c_implicit", "c_shifting", "c_lsb", "c_8bit"):
+        # station 0, band 0
         ph = np.degrees(np.angle(got[p0:p0 + ntones[0]]))
-        exp = -90.0 + 5.0 * np.arange(ntones[0])
+        exp = (0.0 if spec.is_complex else -90.0) + np.array(phases[:ntones[0]])
         fs_edge = offs[0] + np.arange(ntones[0]) * comb["spacing_hz"] < 1e6 * spec.bw_mhz
         d = (ph - exp + 180) % 360 - 180
-        # 12288 samples of 2-bit data: a few degrees of noise per tone; a sign,
-        # bin or sample-offset mistake would move the phases by tens of degrees
-        assert np.all(np.abs(d[fs_edge]) < 20.0), d
-        assert np.mean(np.abs(d[fs_edge])) < 6.0, d
+        # 12288 real or 6144 complex samples of 2-bit data: a few degrees of
+        # noise per tone; a sign, bin or sample-offset mistake would move the
+        # phases by tens of degrees
+        assert np.all(np.abs(d[fs_edge]) < 12.0), d
+        assert np.mean(np.abs(d[fs_edge])) < 4.0, d
     g.close()
 
 

@@ -218,8 +218,9 @@ def test_plan_without_gpu_and_error_paths():
 def test_complex_sampling_plan_rules():
     """Complex sampling: fftchannels = nchan; what the reference refuses
     (post-F fringe rotation) or this build does not take (mixed real and
-    complex datastreams, 1-band 2-bit complex, phase cal of complex data) is
-    refused at planning, by the library and by the oracle alike."""
+    complex datastreams, 1-band 2-bit complex) is refused at planning, by the
+    library and by the oracle alike.  Phase cal of complex data plans, with
+    the same result layout in both."""
     lib, orc = fxcorr.load_gpu_lib(), fxcorr.load_oracle()
     info = PlanInfo()
     job, keep = JobSpec(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, sampling="complex").build()
@@ -235,7 +236,11 @@ def test_complex_sampling_plan_rules():
     assert lib.fxg_plan(C.byref(one), C.byref(info)) == -4
     pc, k4 = JobSpec(nstations=2, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, sampling="complex",
                      pcal_interval_mhz=1.0).build()
-    assert lib.fxg_plan(C.byref(pc), C.byref(info)) == -4
+    assert lib.fxg_plan(C.byref(pc), C.byref(info)) == 0
+    assert orc.orc_plan(C.byref(pc), C.byref(ref)) == 0 and ref.result_length == info.result_length
+    nt, off = C.c_int32(), C.c_int32()
+    assert lib.fxg_plan_pcal_tones(C.byref(pc), 0, 0, C.byref(nt), C.byref(off)) == 0
+    assert (nt.value, off.value) == (16, 1000000)
 
 
 def test_context_refuses_without_gpu():

@@ -3,7 +3,10 @@ pattern (mpifxcorr/src/pcal.cpp:1532-1640: tones with a 5 degree phase slope,
 expected phase -90 deg + 5 deg per tone), by the reference's slow direct
 method (PCal::extractAndIntegrate_reference, pcal.cpp:253-331), by the
 agreement of the shifting and implicit-shift extractors on whole periods,
-and by the tone tables of configuration.cpp:1771-1845 worked by hand."""
+and by the tone tables of configuration.cpp:1771-1845 worked by hand.  The
+complex-sampling extractors (pcal.cpp:707-925, 1107-1285) are pinned by
+closed-form complex combs (single and double sideband) and by the direct sum
+over samples that their bin folding computes."""
 import ctypes as C
 
 import numpy as np
@@ -215,3 +218,133 @@ def test_pcal_layout_matches_oracle():
                     check(lib, lib.fxg_plan_pcal_tones(C.byref(job), d, sb, C.byref(nt), C.byref(off)), "t")
                     ntot += spec.npol * nt.value
             assert a.result_length - x.value == ntot
+
+
+# ---- complex samples (PCalExtractorComplex, PCalExtractorComplexImplicitShift)
+PC_CSHIFT, PC_CIMPLICIT = 4, 5
+
+
+@pytest.fixture(scope="module")
+def orc_c(orc):
+    orc.orc_pcal_extract_c.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                       P(C.c_int64), P(C.c_int64), P(C.c_int64), P(C.c_float), P(C.c_float),
+                                       P(C.c_int32), P(C.c_int64)]
+    return orc
+
+
+def extract_c(orc, bw, spacing, offset, kind, minres, dsb, segs, x):
+    """x: complex samples; segs: (sampleoffset, start, length) in complex samples."""
+    n = len(segs)
+    so = (C.c_int64 * n)(*[s[0] for s in segs])
+    st = (C.c_int64 * n)(*[s[1] for s in segs])
+    ln = (C.c_int64 * n)(*[s[2] for s in segs])
+    data = np.empty(2 * len(x), np.float32)
+    data[0::2], data[1::2] = x.real, x.imag
+    out = np.zeros(2 * 4096, np.float32)
+    nt, ns = C.c_int32(), C.c_int64()
+    k = orc.orc_pcal_extract_c(bw, spacing, int(offset), kind, minres, int(dsb), n, so, st, ln,
+                               data.ctypes.data_as(P(C.c_float)), out.ctypes.data_as(P(C.c_float)),
+                               C.byref(nt), C.byref(ns))
+    assert k >= 0
+    z = out[:2 * nt.value].reshape(-1, 2).astype(np.float64)
+    return k, z[:, 0] + 1j * z[:, 1], ns.value
+
+
+def comb_c(freqs_hz, fs, n, so, phases):
+    """sum_k exp(i (2 pi f_k (so + m) / fs + phase_k)), m < n, at absolute sample so + m."""
+    m = so + np.arange(n, dtype=np.float64)
+    x = np.zeros(n, np.complex128)
+    for f, ph in zip(freqs_hz, phases):
+        x += np.exp(1j * (2 * np.pi * np.mod(f * m, fs) / fs + ph))
+    return x.astype(np.complex64)
+
+
+def test_complex_getnew_choices(orc_c):
+    """PCal::getNew for complex data (pcal.cpp:186-191): fs = bandwidth; the
+    offset period (doubled to the minimum resolution) a multiple of the comb
+    period -> implicit shift, else the shifting extractor; no tone -> dummy."""
+    x = np.zeros(64, np.complex64)
+    k = lambda bw, off, sp, res=1e6: extract_c(orc_c, bw, sp, off, -1, res, 0, [(0, 0, 64)], x)[0]
+    # No = 16e6 / 0.5e6 = 32, Np = 16 -> implicit
+    assert k(16e6, 0.5e6, 1e6) == PC_CIMPLICIT
+    # No = 8 (16e6 / (2 x 8) = 1 MHz stops the doubling), Np = 16 -> shifting
+    assert k(16e6, 2e6, 3e6) == PC_CSHIFT
+    # a 10 kHz resolution doubles No to 1024 -> implicit
+    assert k(16e6, 2e6, 3e6, 10e3) == PC_CIMPLICIT
+    assert k(16e6, -1, 1e6) == PC_DUMMY
+    # tones past the band: none fit -> dummy
+    assert k(1e6, 2e6, 1e6) == PC_DUMMY
+
+
+@pytest.mark.parametrize("dsb", [0, 1])
+@pytest.mark.parametrize("bw,offset,spacing,kind,bins", [
+    # tones 0.5, 1.5, ... 15.5 MHz; N = 32 bins of 0.5 MHz: tone k in bin 1 + 2k
+    (16e6, 0.5e6, 1e6, PC_CIMPLICIT, [1 + 2 * k for k in range(16)]),
+    # tones 2, 5, 8, 11, 14 MHz mixed down by 2 MHz; N = 16 bins of 1 MHz: bin 3k
+    (16e6, 2e6, 3e6, PC_CSHIFT, [3 * k for k in range(5)]),
+    # the shifting extractor on the implicit case: N = 16, mixer period 32, bin k
+    (16e6, 0.5e6, 1e6, PC_CSHIFT, list(range(16))),
+])
+def test_complex_tones_closed_form(orc_c, bw, offset, spacing, kind, bins, dsb):
+    """A complex comb exp(i (2 pi f_k t + k slope)) locked to the absolute
+    sample number comes back as count x exp(i k slope) in tone k: the
+    extractor mixes the first tone to DC (shifting), bins by absolute sample
+    number mod N and reads DFT bin [offset bin +] k x step.  Double sideband
+    reads the DFT reversed (pcal.cpp:906-913, 1269-1276): a comb at bins
+    N - 1 - bin_k then lands in tone k."""
+    fs = bw
+    so, n = 11, 32 * 200
+    N = {PC_CIMPLICIT: 32, PC_CSHIFT: 16}[kind]
+    slope = np.radians(7.0)
+    if kind == PC_CSHIFT:
+        f = [offset + (N - 1 - b if dsb else b) * fs / N for b in bins]
+    else:
+        f = [(N - 1 - b if dsb else b) * fs / N for b in bins]
+    x = comb_c(f, fs, n + so, 0, slope * np.arange(len(bins)))
+    k, tones, count = extract_c(orc_c, bw, spacing, offset, kind, 1e6, dsb,
+                                [(so, so, n // 2), (so + n // 2, so + n // 2, n // 2)], x)
+    assert k == kind and count == n and len(tones) == len(bins)
+    expect = count * np.exp(1j * slope * np.arange(len(bins)))
+    assert np.max(np.abs(tones - expect)) < 2e-4 * count
+
+
+@pytest.mark.parametrize("kind,bw,offset,spacing", [(PC_CIMPLICIT, 8e6, 250e3, 1e6),
+                                                     (PC_CSHIFT, 16e6, 2e6, 3e6)])
+@pytest.mark.parametrize("dsb", [0, 1])
+def test_complex_equals_direct_sum(orc_c, kind, bw, offset, spacing, dsb):
+    """On random complex samples in blocks at their own sample offsets (the
+    per-block adjustSampleOffset of mode.cpp:777), tone k equals the direct
+    sum over samples of x[m] exp(-2 pi i (offset (shifting) / fs + b_k / N)
+    (so + m)), b_k the tone's DFT bin (N - 1 - b_k for double sideband): the
+    bin folding, the mixer table and the tail handling change nothing."""
+    fs = bw
+    rng = np.random.default_rng(31)
+    n = 40000
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    cuts = [0, 4096, 9001, 20000, 33333, n]
+    so0 = 123
+    segs = [(so0 + a, a, b - a) for a, b in zip(cuts[:-1], cuts[1:])]
+    k, tones, _ = extract_c(orc_c, bw, spacing, offset, kind, 1e6, dsb, segs, x)
+    assert k == kind
+    if kind == PC_CIMPLICIT:
+        N = int(fs // np.gcd(int(fs), int(offset)))
+        while fs / (2 * N) > 1e6:
+            N *= 2
+        first, mix = int(np.floor(N * offset / fs)), 0.0
+    else:
+        N = int(fs // np.gcd(int(fs), int(spacing)))
+        while fs / (2 * N) > 1e6:
+            N *= 2
+        first, mix = 0, offset
+    step = int(np.floor(N * spacing / fs))
+    t = so0 + np.arange(n, dtype=np.float64)
+    xd = x.astype(np.complex128)
+    for j, v in enumerate(tones):
+        b = first + j * step
+        if b >= N:
+            assert v == 0
+            continue
+        b = N - 1 - b if dsb else b
+        ph = np.mod(mix * t, fs) / fs + np.mod(b * t, N) / N
+        ref = np.sum(xd * np.exp(-2j * np.pi * ph))
+        assert abs(v - ref) < 2e-3 * np.sqrt(n), (j, v, ref)
