@@ -55,6 +55,10 @@ __global__ __launch_bounds__(256) void k_pcal_accum(PcalArgs A) {
     int sor = pc.rot ? (int)(so % pc.period) : 0;
     if (sor < 0) sor += pc.period;
     const bool fast = G.allvalid != 0;
+    // one band of 2-bit complex samples: window sample L/2 (even start) or
+    // the imaginary part of sample L/2 - 1 (odd start) is erased
+    const int nerase = pc.csamp && S.granularity > 1 ? L / 2 - (int)(t0 & 1) : -1;
+    const bool erase_re = (t0 & 1) == 0;
     for (int b = tid; b < N; b += 256) {
       int n = b - som;
       if (n < 0) n += N;
@@ -70,6 +74,10 @@ __global__ __launch_bounds__(256) void k_pcal_accum(PcalArgs A) {
                              : make_float2(pick4(lut2, code & 3u), pick4(lut2, code >> 2));
           } else {
             x = sample_value_c(S, Q, A.ftab, lutc, t0 + n, G.validend, B.band);
+            if (n == nerase) {                 // the unpacker's erasure (k_prologue)
+              x.y = 0.0f;
+              if (erase_re) x.x = 0.0f;
+            }
           }
           if (pc.rot) {                        // vectorMul_cf32 by the mixer
             int m = sor + n;

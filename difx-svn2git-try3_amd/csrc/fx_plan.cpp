@@ -239,10 +239,6 @@ int check_desc(const fxg_job_desc* d, std::string* err) {
             "all datastreams of a job must share real or complex sampling (one FFT length per job)");
     if (ds.sampling == FXG_SAMPLING_COMPLEX) {
       REQUIRE(ds.format == FXG_FORMAT_VDIF, FXG_EUNSUPPORTED, "complex sampling is VDIF only");
-      // a unit below a byte (1-band 2-bit) has a sample granularity of 2, whose
-      // erasure in Mk5Mode::unpack (mk5mode.cpp:121-140) runs over the complex
-      // array's floats as real samples
-      REQUIRE(ds.nrecbands * 2 * ds.nbits >= 8, FXG_EUNSUPPORTED, "complex sampling needs >= 8 bits per sample-time");
       REQUIRE(d->fringe_order != 0, FXG_EUNSUPPORTED,
               "post-F fringe rotation is not supported for complex data (mode.cpp:1155-1158)");
     }

@@ -221,6 +221,27 @@ __global__ void k_prologue(const DevStation* __restrict__ st, const DevStationSu
   }
   g.weight = (float)good / (float)S.L;
   g.allvalid = good == S.L ? 1 : 0;
+  if (S.cplx && S.granularity > 1) {
+    // one band of 2-bit complex samples (granularity 2): Mk5Mode::unpack
+    // decodes L + 2 samples from the even sample unpackstart and erases
+    // floats [0, m) and [L + m, L + 2) of the complex array viewed as floats
+    // (mk5mode.cpp:121-140), m = nearestsample - unpackstart.  Its good count
+    // is the valid samples of [unpackstart, unpackstart + L + 2) less the
+    // valid floats erased: m = 0 takes both floats of sample unpackstart +
+    // L/2 (inside the window), m = 1 the real float of sample unpackstart and
+    // the imaginary one of sample unpackstart + L/2.  Blocks of such streams
+    // take the general decode path, which zeroes the erased floats.
+    const long long u0 = unpackstart;
+    auto valid1 = [&](long long u) -> long long {
+      const long long f = u / S.framesamples, base = f * (long long)S.framesamples;
+      return f < Q.nframes ? valid_in_frame(S, ftab[Q.fbase + f], u - base, u - base + 1) : 0;
+    };
+    const long long vhalf = valid1(u0 + S.L / 2), vlast = valid1(u0 + S.L + 1);
+    // good counts the window [nearestsample, nearestsample + L)
+    const long long adj = nearestsample == u0 ? valid1(u0 + S.L) + vlast - 2 * vhalf : vlast - vhalf;
+    g.weight = (float)(good + adj) / (float)S.L;
+    g.allvalid = 0;
+  }
   if (Q.fblank) {
     // INTERLACEDVDIF (mk5mode.cpp:145-168): each band loses the nonzero
     // samples that its frames' EDV4 masks blank, band weight = (good -
@@ -1041,6 +1062,12 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   const long long t0 = G.nearestsample;
   const bool fast = G.allvalid != 0;
   const long long bit0 = t0 * UB;
+  // one band of 2-bit complex samples (granularity 2): the unpacker's erasure
+  // by float index zeroes window sample L/2 (even start) or the imaginary
+  // part of window sample L/2 - 1 (odd start); k_prologue sends such blocks
+  // down the general decode path below
+  const int nerase = S.granularity > 1 ? L / 2 - (int)(t0 & 1) : -1;
+  const bool erase_re = (t0 & 1) == 0;
   if (fast) {
     const long long w0 = bit0 >> 5;
     const unsigned* src = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
@@ -1105,6 +1132,10 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
           x = make_float2(lut8s[code & ((1u << NBITS) - 1u)], lut8s[code >> NBITS]);
         } else {
           x = sample_value_c(S, Q, A.ftab, lut8s, t0 + n, G.validend, B.band);
+          if (n == nerase) {
+            x.y = 0.0f;
+            if (erase_re) x.x = 0.0f;
+          }
         }
         const int kk = slog >= 0 ? (n & (s - 1)) : n % s;
         const int jj = slog >= 0 ? (n >> slog) : n / s;

@@ -114,8 +114,8 @@ static int make_fmt(const fxg_datastream* ds, StreamFmt* f) {
   int bitsper = f->nchan * f->unitbits;
   /* complex units smaller than a byte (1-channel 2-bit) have a sample
    * granularity of 2, whose erasure in Mk5Mode::unpack (mk5mode.cpp:121-140)
-   * runs over the complex array's floats as if they were real samples */
-  if (f->cplx && bitsper < 8) return fail(FXG_EUNSUPPORTED, "complex sampling needs >= 8 bits per sample-time");
+   * runs over the complex array's floats as if they were real samples
+   * (mk5_unpack below does the same) */
   if (f->format == FXG_FORMAT_VDIF) {
     if (f->nbits == 2) lut_vdif2(f->lut);
     else if (f->nbits == 8) lut_vdif8(f->lut);
@@ -306,9 +306,10 @@ static bool blank_edv4(const StreamFmt& f, const uint8_t* data, int64_t nbytes, 
   return true;
 }
 
-/* Mk5Mode::unpack (mk5mode.cpp:104-177) for real sampling.  `work` holds
- * nchan*(unpacksamples+granularity) floats; on return the L samples starting
- * at sampleoffset for channel c are at work[c*stride + munged + n].
+/* Mk5Mode::unpack (mk5mode.cpp:104-177).  `work` holds
+ * nchan*(unpacksamples+granularity) samples (complex: float pairs); on return
+ * the L samples starting at sampleoffset for channel c are at
+ * work[c*stride + munged + n] (complex: that sample index times 2).
  * INTERLACEDVDIF: bandweight[c] = (goodsamples - invalid[c]) / L, and the
  * block weight loses the mean invalid count (mk5mode.cpp:145-162). */
 struct Unpacked { int stride; int munged; float weight; std::vector<float> bandweight; };
@@ -320,16 +321,21 @@ static Unpacked mk5_unpack(const StreamFmt& f, const uint8_t* data, int64_t nbyt
   float goodsamples = (float)decode(f, data, nbytes, unpackstart, samplestounpack, work.data());
   int munged = 0;
   if (f.granularity > 1) {
+    /* index i runs over the band array's floats: for complex samples (the
+     * array viewed as floats, real then imaginary) the erasure reaches the
+     * real part of sample 0 and the floats of sample (unpacksamples +
+     * munged) / 2, inside the block's window */
     int erased = 0;
+    const size_t bstride = (size_t)samplestounpack * (f.cplx ? 2 : 1);
     munged = (int)(sampleoffset % f.granularity);
     for (int i = 0; i < munged; i++)
       for (int b = 0; b < f.nchan; b++) {
-        float& v = work[(size_t)b * samplestounpack + i];
+        float& v = work[(size_t)b * bstride + i];
         if (v != 0.0f) { v = 0.0f; erased++; }
       }
     for (int i = unpacksamples + munged; i < samplestounpack; i++)
       for (int b = 0; b < f.nchan; b++) {
-        float& v = work[(size_t)b * samplestounpack + i];
+        float& v = work[(size_t)b * bstride + i];
         if (v != 0.0f) { v = 0.0f; erased++; }
       }
     goodsamples -= erased / (float)f.nchan;

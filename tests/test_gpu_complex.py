@@ -2,7 +2,8 @@
 8-bit) through the GPU against the oracle: the FP32 path within 1e-5 of the
 FP32 oracle (spectra and visibilities), reduced precision against the FP64
 oracle, with a bad frame (a complex 8-bit decoder stops at it) and a
-flagged block."""
+flagged block; one band of 2-bit complex samples, whose unpacker erases a
+sample inside each block's window."""
 import ctypes as C
 
 import numpy as np
@@ -10,7 +11,7 @@ import pytest
 
 from fxcorr import JobSpec
 from fxsynth import make_subint
-from fxtestlib import GpuJob, oracle_results, oracle_spectra, rel_err
+from fxtestlib import GpuJob, oracle_block_info, oracle_results, oracle_spectra, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -97,4 +98,45 @@ def test_complex_8bit_largest(gpu, oracle, nchan):
     sp = oracle_spectra(oracle, job, sd, 2, 4, spec.nrecbands, spec.nchan)
     for b in range(spec.nrecbands):
         assert rel_err(g.spectrum(2, b, 4, spec.nchan), sp[b]) < FP32_TOL, b
+    g.close()
+
+
+@pytest.mark.parametrize("dsb,order", [(0, 1), (1, 2), (0, 2)])
+def test_complex_single_band_2bit(gpu, oracle, dsb, order):
+    """One band of 2-bit complex samples: 4 bits per sample-time, sample
+    granularity 2, so Mk5Mode::unpack erases by float index inside the
+    window (sample L/2 for an even block start, the imaginary part of sample
+    L/2 - 1 for an odd one, mk5mode.cpp:121-140) and moves the weight.
+    Block geometry and weights exactly, spectra and visibilities (phase cal
+    included) within 1e-5 of the FP32 oracle, blocks of both start parities,
+    with a bad frame and a flagged block."""
+    lib, ctx = gpu
+    spec = cspec(0, dsb, 2, nsubbands=1, npol=1, nstations=3, blockspersend=40, fringe_order=order,
+                 pcal_interval_mhz=1.0, base_freq_mhz=8400.01)
+    assert spec.nrecbands * spec.unitbits == 4
+    job, keep = spec.build()
+    # whole-sample delays of 48, -81 and 113 samples: even and odd block
+    # starts; frame 3 of stations 0 (even starts) and 1 (odd starts) is bad,
+    # from inside block 31, whose erased sample falls in it
+    sd = make_subint(spec, seed=14, rho=0.05, delays_us=[3.0, -5.0625, 7.0625], rates_us_per_s=[0.3, -0.5, 0.2],
+                     invalid_frames={0: [3], 1: [3]}, invalid_blocks={2: [3]})
+    g = GpuJob(lib, ctx, job)
+    got = g.process(sd, keep_taps=True)
+    parity = set()
+    for ds in range(spec.nstations):
+        for blk in range(spec.blockspersend):
+            a = g.block_info(ds, blk)
+            b = oracle_block_info(oracle, job, sd, ds, blk)
+            for f in ("status", "nearestsample", "integerdelay"):
+                assert getattr(a, f) == getattr(b, f), (ds, blk, f)
+            if b.status == 0:
+                parity.add(b.nearestsample % 2)
+                for f in ("weight", "fracsampleerror"):
+                    assert getattr(a, f) == getattr(b, f), (ds, blk, f, getattr(a, f), getattr(b, f))
+    assert parity == {0, 1}
+    ref = oracle_results(oracle, job, sd, fp64=False, nthreads=8)
+    assert rel_err(got, ref) < FP32_TOL, rel_err(got, ref)
+    for ds, blk in ((0, 0), (1, 5), (1, 31), (2, 7)):
+        sp = oracle_spectra(oracle, job, sd, ds, blk, 1, spec.nchan)
+        assert rel_err(g.spectrum(ds, 0, blk, spec.nchan), sp[0]) < FP32_TOL, (ds, blk)
     g.close()

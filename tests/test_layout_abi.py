@@ -218,9 +218,9 @@ def test_plan_without_gpu_and_error_paths():
 def test_complex_sampling_plan_rules():
     """Complex sampling: fftchannels = nchan; what the reference refuses
     (post-F fringe rotation) or this build does not take (mixed real and
-    complex datastreams, 1-band 2-bit complex) is refused at planning, by the
-    library and by the oracle alike.  Phase cal of complex data plans, with
-    the same result layout in both."""
+    complex datastreams) is refused at planning, by the library and by the
+    oracle alike.  One band of 2-bit complex samples and phase cal of complex
+    data plan, with the same result layout in both."""
     lib, orc = fxcorr.load_gpu_lib(), fxcorr.load_oracle()
     info = PlanInfo()
     job, keep = JobSpec(nstations=3, nsubbands=2, npol=2, bw_mhz=16.0, nchan=256, sampling="complex").build()
@@ -233,7 +233,8 @@ def test_complex_sampling_plan_rules():
     job.datastreams[1].sampling = 0
     assert lib.fxg_plan(C.byref(job), C.byref(info)) == -4
     one, k3 = JobSpec(nstations=2, nsubbands=1, npol=1, bw_mhz=16.0, nchan=256, sampling="complex").build()
-    assert lib.fxg_plan(C.byref(one), C.byref(info)) == -4
+    assert lib.fxg_plan(C.byref(one), C.byref(info)) == 0
+    assert orc.orc_plan(C.byref(one), C.byref(ref)) == 0 and ref.result_length == info.result_length
     pc, k4 = JobSpec(nstations=2, nsubbands=1, npol=2, bw_mhz=16.0, nchan=256, sampling="complex",
                      pcal_interval_mhz=1.0).build()
     assert lib.fxg_plan(C.byref(pc), C.byref(info)) == 0

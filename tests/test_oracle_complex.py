@@ -90,15 +90,45 @@ def test_complex_8bit_stops_at_blank_frame(oracle):
     assert abs(w - cut / n) < 1e-6
 
 
-def test_complex_single_channel_2bit_refused(oracle):
-    spec = JobSpec(nstations=2, nsubbands=1, npol=1, bw_mhz=16.0, nchan=128, nbits=2, sampling="complex")
+@pytest.mark.parametrize("first", [1000, 1001, 2 * 16000 - 37, 2 * 16000 - 36, 2 * 16000 - 65, 2 * 16000 - 64])
+def test_complex_single_channel_2bit_erasure(oracle, first):
+    """One band of 2-bit complex samples: 4 bits per sample-time, so the
+    decoder works on byte pairs (sample granularity 2, format_vdif.c:3845)
+    and Mk5Mode::unpack unpacks nchan + 2 samples from the even sample at or
+    before the block start, then erases floats [0, munged) and [nchan +
+    munged, nchan + 2) of the band array (mk5mode.cpp:121-140): indices of
+    floats, not of complex samples.  Viewed as floats, real then imaginary,
+    that zeroes the whole sample nchan / 2 (even start) or the real part of
+    the sample before the window and the imaginary part of sample nchan / 2
+    (odd start), and the weight is (decoded good samples - nonzero floats
+    erased) / nchan.  Checked against the frames' bits read with numpy, with
+    a frame marked invalid so that the window runs into it."""
+    spec = JobSpec(nstations=1, nsubbands=1, npol=1, bw_mhz=16.0, nchan=128, nbits=2, blockspersend=4,
+                   sampling="complex")
     job, keep = spec.build()
-    info = fxcorr.PlanInfo()
-    sd = make_subint(spec, seed=1)
-    out = np.zeros(64, dtype=np.complex64)
-    sub = sd.subint_desc()
-    st, k2 = sd.station_inputs()
-    assert oracle.orc_spectra(C.byref(job), C.byref(sub), st, 0, 0, 0, out.ctypes.data) == fxcorr.FXG_EUNSUPPORTED
+    fs = spec.framesamples
+    assert fs == 16000 and spec.unitbits == 4
+    sd = make_subint(spec, seed=6, delays_us=[0.0], invalid_frames={0: [2]})
+    n = spec.nchan
+    munged = first % 2
+    start = first - munged
+    got, w = unpack_complex(oracle, spec, job, sd, 0, first, n)
+    dec = numpy_decode(spec, sd.stations[0].raw, start, n + 2)[0]
+    # the decoder's blanking: a frame marked invalid decodes as zeros
+    f_of = (start + np.arange(n + 2)) // fs
+    dec[f_of == 2] = 0
+    good = int(np.count_nonzero(f_of != 2))
+    fl = np.empty(2 * (n + 2), np.float64)
+    fl[0::2], fl[1::2] = dec.real, dec.imag
+    erase = list(range(munged)) + list(range(n + munged, n + 2))
+    erased = int(np.count_nonzero(fl[erase]))
+    fl[erase] = 0.0
+    want = (fl[0::2] + 1j * fl[1::2])[munged:munged + n]
+    np.testing.assert_array_equal(got[0], want)
+    assert w == np.float32(np.float32(good) - np.float32(erased)) / np.float32(n)
+    # the erased sample sits inside the window
+    k = n // 2 - munged
+    assert want[k].imag == 0 and (munged or want[k].real == 0)
 
 
 def lo_of(spec, sb, lsb, dsb):
