@@ -149,7 +149,74 @@ def time_job(lib, ctx, spec, sd, raws, steps, warmup, torch, autotune=False, gra
     job.close()
     del dres
     time_job.schedule = {"schedule": sched.as_dict(), "tuning": tuning, "graph": graph}
+    time_job.sched = sched
     return el, lat, ms, n, info
+
+
+def pcie_inclusive(lib, ctx, spec, sd, raws, steps, torch, sched, graph):
+    """The same subintegration with the raw VDIF handed over in pinned host
+    memory.  Submitted back to back (fxg_subint_submit), each subint's PCIe
+    copy runs on the job's copy stream under the previous one's kernels;
+    one at a time (fxg_subint_process), copy and kernels add up.  Reported
+    beside the headline, never as it."""
+    from fxcorr import STAGE_NAMES, check
+    from fxsynth import station_inputs_device
+    host = []
+    for r in raws:
+        h = torch.empty(r.numel(), dtype=torch.uint8, pin_memory=True)
+        h.copy_(r)
+        host.append(h)
+    torch.cuda.synchronize()
+    job = Job(lib, ctx, spec)
+    check(lib, lib.fxg_job_set_schedule(job.h, C.byref(sched)), "set_schedule")
+    if graph:
+        check(lib, lib.fxg_job_set_graph(job.h, 1), "set_graph")
+    sub = sd.subint_desc()
+    st, keep = station_inputs_device(sd, host)           # host pointers this time
+    dres = torch.empty(2 * job.info.result_length, dtype=torch.float32, device="cuda:0")
+
+    def run():
+        check(lib, lib.fxg_subint_process(job.h, C.byref(sub), st, 0, C.c_void_p(dres.data_ptr()), 1),
+              "fxg_subint_process")
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    el_sync = time.perf_counter() - t0
+    # steady state: the period between the first and the last completion of
+    # nsub back-to-back submissions (the first copy has no kernels to hide
+    # under, the last kernels no copy)
+    nsub = 4 * steps
+    tickets = []
+    for _ in range(nsub):
+        t = C.c_int64()
+        check(lib, lib.fxg_subint_submit(job.h, C.byref(sub), st, 0, C.c_void_p(dres.data_ptr()), 1,
+                                         C.byref(t)), "fxg_subint_submit")
+        tickets.append(t.value)
+    check(lib, lib.fxg_subint_wait(job.h, tickets[0]), "fxg_subint_wait")
+    t0 = time.perf_counter()
+    check(lib, lib.fxg_subint_wait(job.h, tickets[-1]), "fxg_subint_wait")
+    el = (time.perf_counter() - t0) / (nsub - 1) * steps
+    lib.fxg_job_set_profiling(job.h, 1)
+    job.stage_times(reset=True)
+    run()
+    ms, n = job.stage_times(reset=True)
+    lib.fxg_job_set_profiling(job.h, 0)
+    job.close()
+    nbytes = sum(int(h.numel()) for h in host)
+    del dres, host
+    h2d = ms[STAGE_NAMES.index("h2d")]
+    samples = spec.nstations * spec.nrecbands * spec.blockspersend * spec.fftchannels
+    return {"value": samples * steps / el / 1e9, "unit": "Gsamples/s", "ms_per_step": el / steps * 1e3,
+            "submissions": nsub, "one_at_a_time": {"value": samples * steps / el_sync / 1e9,
+                                                  "ms_per_step": el_sync / steps * 1e3, "steps": steps},
+            "h2d_ms": h2d, "h2d_gbs": nbytes / (h2d * 1e-3) / 1e9 if h2d > 0 else None,
+            "raw_bytes": nbytes,
+            "input": "raw VDIF in pinned host memory; submitted back to back, each subint's copy overlaps "
+                     "the previous subint's kernels; value = the steady-state period between completions"}
 
 
 def vis_errors(lib, ctx, orc, seed, blocks, threads, torch):
@@ -589,6 +656,8 @@ def main():
     ap.add_argument("--subint", type=float, default=0.128, help="uneven: subintegration length in seconds")
     ap.add_argument("--no-serving", action="store_true", help="default run: skip the serving workload")
     ap.add_argument("--no-uneven", action="store_true", help="default run: skip the six uneven jobs")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="skip the run with the raw data in pinned host memory (PCIe copy included)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -650,6 +719,13 @@ def main():
     value = samples * args.steps / el / 1e9
     log(f"  {value:.1f} Gsamples/s  ({el / args.steps * 1e3:.1f} ms/subint)")
     stage = {fxcorr.STAGE_NAMES[i]: {"ms": ms[i], "launches": n[i]} for i in range(len(ms)) if n[i]}
+
+    pcie = None
+    if not args.no_pcie:
+        log("timing with the raw data in pinned host memory (PCIe copy included)")
+        pcie = pcie_inclusive(lib, ctx, spec, sd, raws, 2, torch, time_job.sched, args.graph)
+        log(f"  {pcie['value']:.1f} Gsamples/s ({pcie['ms_per_step']:.1f} ms/subint back to back; one at a "
+            f"time {pcie['one_at_a_time']['ms_per_step']:.1f} ms, h2d {pcie['h2d_ms']:.1f} ms)")
 
     fp32 = None
     if not args.no_fp32 and args.precision != "fp32":
@@ -737,6 +813,7 @@ def main():
         "latency_ms": job_lat,
         "stages_ms": {k: v["ms"] for k, v in stage.items()},
         "plain_fp32": fp32,
+        "pcie_inclusive": pcie,
         "mixed_vs_fp32": (value / fp32["value"]) if fp32 else None,
         "vis_error_vs_fp64": errors,
         "roofline": roof,

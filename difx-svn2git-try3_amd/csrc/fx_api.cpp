@@ -116,6 +116,16 @@ struct fxg_job {
   DBuf d_bandlist;                // global bands grouped by sample size
   int nb2 = 0, nb8 = 0;           // bands of 2-bit and 8-bit datastreams
   std::vector<DBuf> d_raw;        // staged host data per datastream
+  // host inputs of fxg_subint_process / fxg_subint_submit: two sets of raw
+  // buffers filled on a copy stream, so that one subintegration's PCIe copy
+  // runs under the previous one's kernels.  raw_free[k] is recorded on the
+  // job's stream after the last kernel of the subintegration that read set
+  // k; raw_copied[k] after set k's copies on the copy stream.
+  std::vector<DBuf> d_rawp[2];
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t raw_free[2] = {nullptr, nullptr}, raw_copied[2] = {nullptr, nullptr};
+  bool raw_free_rec[2] = {false, false};
+  int rawp_next = 0;
   // big buffers
   DBuf d_spec, d_V;
   DBuf d_z;                       // FFTs above 16384 points: the sub-FFT scratch
@@ -537,6 +547,13 @@ static void job_free(fxg_job* j) {
                   &j->d_eoff, &j->d_kurt, &j->d_kurtout, &j->d_kurtnz};
   for (DBuf* b : bufs) b->release();
   for (DBuf& b : j->d_raw) b.release();
+  if (j->copy_stream) { (void)hipStreamSynchronize(j->copy_stream); (void)hipStreamDestroy(j->copy_stream); j->copy_stream = nullptr; }
+  for (int k = 0; k < 2; k++) {
+    for (DBuf& b : j->d_rawp[k]) b.release();
+    if (j->raw_free[k]) (void)hipEventDestroy(j->raw_free[k]);
+    if (j->raw_copied[k]) (void)hipEventDestroy(j->raw_copied[k]);
+    j->raw_free[k] = j->raw_copied[k] = nullptr;
+  }
   j->d_digest.release();
   for (auto& hs : j->stage) {
     if (hs.done) (void)hipEventSynchronize(hs.done), (void)hipEventDestroy(hs.done);
@@ -886,11 +903,29 @@ struct LaunchShape {
 };
 
 // Raw data (host inputs), the per-datastream subint table and the valid
-// flags of one subintegration, copied to the device on stream s.
+// flags of one subintegration, copied to the device on stream s.  With
+// raw_set non-null, host inputs go into the next of the two pipelined raw
+// sets on the copy stream (s waits for them); *raw_set receives the set, or
+// -1 when nothing was copied that way.
 static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_station_input* st,
-                         int data_on_device, hipStream_t s, LaunchShape* sh) {
+                         int data_on_device, hipStream_t s, LaunchShape* sh, int* raw_set = nullptr) {
   Plan& P = j->plan;
   int rc;
+  int set = -1;
+  if (raw_set) {
+    *raw_set = -1;
+    if (!data_on_device) {
+      set = j->rawp_next;
+      if (!j->copy_stream) HIPCHK(hipStreamCreateWithFlags(&j->copy_stream, hipStreamNonBlocking));
+      for (int k = 0; k < 2; k++) {
+        if (!j->raw_free[k]) HIPCHK(hipEventCreateWithFlags(&j->raw_free[k], hipEventDisableTiming));
+        if (!j->raw_copied[k]) HIPCHK(hipEventCreateWithFlags(&j->raw_copied[k], hipEventDisableTiming));
+      }
+      j->d_rawp[set].resize(P.nds);
+      // the set's previous reader must have finished before it is refilled
+      if (j->raw_free_rec[set]) HIPCHK(hipStreamWaitEvent(j->copy_stream, j->raw_free[set], 0));
+    }
+  }
   std::vector<DevStationSub> ss(P.nds);
   std::vector<int> flags;
   long long fbase = 0, maxframes = 0, ctbytes = 0, maxsamples = 0;
@@ -901,7 +936,14 @@ static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stati
     DevStationSub& q = ss[d];
     if (in.nbytes < 0 || (in.nbytes > 0 && !in.data)) return set_err(FXG_EINVAL, "station data");
     const unsigned char* dp = in.data;
-    if (!data_on_device && in.nbytes > 0) {
+    if (!data_on_device && in.nbytes > 0 && set >= 0) {
+      DBuf& b = j->d_rawp[set][d];
+      // a larger buffer replaces the set's: its last reader must be done
+      if ((size_t)in.nbytes > b.bytes && j->raw_free_rec[set]) HIPCHK(hipEventSynchronize(j->raw_free[set]));
+      if ((rc = b.ensure((size_t)in.nbytes))) return rc;
+      HIPCHK(hipMemcpyAsync(b.p, in.data, (size_t)in.nbytes, hipMemcpyHostToDevice, j->copy_stream));
+      dp = b.as<unsigned char>();
+    } else if (!data_on_device && in.nbytes > 0) {
       if ((rc = j->d_raw[d].ensure((size_t)in.nbytes))) return rc;
       if ((rc = j->ev_begin(FXG_STAGE_H2D))) return rc;
       HIPCHK(hipMemcpyAsync(j->d_raw[d].p, in.data, (size_t)in.nbytes, hipMemcpyHostToDevice, s));
@@ -936,6 +978,12 @@ static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stati
     for (int k = 0; k < q.nflags; k++) flags.push_back(in.validflags[k]);
     for (int k = 0; k < 3; k++) q.interp[k] = in.interpolator[k];
     q.offsec = sub->sec; q.offns = sub->ns; q.scanstart = sub->scan_start_sec; q.pad = 0;
+  }
+  if (set >= 0) {
+    HIPCHK(hipEventRecord(j->raw_copied[set], j->copy_stream));
+    HIPCHK(hipStreamWaitEvent(s, j->raw_copied[set], 0));
+    j->rawp_next ^= 1;
+    *raw_set = set;
   }
   if ((rc = j->d_ss.ensure(sizeof(DevStationSub) * ss.size())) ||
       (rc = j->d_flags.ensure(sizeof(int) * std::max<size_t>(1, flags.size()))) ||
@@ -1211,7 +1259,10 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
   int rc;
   j->prepared_key.clear();
   LaunchShape sh;
-  if ((rc = upload_subint(j, sub, st, data_on_device, s, &sh))) return rc;
+  // host inputs without stage profiling: pipelined copies (the profiled
+  // path keeps them on the job's stream, where the H2D stage is timed)
+  int raw_set = -1;
+  if ((rc = upload_subint(j, sub, st, data_on_device, s, &sh, j->profiling ? nullptr : &raw_set))) return rc;
   float* dres = results_on_device ? results : j->d_results.as<float>();
   if (j->use_graph && !j->profiling) {
     std::vector<unsigned char> key = graph_key(j, sh, dres);
@@ -1230,6 +1281,10 @@ static int enqueue_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stat
   } else {
     if ((rc = launch_front(j, sh, s))) return rc;
     if ((rc = launch_back(j, dres, s))) return rc;
+  }
+  if (raw_set >= 0) {
+    HIPCHK(hipEventRecord(j->raw_free[raw_set], s));
+    j->raw_free_rec[raw_set] = true;
   }
   if (!results_on_device) {
     const size_t resbytes = sizeof(float) * 2 * (size_t)P.info.result_length;

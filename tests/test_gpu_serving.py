@@ -2,6 +2,9 @@
 request loop (include/fxcorr_serve.h) on the GPU.
 
   - a submitted subintegration gives the same results as fxg_subint_process;
+  - host inputs go through two raw buffer sets on a copy stream: submissions
+    of alternating data, eagerly and as graph launches, each equal the same
+    data processed alone with its raw bytes already on the device;
   - several configurations of different sizes, with requests interleaved so
     that work of different configurations and slots overlaps (with no bound
     on the requests on the device at once, and with one or two started in
@@ -84,6 +87,46 @@ def test_submit_wait_equals_process(gpu):
     for out in outs:
         assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
     assert lib.fxg_subint_wait(g.h, tickets[-1] + 5) == fxcorr.FXG_EINVAL
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [0, 1])
+def test_host_inputs_pipelined(gpu, graph):
+    """Host raw data copied on the job's copy stream into one of two buffer
+    sets while the previous submission's kernels run: six submissions of
+    three different subintegrations in turn (a set is refilled while the
+    other is read), bit for bit equal to device-resident inputs."""
+    lib, ctx = gpu
+    spec = SPECS[1]
+    job, keep = spec.build()
+    sds = [make_subint(spec, seed=60 + k, invalid_frames={k: [1]}) for k in range(3)]
+    g = GpuJob(lib, ctx, job)
+    n = g.info.result_length
+    refs = []
+    for sd in sds:
+        dev = DeviceSubint(lib, ctx, sd)
+        res = device_results(lib, ctx, n)
+        check(lib, lib.fxg_subint_process(g.h, C.byref(dev.sub), dev.st, 1, res, 1), "process")
+        refs.append(fetch(lib, ctx, res, n))
+        lib.fxg_device_free(ctx, res)
+        dev.free()
+    check(lib, lib.fxg_job_set_graph(g.h, graph), "graph")
+    ins = [sd.station_inputs() for sd in sds]
+    subs = [sd.subint_desc() for sd in sds]
+    outs, tickets = [], []
+    for r in range(6):
+        k = r % 3
+        out = np.zeros(n, dtype=np.complex64)
+        t = C.c_int64()
+        check(lib, lib.fxg_subint_submit(g.h, C.byref(subs[k]), ins[k][0], 0, out.ctypes.data, 0, C.byref(t)),
+              "submit")
+        outs.append((k, out))
+        tickets.append(t.value)
+    check(lib, lib.fxg_subint_wait(g.h, tickets[-1]), "wait")
+    for k, out in outs:
+        assert np.array_equal(out.view(np.uint32), refs[k].view(np.uint32)), k
+    assert not np.array_equal(refs[0], refs[1])
     g.close()
 
 
