@@ -292,7 +292,9 @@ int fxg_job_plan_info(const fxg_job* job, fxg_plan_info* info);
 
 /* Process one subintegration: stations[ndatastreams].  If data_on_device is
  * nonzero, stations[i].data are device pointers that stay valid until the call
- * returns; otherwise they are host pointers copied in by the call.  results
+ * returns; otherwise they are host pointers copied in by the call (on the
+ * job's copy stream, into one of two device buffer sets, with stage
+ * profiling off; on the job's stream with it on).  results
  * receives result_length cf32 values (interleaved re,im floats), zeroed and
  * accumulated exactly as a Core slot (core.cpp:319, 1048-1108); if
  * results_on_device is nonzero it is a device pointer. */
@@ -305,7 +307,9 @@ int fxg_subint_process(fxg_job* job, const fxg_subint_desc* sub,
  * can keep several slots of several jobs in flight).  Inputs and results must
  * stay valid until fxg_subint_wait returns for the ticket; with host results
  * (results_on_device == 0) the call may block on the copy.  Tickets of one job
- * complete in submission order. */
+ * complete in submission order.  Host raw data (pinned for an asynchronous
+ * copy) of one submission is copied while the previous submission's kernels
+ * run. */
 int fxg_subint_submit(fxg_job* job, const fxg_subint_desc* sub,
                       const fxg_station_input* stations, int data_on_device,
                       float* results, int results_on_device, int64_t* ticket);
