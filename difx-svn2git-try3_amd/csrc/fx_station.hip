@@ -527,17 +527,23 @@ __global__ __launch_bounds__(256) void k_cornerturn2(const DevStation* __restric
   long long f = fw;
   long long r = base - fw * spf + threadIdx.x;
   while (r >= spf) { r -= spf; f++; }
-  for (int i = threadIdx.x; i < TPW; i += 256) {
-    const long long tau = base + i;
-    VW v;
+  // all 16 loads in flight before the first LDS store
+  VW v[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const long long tau = base + threadIdx.x + k * 256;
     if (tau < total) {
-      v = ((const VW*)(Q.data + f * S.framebytes + S.headerbytes))[r];
+      v[k] = ((const VW*)(Q.data + f * S.framebytes + S.headerbytes))[r];
     } else {
-      memset(&v, 0, sizeof v);
+      memset(&v[k], 0, sizeof v[k]);
     }
-    buf[i + (i >> 4)] = v;
     r += 256;
     while (r >= spf) { r -= spf; f++; }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int i = threadIdx.x + k * 256;
+    buf[i + (i >> 4)] = v[k];
   }
   __syncthreads();
   const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
