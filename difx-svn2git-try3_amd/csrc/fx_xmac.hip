@@ -163,31 +163,54 @@ __global__ __launch_bounds__(256) void k_weights(const DevStation* __restrict__ 
 // scatter: result[seg.resoff + c] = mean of V[seg.voff + c avg + i], i < avg
 // (conjugated if flagged), scaled by vscale; float segments copy one weight
 // ---------------------------------------------------------------------------
-__global__ void k_scatter(const DevSeg* __restrict__ segs, int nseg, const float2* __restrict__ V,
-                          const float* __restrict__ w, float vscale, float* __restrict__ res) {
+constexpr int kScatterPer = 4;   // channels per thread (their loads in flight together)
+
+__global__ __launch_bounds__(256) void k_scatter(const DevSeg* __restrict__ segs, int nseg,
+                                                 const float2* __restrict__ V, const float* __restrict__ w,
+                                                 float vscale, float* __restrict__ res) {
   int sidx = blockIdx.y;
   if (sidx >= nseg) return;
   const DevSeg g = segs[sidx];
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cb = blockIdx.x * (256 * kScatterPer) + threadIdx.x;
   if (g.kind == SEG_FLOATS) {
-    if (c == 0) res[g.resoff] = w[g.voff];
+    if (cb == 0) res[g.resoff] = w[g.voff];
     return;
   }
-  if (c >= g.nchan) return;
-  float2 v;
+  if (cb >= g.nchan) return;
+  float2 v[kScatterPer];
   if (g.avg == 1) {
-    v = V[g.voff + c];
+#pragma unroll
+    for (int q = 0; q < kScatterPer; q++) {
+      const int c = cb + q * 256;
+      v[q] = c < g.nchan ? V[g.voff + c] : make_float2(0.0f, 0.0f);
+    }
   } else {
     // vectorMean_cf32 of the group: sum in channel order, then divide
-    v = make_float2(0.0f, 0.0f);
-    const float2* p = V + g.voff + (long long)c * g.avg;
-    for (int i = 0; i < g.avg; i++) { v.x += p[i].x; v.y += p[i].y; }
-    v.x /= (float)g.avg;
-    v.y /= (float)g.avg;
+#pragma unroll
+    for (int q = 0; q < kScatterPer; q++) {
+      const int c = cb + q * 256;
+      v[q] = make_float2(0.0f, 0.0f);
+      if (c >= g.nchan) continue;
+      const float2* p = V + g.voff + (long long)c * g.avg;
+      for (int i = 0; i < g.avg; i++) { v[q].x += p[i].x; v[q].y += p[i].y; }
+      v[q].x /= (float)g.avg;
+      v[q].y /= (float)g.avg;
+    }
   }
-  if (g.conj) v.y = -v.y;
-  res[2 * (g.resoff + c)] = v.x * vscale;
-  res[2 * (g.resoff + c) + 1] = v.y * vscale;
+  const bool al8 = (reinterpret_cast<unsigned long long>(res) & 7) == 0;
+#pragma unroll
+  for (int q = 0; q < kScatterPer; q++) {
+    const int c = cb + q * 256;
+    if (c >= g.nchan) break;
+    float2 y = v[q];
+    if (g.conj) y.y = -y.y;
+    if (al8) {
+      reinterpret_cast<float2*>(res)[g.resoff + c] = make_float2(y.x * vscale, y.y * vscale);   // one 8-byte store
+    } else {
+      res[2 * (g.resoff + c)] = y.x * vscale;
+      res[2 * (g.resoff + c) + 1] = y.y * vscale;
+    }
+  }
 }
 
 hipError_t launch_xmac_f32(const XmacArgs& a, int prec, hipStream_t stream) {
@@ -252,7 +275,7 @@ hipError_t launch_weights(const DevStation* st, const DevStationSub* ss, const D
 hipError_t launch_scatter(const DevSeg* segs, int nseg, int maxlen, const float2* V, const float* w, float vscale,
                           float* res, hipStream_t stream) {
   if (nseg == 0) return hipSuccess;
-  int blocks = (maxlen + 255) / 256;
+  int blocks = (maxlen + 256 * kScatterPer - 1) / (256 * kScatterPer);
   if (blocks < 1) blocks = 1;
   // grid.y is limited to 65535: launch in slices
   for (int s0 = 0; s0 < nseg; s0 += 65535) {
