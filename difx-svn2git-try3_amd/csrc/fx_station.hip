@@ -695,6 +695,39 @@ __device__ __forceinline__ void zero_spec(void* spectra, int prec, long long out
   }
 }
 
+// The block-independent LDS tables of a single-workgroup F stage (split
+// twiddles, the second pass's 256 twiddles, the 8-bit level table), loaded
+// into registers at the start of the kernel: their loads then overlap the
+// block's record loads and stream-word loads instead of each waiting on its
+// own round trip before its LDS store.
+template <int L, int NT, int NBITS, bool MID>
+struct TabStage {
+  static constexpr int NLO = (64 + NT - 1) / NT, NHI = (L / 64 + NT - 1) / NT;
+  static constexpr int N256 = MID ? (256 + NT - 1) / NT : 0, NL8 = NBITS == 8 ? (256 + NT - 1) / NT : 0;
+  float2 lo[NLO], hi[NHI], t256[N256 > 0 ? N256 : 1];
+  float l8[NL8 > 0 ? NL8 : 1];
+  __device__ __forceinline__ void load(const float2* __restrict__ tw, const float* __restrict__ lut8, int tid) {
+#pragma unroll
+    for (int q = 0; q < NLO; q++) { const int i = tid + q * NT; if (i < 64) lo[q] = tw[i]; }
+#pragma unroll
+    for (int q = 0; q < NHI; q++) { const int i = tid + q * NT; if (i < L / 64) hi[q] = tw[64 + i]; }
+#pragma unroll
+    for (int q = 0; q < N256; q++) { const int i = tid + q * NT; if (i < 256) t256[q] = tw[64 + L / 64 + i]; }
+#pragma unroll
+    for (int q = 0; q < NL8; q++) { const int i = tid + q * NT; if (i < 256) l8[q] = lut8[i]; }
+  }
+  __device__ __forceinline__ void store(float2* twlo, float2* twhi, float2* tw256, float* lut8s, int tid) const {
+#pragma unroll
+    for (int q = 0; q < NLO; q++) { const int i = tid + q * NT; if (i < 64) twlo[i] = lo[q]; }
+#pragma unroll
+    for (int q = 0; q < NHI; q++) { const int i = tid + q * NT; if (i < L / 64) twhi[i] = hi[q]; }
+#pragma unroll
+    for (int q = 0; q < N256; q++) { const int i = tid + q * NT; if (i < 256) tw256[i] = t256[q]; }
+#pragma unroll
+    for (int q = 0; q < NL8; q++) { const int i = tid + q * NT; if (i < 256) lut8s[i] = l8[q]; }
+  }
+};
+
 // Last pass of radix 2 (L = 2 * 16^a) for thread j: butterflies k = j + u L/16,
 // u < 8, with W_L^k = W_L^j W_16^u; only the kept sideband's output of each
 // butterfly; channel c = k has c mod s = j mod s (one fsub value per thread,
@@ -765,6 +798,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   // lines that neighbouring blocks write in the grouped spectra layout)
   const int cb = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
   const int blk = A.chunk0 + cb;
+  TabStage<L, NT, NBITS, (Sh::NMID > 0)> ts;
+  ts.load(A.tw, A.lut8, tid);
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
   const DevStation S = A.st[B.station];
@@ -782,11 +817,17 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   const bool fast = G.allvalid != 0;
   // ---- stage the block's samples of this band (corner-turned stream)
   long long bit0 = t0 * NBITS;
-  if (fast) {
+  constexpr int NRAW = (RAWW + NT - 1) / NT;
+  unsigned rw[NRAW];
+  if (fast) {                                   // loads now, LDS stores after the rotator tables
     const long long w0 = bit0 >> 5;
     const unsigned* src = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
     const long long nwords = Q.ctstride >> 2;
-    for (int i = tid; i < RAWW; i += NT) raw[i] = (w0 + i < nwords) ? src[w0 + i] : 0u;
+#pragma unroll
+    for (int q = 0; q < NRAW; q++) {
+      const int i = tid + q * NT;
+      rw[q] = (i < RAWW && w0 + i < nwords) ? src[w0 + i] : 0u;
+    }
   }
 
   float2* rsub = tabs;
@@ -794,17 +835,18 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   float2* fsub = rstep + S.nfrsteps;
   float2* fstep = fsub + S.s;
   float2* rquad = fstep + S.nfracsteps;         // order 2 only
-  for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
-  for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
-  if (Sh::NMID > 0)
-    for (int i = tid; i < 256; i += NT) tw256[i] = A.tw[64 + L / 64 + i];
 
   // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
   rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
-  {
-    if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
-    else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];   // (S.lut[tid] would put S in scratch)
+  ts.store(twlo, twhi, tw256, lut8s, tid);
+  if (fast) {
+#pragma unroll
+    for (int q = 0; q < NRAW; q++) {
+      const int i = tid + q * NT;
+      if (i < RAWW) raw[i] = rw[q];
+    }
   }
+  if (NBITS != 8 && tid < 4) lut8s[tid] = A.st[B.station].lut[tid];   // (S.lut[tid] would put S in scratch)
   __syncthreads();
 
   const float4 lut2 = make_float4(S.lut[0], S.lut[1], S.lut[2], S.lut[3]);
@@ -1053,6 +1095,8 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   const int tid = threadIdx.x;
   const int cb = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
   const int blk = A.chunk0 + cb;
+  TabStage<L, NT, NBITS, (Sh::NMID > 0)> ts;
+  ts.load(A.tw, A.lut8, tid);
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
   const DevStation S = A.st[B.station];
@@ -1074,24 +1118,33 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   // down the general decode path below
   const int nerase = S.granularity > 1 ? L / 2 - (int)(t0 & 1) : -1;
   const bool erase_re = (t0 & 1) == 0;
-  if (fast) {
+  constexpr int NRAW = (RAWW + NT - 1) / NT;
+  unsigned rw[NRAW];
+  if (fast) {                                   // loads now, LDS stores after the rotator tables
     const long long w0 = bit0 >> 5;
     const unsigned* src = (const unsigned*)(A.ct + Q.ctbase + (long long)B.band * Q.ctstride);
     const long long nwords = Q.ctstride >> 2;
-    for (int i = tid; i < RAWW; i += NT) raw[i] = (w0 + i < nwords) ? src[w0 + i] : 0u;
+#pragma unroll
+    for (int q = 0; q < NRAW; q++) {
+      const int i = tid + q * NT;
+      rw[q] = (i < RAWW && w0 + i < nwords) ? src[w0 + i] : 0u;
+    }
   }
   float2* rsub = tabs;
   float2* rstep = tabs + S.s;
   float2* fsub = rstep + S.nfrsteps;
   float2* fstep = fsub + S.s;
   float2* rquad = fstep + S.nfracsteps;
-  for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
-  for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
-  if (Sh::NMID > 0)
-    for (int i = tid; i < 256; i += NT) tw256[i] = A.tw[64 + L / 64 + i];
   rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
-  if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
-  else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
+  ts.store(twlo, twhi, tw256, lut8s, tid);
+  if (fast) {
+#pragma unroll
+    for (int q = 0; q < NRAW; q++) {
+      const int i = tid + q * NT;
+      if (i < RAWW) raw[i] = rw[q];
+    }
+  }
+  if (NBITS != 8 && tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
   __syncthreads();
 
   const int s = S.s, slog = S.slog;
