@@ -1342,6 +1342,8 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   const int tid = threadIdx.x;
   const int cbl = blockIdx.x, n1 = blockIdx.z;   // block within the sub-chunk, decimation phase
   const int blk = A.chunk0 + A.zcb0 + cbl;
+  TabStage<L, NT, NBITS, false> ts;             // loads in flight while the records arrive
+  ts.load(A.tw, A.lut8, tid);
   const int gband = A.bandlist ? A.bandlist[blockIdx.y] : blockIdx.y;
   const DevBand B = A.bands[gband];
   const DevStation S = A.st[B.station];
@@ -1353,15 +1355,25 @@ __global__ __launch_bounds__(FftShape<M>::NT) void k_bigfft_a(FStageArgs A) {
   float2* rsub = tabs;
   float2* rstep = tabs + S.s;
   float2* rquad = rstep + S.nfrsteps;
-  for (int i = tid; i < 64; i += NT) twlo[i] = A.tw[i];
-  for (int i = tid; i < L / 64; i += NT) twhi[i] = A.tw[64 + i];
   {
+    // the rotator table entries, all loads in flight before the LDS stores
+    constexpr int NROT = (fxg_rotator_table_entries(L) + NT - 1) / NT;
     const float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn;
     const int nrot = S.order == 0 ? 0 : (S.order == 2 ? 2 * S.s : S.s) + S.nfrsteps;
-    for (int i = tid; i < nrot; i += NT) tabs[i] = rt[i];
+    float2 rr[NROT];
+#pragma unroll
+    for (int q = 0; q < NROT; q++) {
+      const int i = tid + q * NT;
+      if (i < nrot) rr[q] = rt[i];
+    }
+    ts.store(twlo, twhi, nullptr, lut8s, tid);
+#pragma unroll
+    for (int q = 0; q < NROT; q++) {
+      const int i = tid + q * NT;
+      if (i < nrot) tabs[i] = rr[q];
+    }
   }
-  if (NBITS == 8) for (int k = tid; k < 256; k += NT) lut8s[k] = A.lut8[k];
-  else if (tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
+  if (NBITS != 8 && tid < 4) lut8s[tid] = A.st[B.station].lut[tid];
   __syncthreads();
 
   const float4 lut2 = make_float4(S.lut[0], S.lut[1], S.lut[2], S.lut[3]);
