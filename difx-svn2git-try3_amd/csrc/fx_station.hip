@@ -1544,16 +1544,30 @@ __global__ __launch_bounds__(256) void k_bigfft_b(FStageArgs A) {
   }
   float2* fsub = tabs;
   float2* fstep = tabs + S.s;
-  {
-    const float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn + 2 * S.s + S.nfrsteps;
-    for (int i = tid; i < S.s + S.nfracsteps; i += 256) tabs[i] = rt[i];
-  }
-  __syncthreads();
-  const int s = S.s, slog = S.slog;
+  // the sub-FFT outputs and the fractional-sample tables: every load in
+  // flight before the first LDS store and the barrier
   const float2* z = A.zbuf + ((long long)gband * A.zcap + cbl) * 16 * M + k2;
   float2 v[16];
 #pragma unroll
   for (int n1 = 0; n1 < 16; n1++) v[n1] = z[(long long)n1 * M];
+  {
+    constexpr int NFR = (fxg_rotator_table_entries(L) + 255) / 256;
+    const float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn + 2 * S.s + S.nfrsteps;
+    const int nfr = S.s + S.nfracsteps;
+    float2 fr[NFR];
+#pragma unroll
+    for (int q = 0; q < NFR; q++) {
+      const int i = tid + q * 256;
+      if (i < nfr) fr[q] = rt[i];
+    }
+#pragma unroll
+    for (int q = 0; q < NFR; q++) {
+      const int i = tid + q * 256;
+      if (i < nfr) tabs[i] = fr[q];
+    }
+  }
+  __syncthreads();
+  const int s = S.s, slog = S.slog;
   DFT<16>::run(v);
   const int mh = B.lsb ? 8 : 0;          // USB keeps k1 < 8, real LSB k1 >= 8
   const float scale = (A.prec == FXG_XMAC_FP32 || A.prec == FXG_XMAC_FP8) ? 1.0f : A.scale;
