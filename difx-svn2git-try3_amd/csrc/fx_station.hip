@@ -582,11 +582,12 @@ __device__ __forceinline__ float2 angle_rotator(float x) {
   return turn_rotator((float)t);
 }
 
+// C: the recorded channels, L/2 (real) or L (complex) -- a compile-time
+// power of two at every call, so the divisions by it are exact products
 __device__ __forceinline__ void rotator_tables(const DevBand& B, const DevStation& S, const DevBlock& G, int L,
-                                               int tid, int NT, bool rot, bool frac, float2* rsub,
+                                               int C, int tid, int NT, bool rot, bool frac, float2* rsub,
                                                float2* rstep, float2* fsub, float2* fstep, float2* rquad) {
 #pragma clang fp contract(off)
-  const int C = S.C;                 // recorded channels: L/2 (real) or L (complex)
   const int s = S.s;
   const double lofreq = B.lofreq, looffset = B.looffset;
   const bool looff = looffset > 0.0 || looffset < 0.0;
@@ -837,7 +838,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage(FStageArgs A) {
   float2* rquad = fstep + S.nfracsteps;         // order 2 only
 
   // ---- rotator tables: FP64 phase, FP32 residual (mode.cpp:909-978, 1054-1097)
-  rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
+  rotator_tables(B, S, G, L, L / 2, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
   ts.store(twlo, twhi, tw256, lut8s, tid);
   if (fast) {
 #pragma unroll
@@ -1135,7 +1136,7 @@ __global__ __launch_bounds__(FftShape<L>::NT) void k_fstage_c(FStageArgs A) {
   float2* fsub = rstep + S.nfrsteps;
   float2* fstep = fsub + S.s;
   float2* rquad = fstep + S.nfracsteps;
-  rotator_tables(B, S, G, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
+  rotator_tables(B, S, G, L, L, tid, NT, true, true, rsub, rstep, fsub, fstep, rquad);
   ts.store(twlo, twhi, tw256, lut8s, tid);
   if (fast) {
 #pragma unroll
@@ -1324,7 +1325,7 @@ __global__ __launch_bounds__(256) void k_rotab(FStageArgs A) {
   const DevBlock G = A.blocks[(long long)B.station * A.nblocks_subint + blk];
   if (G.status != 0) return;
   float2* rt = A.rtab + ((long long)gband * A.zcap + cbl) * A.rtabn;
-  rotator_tables(B, S, G, L, threadIdx.x, 256, true, true, rt, rt + S.s, rt + 2 * S.s + S.nfrsteps,
+  rotator_tables(B, S, G, L, L / 2, threadIdx.x, 256, true, true, rt, rt + S.s, rt + 2 * S.s + S.nfrsteps,
                  rt + 3 * S.s + S.nfrsteps, rt + S.s + S.nfrsteps);
 }
 
