@@ -606,6 +606,11 @@ int fxg_job_create(fxg_ctx* c, const fxg_job_desc* desc, fxg_job** out) {
   j->desc.baselines = j->bls.empty() ? nullptr : j->bls.data();
   auto fail = [&](int code) { job_free(j); delete j; return code; };
   if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(FXG_EHIP, "stream"));
+  // the host-input copy stream, created right after the job's own so that
+  // the runtime's round of hardware queues puts the two on different queues
+  // (a copy queued behind the job's kernels could not overlap them)
+  if (hipStreamCreateWithFlags(&j->copy_stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(FXG_EHIP, "copy stream"));
 
   const int prec = desc->xmac_precision;
   // spectrum scale for reduced precision: 2^-round(log2(sqrt(L)))
@@ -916,7 +921,6 @@ static int upload_subint(fxg_job* j, const fxg_subint_desc* sub, const fxg_stati
     *raw_set = -1;
     if (!data_on_device) {
       set = j->rawp_next;
-      if (!j->copy_stream) HIPCHK(hipStreamCreateWithFlags(&j->copy_stream, hipStreamNonBlocking));
       for (int k = 0; k < 2; k++) {
         if (!j->raw_free[k]) HIPCHK(hipEventCreateWithFlags(&j->raw_free[k], hipEventDisableTiming));
         if (!j->raw_copied[k]) HIPCHK(hipEventCreateWithFlags(&j->raw_copied[k], hipEventDisableTiming));
