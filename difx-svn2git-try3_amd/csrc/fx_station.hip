@@ -468,9 +468,23 @@ __global__ __launch_bounds__(256) void k_cornerturn(const DevStation* __restrict
   if (f >= Q.nframes) return;
   const unsigned char* src = Q.data + f * S.framebytes + S.headerbytes;
   if ((((unsigned long long)Q.data | (unsigned long long)S.framebytes | (unsigned long long)S.headerbytes) & 3) == 0) {
+    // eight words per thread in flight before their LDS stores
     const unsigned* s32 = (const unsigned*)src;
     unsigned* p32 = (unsigned*)pay;
-    for (int i = threadIdx.x; i < S.databytes / 4; i += blockDim.x) p32[i] = s32[i];
+    const int nw = S.databytes / 4;
+    for (int i0 = threadIdx.x; i0 < nw; i0 += 8 * blockDim.x) {
+      unsigned w[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int i = i0 + q * blockDim.x;
+        if (i < nw) w[q] = s32[i];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int i = i0 + q * blockDim.x;
+        if (i < nw) p32[i] = w[q];
+      }
+    }
   } else {
     for (int i = threadIdx.x; i < S.databytes; i += blockDim.x) pay[i] = src[i];
   }
